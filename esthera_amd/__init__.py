@@ -1,0 +1,417 @@
+"""esthera_amd - MI355X (gfx950) particle-filter engine.
+
+The engine is the C-ABI shared library esthera_amd/libesthera.so (hand-written
+HIP kernels, include/esthera.h).  This package is its Python host mirror: a
+ctypes binding plus thin classes that read like the reference's driver
+(src/partfilt/opencl/pf_generic.c: particle_filter(m, G, t, thr, file)).
+
+There is no fallback: if the library is missing or no GPU is present, the
+calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libesthera.so")
+
+# --------------------------------------------------------------- constants
+ESTH_OK = 0
+MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6 = 0, 1, 2
+TOPO_NONE, TOPO_RING, TOPO_TORUS, TOPO_ALL2ALL = 0, 1, 2, 3
+RESAMPLE_MULTINOMIAL, RESAMPLE_SYSTEMATIC, RESAMPLE_ALIAS = 0, 1, 2
+WEIGHTS_LINEAR, WEIGHTS_LOG = 0, 1
+RNG_MTGP_COMPAT, RNG_MTGP_PER_FILTER = 0, 1
+FRAME_FLOATS = 336
+MAX_LANDMARKS = 64
+STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12}
+NOISE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 6}
+
+
+# ------------------------------------------------------------- structures
+class FilterDesc(C.Structure):
+    _fields_ = [("model", C.c_int), ("num_particles", C.c_uint32), ("num_blocks", C.c_uint32),
+                ("num_transfer", C.c_uint32), ("topology", C.c_int), ("torus_dimx", C.c_int),
+                ("torus_dimy", C.c_int), ("resampler", C.c_int), ("resampling_threshold", C.c_float),
+                ("adaptive", C.c_int), ("weights", C.c_int), ("rng", C.c_int), ("seed", C.c_uint32)]
+
+
+class Estimate(C.Structure):
+    _fields_ = [("state_dim", C.c_int), ("mean", C.c_double * 16), ("best", C.c_float * 16),
+                ("best_weight", C.c_float), ("best_block", C.c_int), ("log_weight_max", C.c_double),
+                ("weight_sum", C.c_double), ("step", C.c_uint64)]
+
+
+class RobotArm5Control(C.Structure):
+    _fields_ = [("angles", C.c_float * 5)]
+
+
+class RobotArm5Measurement(C.Structure):
+    _fields_ = [("angles", C.c_float * 5), ("x", C.c_float), ("y", C.c_float)]
+
+
+class UnicycleParams(C.Structure):
+    _fields_ = [("num_landmarks", C.c_int), ("sigma_x", C.c_float), ("sigma_y", C.c_float),
+                ("sigma_theta", C.c_float), ("sigma_range", C.c_float), ("sigma_bearing", C.c_float),
+                ("lx", C.c_float * MAX_LANDMARKS), ("ly", C.c_float * MAX_LANDMARKS)]
+
+
+class UnicycleControl(C.Structure):
+    _fields_ = [("v", C.c_float), ("omega", C.c_float)]
+
+
+class UnicycleMeasurement(C.Structure):
+    _fields_ = [("range", C.c_float * MAX_LANDMARKS), ("bearing", C.c_float * MAX_LANDMARKS)]
+
+
+class DH6Params(C.Structure):
+    _fields_ = [("a", C.c_float * 6), ("d", C.c_float * 6), ("alpha", C.c_float * 6),
+                ("sigma_acc", C.c_float), ("sigma_pos", C.c_float), ("sigma_enc", C.c_float)]
+
+
+class DH6Control(C.Structure):
+    _fields_ = [("unused", C.c_float)]
+
+
+class DH6Measurement(C.Structure):
+    _fields_ = [("p", C.c_float * 3), ("q", C.c_float * 6)]
+
+
+class Result(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("track_id", C.c_uint64), ("status", C.c_int), ("kind", C.c_int),
+                ("submit_us", C.c_double), ("done_us", C.c_double), ("estimate", Estimate)]
+
+
+PARAMS_TYPE = {MODEL_ROBOTARM5: None, MODEL_UNICYCLE: UnicycleParams, MODEL_DH6: DH6Params}
+CONTROL_TYPE = {MODEL_ROBOTARM5: RobotArm5Control, MODEL_UNICYCLE: UnicycleControl, MODEL_DH6: DH6Control}
+MEAS_TYPE = {MODEL_ROBOTARM5: RobotArm5Measurement, MODEL_UNICYCLE: UnicycleMeasurement,
+             MODEL_DH6: DH6Measurement}
+
+
+class EsthError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"esthera error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+_P = C.c_void_p
+_SIGS = {
+    "esth_last_error": ([], C.c_char_p),
+    "esth_version": ([], C.c_int),
+    "esth_model_lookup": ([C.c_char_p], C.c_int),
+    "esth_model_info": ([C.c_int, _P, _P, _P, _P, _P], C.c_int),
+    "esth_model_build_frame": ([C.c_int, _P, _P, _P, C.c_float, _P], C.c_int),
+    "esth_filter_desc_default": ([_P, C.c_int], None),
+    "esth_engine_create": ([C.c_int, C.c_size_t, _P], C.c_int),
+    "esth_engine_destroy": ([_P], C.c_int),
+    "esth_engine_memory": ([_P, _P, _P], C.c_int),
+    "esth_filter_create": ([_P, _P, _P, C.c_size_t, _P], C.c_int),
+    "esth_filter_destroy": ([_P], C.c_int),
+    "esth_filter_init_uniform": ([_P, _P], C.c_int),
+    "esth_filter_init": ([_P, _P], C.c_int),
+    "esth_filter_step": ([_P, _P, _P, C.c_float], C.c_int),
+    "esth_filter_sync": ([_P], C.c_int),
+    "esth_filter_estimate": ([_P, _P], C.c_int),
+    "esth_filter_read_particles": ([_P, _P, _P], C.c_int),
+    "esth_filter_set_timing": ([_P, C.c_int], C.c_int),
+    "esth_filter_stage_times": ([_P, _P], C.c_int),
+    "esth_filter_snapshot_size": ([_P, _P], C.c_int),
+    "esth_filter_snapshot": ([_P, _P, C.c_size_t], C.c_int),
+    "esth_filter_restore": ([_P, _P, C.c_size_t], C.c_int),
+    "esth_filter_stream": ([_P, _P], C.c_int),
+    "esth_filter_use_graph": ([_P, C.c_int], C.c_int),
+    "esth_particle_filter": ([C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p, C.c_uint32, C.c_int,
+                              _P, _P], C.c_int),
+    "esth_trace_load_robotarm": ([C.c_char_p, C.c_int, _P, _P, _P, _P, _P], C.c_int),
+    "esth_robotarm5_estimate_error": ([_P, _P], C.c_float),
+    "esth_unicycle_scenario": ([C.c_uint32, C.c_int, C.c_float, C.c_int, C.c_float, _P, _P, _P, _P], C.c_int),
+    "esth_dh6_scenario": ([C.c_uint32, C.c_int, C.c_float, _P, _P, _P], C.c_int),
+    "esth_batch_create": ([_P, C.c_int, _P, _P, C.c_size_t, _P], C.c_int),
+    "esth_batch_destroy": ([_P], C.c_int),
+    "esth_batch_init_uniform": ([_P, _P], C.c_int),
+    "esth_batch_step": ([_P, _P, C.c_size_t, _P, C.c_size_t, _P], C.c_int),
+    "esth_batch_sync": ([_P], C.c_int),
+    "esth_batch_estimate": ([_P, C.c_int, _P], C.c_int),
+    "esth_batch_read_particles": ([_P, C.c_int, _P, _P], C.c_int),
+    "esth_batch_use_graph": ([_P, C.c_int], C.c_int),
+    "esth_batch_step_time": ([_P, _P], C.c_int),
+    "esth_server_create": ([_P, C.c_int, _P], C.c_int),
+    "esth_server_destroy": ([_P], C.c_int),
+    "esth_server_open": ([_P, _P, _P, C.c_size_t, _P, _P, _P], C.c_int),
+    "esth_server_update": ([_P, C.c_uint64, _P, _P, C.c_float, _P], C.c_int),
+    "esth_server_close": ([_P, C.c_uint64, _P], C.c_int),
+    "esth_server_pump": ([_P], C.c_int),
+    "esth_server_poll": ([_P, _P, C.c_int, _P], C.c_int),
+    "esth_server_drain": ([_P], C.c_int),
+    "esth_server_stats": ([_P, _P, _P, _P], C.c_int),
+    "esth_set_device": ([C.c_int], C.c_int),
+    "esth_dev_alloc": ([C.c_size_t, _P], C.c_int),
+    "esth_dev_free": ([_P], C.c_int),
+    "esth_memcpy_htod": ([_P, _P, C.c_size_t], C.c_int),
+    "esth_memcpy_dtoh": ([_P, _P, C.c_size_t], C.c_int),
+    "esth_memset_d": ([_P, C.c_int, C.c_size_t], C.c_int),
+    "esth_dev_sync": ([], C.c_int),
+    "esth_mtgp_params": ([], C.POINTER(C.c_uint32)),
+    "esth_mtgp_seed_state": ([C.c_int, C.c_uint32, _P], C.c_int),
+    "esth_stage_mtgp": ([_P, C.c_int, C.c_int, C.c_int, _P, C.c_int], C.c_int),
+    "esth_stage_boxmuller": ([_P, C.c_int], C.c_int),
+    "esth_stage_sample_weight": ([C.c_int, _P, C.c_uint32, _P, _P, C.c_int, _P], C.c_int),
+    "esth_stage_sort": ([_P, _P, _P, C.c_int, C.c_uint32, C.c_uint32, _P, _P], C.c_int),
+    "esth_stage_argmax": ([_P, C.c_uint32, C.c_int, _P], C.c_int),
+    "esth_stage_exchange": ([_P, _P, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int],
+                            C.c_int),
+    "esth_stage_resample": ([C.c_int, _P, _P, C.c_int, _P, _P, C.c_uint32, C.c_uint32, C.c_float, C.c_int,
+                             C.c_int, _P], C.c_int),
+    "esth_stage_alias_table": ([_P, C.c_uint32, C.c_uint32, C.c_int, _P, _P], C.c_int),
+    "esth_stage_math": ([C.c_int, _P, _P, _P, C.c_int], C.c_int),
+}
+
+
+def lib():
+    """The loaded engine library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"engine library not built: {LIB_PATH} (python -m esthera_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != ESTH_OK:
+        raise EsthError(rc, lib().esth_last_error().decode())
+    return rc
+
+
+def ptr(x):
+    """void* of a ctypes object, numpy array or DeviceBuffer (None -> NULL)."""
+    if x is None:
+        return None
+    if isinstance(x, DeviceBuffer):
+        return x.ptr
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data_as(C.c_void_p)
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    return C.cast(C.pointer(x), C.c_void_p)
+
+
+# ------------------------------------------------------------- device memory
+class DeviceBuffer:
+    """A device allocation made through the engine's C ABI (hipMalloc)."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        check(lib().esth_dev_alloc(nbytes, C.byref(p)))
+        self.ptr = p
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(max(a.nbytes, 4))
+        if a.nbytes:
+            check(lib().esth_memcpy_htod(b.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return b
+
+    def to_array(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().esth_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        check(lib().esth_memcpy_htod(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().esth_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync():
+    check(lib().esth_dev_sync())
+
+
+# ------------------------------------------------------------------ engine
+class Engine:
+    """One engine per GPU; owns the HBM budget."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = C.c_void_p()
+        check(lib().esth_engine_create(device, hbm_budget, C.byref(h)))
+        self.h = h
+
+    def memory(self):
+        used, budget = C.c_size_t(), C.c_size_t()
+        check(lib().esth_engine_memory(self.h, C.byref(used), C.byref(budget)))
+        return used.value, budget.value
+
+    def close(self):
+        if self.h:
+            lib().esth_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def filter_desc(model, m=256, G=16, t=1, topology=TOPO_RING, resampler=RESAMPLE_MULTINOMIAL, threshold=1.0,
+                adaptive=0, weights=WEIGHTS_LINEAR, rng=RNG_MTGP_COMPAT, seed=1234, dimx=0, dimy=0):
+    d = FilterDesc()
+    d.model, d.num_particles, d.num_blocks, d.num_transfer = model, m, G, t
+    d.topology, d.torus_dimx, d.torus_dimy = topology, dimx, dimy
+    d.resampler, d.resampling_threshold, d.adaptive = resampler, threshold, adaptive
+    d.weights, d.rng, d.seed = weights, rng, seed
+    return d
+
+
+class Filter:
+    """An island-model filter: m particles per sub-filter, G sub-filters.
+
+    Mirrors particle_filter() of pf_generic.c:245-887: create, set the
+    initial particles, then step(control, measurement, dt) per trace row."""
+
+    def __init__(self, engine, desc, params=None):
+        self.engine = engine
+        self.desc = desc
+        self.S = STATE_DIM[desc.model]
+        self.N = desc.num_particles * desc.num_blocks
+        self._params = params
+        h = C.c_void_p()
+        psize = C.sizeof(params) if params is not None else 0
+        check(lib().esth_filter_create(engine.h, ptr(desc), ptr(params), psize, C.byref(h)))
+        self.h = h
+
+    def init_uniform(self, state):
+        s = np.ascontiguousarray(state, dtype=np.float32)
+        check(lib().esth_filter_init_uniform(self.h, ptr(s)))
+
+    def init(self, soa):
+        s = np.ascontiguousarray(soa, dtype=np.float32)
+        assert s.size == self.S * self.N
+        check(lib().esth_filter_init(self.h, ptr(s)))
+
+    def step(self, control, measurement, dt):
+        check(lib().esth_filter_step(self.h, ptr(control), ptr(measurement), dt))
+
+    def sync(self):
+        check(lib().esth_filter_sync(self.h))
+
+    def estimate(self):
+        e = Estimate()
+        check(lib().esth_filter_estimate(self.h, C.byref(e)))
+        return e
+
+    def particles(self):
+        st = np.empty(self.S * self.N, np.float32)
+        w = np.empty(self.N, np.float32)
+        check(lib().esth_filter_read_particles(self.h, ptr(st), ptr(w)))
+        return st, w
+
+    def set_timing(self, on=True):
+        check(lib().esth_filter_set_timing(self.h, 1 if on else 0))
+
+    def stage_times(self):
+        us = np.zeros(6, np.float32)
+        check(lib().esth_filter_stage_times(self.h, ptr(us)))
+        return us
+
+    def use_graph(self, on=True):
+        check(lib().esth_filter_use_graph(self.h, 1 if on else 0))
+
+    def snapshot(self):
+        n = C.c_size_t()
+        check(lib().esth_filter_snapshot_size(self.h, C.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        check(lib().esth_filter_snapshot(self.h, ptr(buf), n.value))
+        return buf
+
+    def restore(self, buf):
+        check(lib().esth_filter_restore(self.h, ptr(buf), buf.nbytes))
+
+    def close(self):
+        if self.h:
+            lib().esth_filter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def particle_filter(m, G, t, threshold, trace_path, seed=1234, resampler=RESAMPLE_MULTINOMIAL):
+    """particle_filter(m, G, t, thr, file) of pf_generic.c:245 for the robot
+    arm: returns (mean estimate error over steps, steps)."""
+    err = C.c_double()
+    steps = C.c_int()
+    check(lib().esth_particle_filter(m, G, t, threshold, trace_path.encode(), seed, resampler, C.byref(err),
+                                     C.byref(steps)))
+    return err.value, steps.value
+
+
+def unicycle_scenario(seed, num_landmarks, box, steps, dt=0.1):
+    p = UnicycleParams()
+    ctrl = (UnicycleControl * steps)()
+    meas = (UnicycleMeasurement * steps)()
+    truth = np.zeros(steps * 3, np.float32)
+    check(lib().esth_unicycle_scenario(seed, num_landmarks, box, steps, dt, ptr(p), ctrl, meas, ptr(truth)))
+    return p, ctrl, meas, truth.reshape(steps, 3)
+
+
+def dh6_scenario(seed, steps, dt=0.01):
+    p = DH6Params()
+    meas = (DH6Measurement * steps)()
+    truth = np.zeros((steps + 1) * 12, np.float32)
+    check(lib().esth_dh6_scenario(seed, steps, dt, ptr(p), meas, ptr(truth)))
+    return p, meas, truth.reshape(steps + 1, 12)
+
+
+def load_robotarm_trace(path, max_rows=100000):
+    meas = (RobotArm5Measurement * max_rows)()
+    ctrl = (RobotArm5Control * max_rows)()
+    dt = np.zeros(max_rows, np.float32)
+    truth = np.zeros(max_rows * 9, np.float32)
+    rows = C.c_int()
+    check(lib().esth_trace_load_robotarm(path.encode(), max_rows, meas, ctrl, ptr(dt), ptr(truth), C.byref(rows)))
+    n = rows.value
+    return meas[:n], ctrl[:n], dt[:n], truth[: n * 9].reshape(n, 9)
+
+
+def build_frame(model, params, control, measurement, dt):
+    fr = np.zeros(FRAME_FLOATS, np.float32)
+    check(lib().esth_model_build_frame(model, ptr(params), ptr(control), ptr(measurement), dt, ptr(fr)))
+    return fr
+
+
+def mtgp_params():
+    p = lib().esth_mtgp_params()
+    return np.ctypeslib.as_array(p, shape=(200 * 52,)).reshape(200, 52).copy()
+
+
+def mtgp_seed_state(set_idx, seed):
+    st = np.zeros(351, np.uint32)
+    check(lib().esth_mtgp_seed_state(set_idx, seed & 0xFFFFFFFF, ptr(st)))
+    return st

@@ -1,0 +1,494 @@
+/*
+ * esth_engine.hip - host side of the C ABI: engine, kernel launchers, stage
+ * entry points, single filters, the reference-compatible driver, trace I/O
+ * and synthetic scenarios.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "esth_internal.h"
+#include "esth_kernels.h"
+
+/* ================================================================ errors */
+static thread_local std::string g_last_error;
+
+int esth_fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+extern "C" const char* esth_last_error(void) { return g_last_error.c_str(); }
+extern "C" int esth_version(void) { return ESTH_VERSION; }
+
+/* ================================================== MTGP parameter data
+ * The 200 MTGP32-11213 parameter sets (BSD-licensed data of the MTGP
+ * authors, esthera_amd/data/mtgp32dc_11213.txt), compiled in by the build. */
+static const uint32_t g_mtgp_table[ESTH_MTGP_SETS * ESTH_MTGP_PARAM_WORDS] = {
+#include "mtgp32dc_11213.inc"
+};
+
+const uint32_t* esth_param_table() { return g_mtgp_table; }
+extern "C" const uint32_t* esth_mtgp_params(void) { return g_mtgp_table; }
+
+/* mtgp32_init_state, mtgp32-fast.c:119-137 */
+extern "C" int esth_mtgp_seed_state(int set, uint32_t seed, uint32_t* st) {
+    if (set < 0 || set >= ESTH_MTGP_SETS || !st) return esth_fail(ESTH_ERR_INVALID, "bad MTGP set %d", set);
+    const uint32_t* tbl = g_mtgp_table + (size_t)set * ESTH_MTGP_PARAM_WORDS + 4;
+    uint32_t hidden = tbl[4] ^ (tbl[8] << 16);
+    uint32_t tmp = hidden;
+    tmp += tmp >> 16;
+    tmp += tmp >> 8;
+    uint32_t fill = (tmp & 0xffu) * 0x01010101u;
+    for (int i = 0; i < ESTH_MTGP_N; ++i) st[i] = fill;
+    st[0] = seed;
+    st[1] = hidden;
+    for (int i = 1; i < ESTH_MTGP_N; ++i) st[i] ^= 1812433253u * (st[i - 1] ^ (st[i - 1] >> 30)) + (uint32_t)i;
+    return ESTH_OK;
+}
+
+/* ================================================================ models */
+extern "C" int esth_model_lookup(const char* name) {
+    if (!name) return -1;
+    if (!strcmp(name, "robotarm5") || !strcmp(name, "robotarm")) return ESTH_MODEL_ROBOTARM5;
+    if (!strcmp(name, "unicycle")) return ESTH_MODEL_UNICYCLE;
+    if (!strcmp(name, "dh6")) return ESTH_MODEL_DH6;
+    return -1;
+}
+
+int esth_params_size(int model, size_t* sz) {
+    switch (model) {
+    case ESTH_MODEL_ROBOTARM5: *sz = 0; return ESTH_OK;
+    case ESTH_MODEL_UNICYCLE: *sz = sizeof(esth_unicycle_params); return ESTH_OK;
+    case ESTH_MODEL_DH6: *sz = sizeof(esth_dh6_params); return ESTH_OK;
+    default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
+    }
+}
+
+extern "C" int esth_model_info(int model, int* S, int* V, size_t* ps, size_t* cs, size_t* ms) {
+    int s, v;
+    if (esth_model_dims(model, &s, &v)) return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
+    if (S) *S = s;
+    if (V) *V = v;
+    size_t p = 0;
+    esth_params_size(model, &p);
+    if (ps) *ps = p;
+    if (cs) *cs = model == ESTH_MODEL_ROBOTARM5 ? sizeof(esth_robotarm5_control)
+                  : model == ESTH_MODEL_UNICYCLE ? sizeof(esth_unicycle_control)
+                                                 : sizeof(esth_dh6_control);
+    if (ms) *ms = model == ESTH_MODEL_ROBOTARM5 ? sizeof(esth_robotarm5_measurement)
+                  : model == ESTH_MODEL_UNICYCLE ? sizeof(esth_unicycle_measurement)
+                                                 : sizeof(esth_dh6_measurement);
+    return ESTH_OK;
+}
+
+extern "C" int esth_model_build_frame(int model, const void* params, const void* control,
+                                      const void* measurement, float dt, float* frame) {
+    if (!frame) return esth_fail(ESTH_ERR_INVALID, "null frame");
+    if (esth_build_frame_impl(model, params, control, measurement, dt, frame))
+        return esth_fail(ESTH_ERR_INVALID, "model %d: bad params/control/measurement", model);
+    return ESTH_OK;
+}
+
+static bool is_pow2(uint32_t x) { return x && !(x & (x - 1)); }
+
+int esth_check_desc(const esth_filter_desc* d, int* S, int* V) {
+    if (!d) return esth_fail(ESTH_ERR_INVALID, "null descriptor");
+    if (esth_model_dims(d->model, S, V)) return esth_fail(ESTH_ERR_INVALID, "unknown model %d", d->model);
+    const uint32_t m = d->num_particles, G = d->num_blocks, t = d->num_transfer;
+    if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES)
+        return esth_fail(ESTH_ERR_INVALID, "num_particles %u: need a power of two in [2, %d]", m, ESTH_MAX_NUM_PARTICLES);
+    if (G < 1 || G > ESTH_MAX_NUM_BLOCKS)
+        return esth_fail(ESTH_ERR_INVALID, "num_blocks %u exceeds maximum %d", G, ESTH_MAX_NUM_BLOCKS);
+    if (t > ESTH_MAX_NUM_TRANSFER) return esth_fail(ESTH_ERR_INVALID, "num_transfer %u exceeds maximum", t);
+    if (d->topology < ESTH_TOPO_NONE || d->topology > ESTH_TOPO_ALL2ALL)
+        return esth_fail(ESTH_ERR_INVALID, "bad topology %d", d->topology);
+    if (t > 0 && d->topology != ESTH_TOPO_NONE) {
+        if (d->topology == ESTH_TOPO_TORUS) {
+            if (4ull * t > m) return esth_fail(ESTH_ERR_INVALID, "torus exchange needs 4t <= m (t=%u, m=%u)", t, m);
+            int dx = d->torus_dimx, dy = d->torus_dimy;
+            if (dx <= 0 || dy <= 0) esth_torus_dims(G, &dx, &dy);
+            if ((uint64_t)dx * (uint64_t)dy != G)
+                return esth_fail(ESTH_ERR_INVALID, "torus %dx%d does not cover %u sub-filters", dx, dy, G);
+        } else if (2ull * t > m) {
+            return esth_fail(ESTH_ERR_INVALID, "exchange needs 2t <= m (t=%u, m=%u)", t, m);
+        }
+        if (d->topology == ESTH_TOPO_ALL2ALL && t > 64)
+            return esth_fail(ESTH_ERR_UNSUPPORTED, "all-to-all exchange supports t <= 64");
+    }
+    if (d->resampler < ESTH_RESAMPLE_MULTINOMIAL || d->resampler > ESTH_RESAMPLE_ALIAS)
+        return esth_fail(ESTH_ERR_INVALID, "bad resampler %d", d->resampler);
+    if (d->adaptive && d->resampler != ESTH_RESAMPLE_ALIAS)
+        return esth_fail(ESTH_ERR_UNSUPPORTED, "adaptive resampling is defined for the alias resampler");
+    if (!(d->resampling_threshold >= 0.0f) || d->resampling_threshold > 1.0f)
+        return esth_fail(ESTH_ERR_INVALID, "resampling threshold %f outside [0, 1]", (double)d->resampling_threshold);
+    if (d->weights != ESTH_WEIGHTS_LINEAR && d->weights != ESTH_WEIGHTS_LOG)
+        return esth_fail(ESTH_ERR_INVALID, "bad weights mode %d", d->weights);
+    if (d->rng != ESTH_RNG_MTGP_COMPAT && d->rng != ESTH_RNG_MTGP_PER_FILTER)
+        return esth_fail(ESTH_ERR_INVALID, "bad rng mode %d", d->rng);
+    return ESTH_OK;
+}
+
+extern "C" void esth_filter_desc_default(esth_filter_desc* d, int model) {
+    if (!d) return;
+    memset(d, 0, sizeof(*d));
+    d->model = model;
+    d->num_particles = 256;
+    d->num_blocks = 16;
+    d->num_transfer = 1;
+    d->topology = ESTH_TOPO_RING;
+    d->resampler = ESTH_RESAMPLE_MULTINOMIAL;
+    d->resampling_threshold = 1.0f;
+    d->weights = ESTH_WEIGHTS_LINEAR;
+    d->rng = ESTH_RNG_MTGP_COMPAT;
+    d->seed = 1234;
+}
+
+/* ================================================================ engine */
+extern "C" int esth_set_device(int device) {
+    ESTH_HIP(hipSetDevice(device));
+    return ESTH_OK;
+}
+
+extern "C" int esth_engine_create(int device, size_t budget, esth_engine** out) {
+    if (!out) return esth_fail(ESTH_ERR_INVALID, "null out");
+    int n = 0;
+    ESTH_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return esth_fail(ESTH_ERR_INVALID, "device %d of %d", device, n);
+    ESTH_HIP(hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    ESTH_HIP(hipMemGetInfo(&fr, &tot));
+    esth_engine* e = new esth_engine();
+    e->device = device;
+    e->budget = budget ? budget : (size_t)(0.9 * (double)fr);
+    hipError_t err = hipMalloc((void**)&e->d_params, sizeof(g_mtgp_table));
+    if (err != hipSuccess) { delete e; return esth_fail(ESTH_ERR_DEVICE, "hipMalloc params: %s", hipGetErrorString(err)); }
+    err = hipMemcpy(e->d_params, g_mtgp_table, sizeof(g_mtgp_table), hipMemcpyHostToDevice);
+    if (err != hipSuccess) { hipFree(e->d_params); delete e; return esth_fail(ESTH_ERR_DEVICE, "upload params: %s", hipGetErrorString(err)); }
+    *out = e;
+    return ESTH_OK;
+}
+
+extern "C" int esth_engine_destroy(esth_engine* e) {
+    if (!e) return ESTH_OK;
+    hipSetDevice(e->device);
+    hipFree(e->d_params);
+    delete e;
+    return ESTH_OK;
+}
+
+extern "C" int esth_engine_memory(const esth_engine* e, size_t* used, size_t* budget) {
+    if (!e) return esth_fail(ESTH_ERR_INVALID, "null engine");
+    if (used) *used = e->used;
+    if (budget) *budget = e->budget;
+    return ESTH_OK;
+}
+
+int esth_engine_alloc(esth_engine* e, size_t bytes, void** p) {
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->used + bytes > e->budget)
+        return esth_fail(ESTH_ERR_BUDGET, "HBM budget: %zu used + %zu requested > %zu", e->used, bytes, e->budget);
+    hipError_t err = hipMalloc(p, bytes ? bytes : 4);
+    if (err != hipSuccess) return esth_fail(ESTH_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(err));
+    e->used += bytes;
+    return ESTH_OK;
+}
+
+void esth_engine_release(esth_engine* e, void* p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipFree(p);
+    e->used -= bytes;
+}
+
+/* ============================================================ launchers */
+static int grid_for(uint64_t threads, int block) { return (int)((threads + block - 1) / block); }
+
+int esth_launch_mtgp(const uint32_t* d_params, uint32_t* d_status, const esth_mtgp_stream* d_streams,
+                     int nstreams, void* d_out, int float_mode, hipStream_t s) {
+    if (nstreams <= 0) return ESTH_OK;
+    const uint32_t mask = g_mtgp_table[3];
+    hipLaunchKernelGGL(k_mtgp, dim3(nstreams), dim3(ESTH_MTGP_THREADS), 0, s, d_params, mask, d_status,
+                       d_streams, (uint32_t*)d_out, float_mode);
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame,
+                              int log_w, float* w, hipStream_t s) {
+    const int grid = grid_for(N / 2, 256);
+    switch (model) {
+    case ESTH_MODEL_ROBOTARM5:
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
+        break;
+    case ESTH_MODEL_UNICYCLE:
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
+        break;
+    case ESTH_MODEL_DH6:
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_DH6>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
+        break;
+    default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
+    }
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+static int log2u(uint32_t m) { int k = 0; while ((1u << k) < m) ++k; return k; }
+
+template <int LOGM, int S>
+static void launch_sort_t(float* w, const float* st_in, float* st_out, uint32_t N, uint32_t G, float2* winner,
+                          int* perm, uint32_t t, float* bt_w, float* bt_s, float* part, int log_w, hipStream_t s) {
+    using TL = esth_tile<LOGM>;
+    const uint64_t waves = (G + (ESTH_WAVE / TL::LPS) - 1) / (ESTH_WAVE / TL::LPS);
+    hipLaunchKernelGGL((k_sort<LOGM, S>), dim3(grid_for(waves * ESTH_WAVE, 256)), dim3(256), 0, s, w, st_in,
+                       st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w);
+}
+
+template <int LOGM, int S>
+static void launch_resample_t(float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
+                              uint32_t G, float thr, int systematic, int log_w, int* sel, hipStream_t s) {
+    using TL = esth_tile<LOGM>;
+    const uint64_t waves = (G + (ESTH_WAVE / TL::LPS) - 1) / (ESTH_WAVE / TL::LPS);
+    hipLaunchKernelGGL((k_resample<LOGM, S>), dim3(grid_for(waves * ESTH_WAVE, 256)), dim3(256), 0, s, st_main,
+                       st_tmp, w, rnd, N, G, thr, systematic, log_w, sel);
+}
+
+#define ESTH_DISPATCH_LOGM(FN, S, ...)                 \
+    switch (lm) {                                      \
+    case 1: FN<1, S>(__VA_ARGS__); break;              \
+    case 2: FN<2, S>(__VA_ARGS__); break;              \
+    case 3: FN<3, S>(__VA_ARGS__); break;              \
+    case 4: FN<4, S>(__VA_ARGS__); break;              \
+    case 5: FN<5, S>(__VA_ARGS__); break;              \
+    case 6: FN<6, S>(__VA_ARGS__); break;              \
+    case 7: FN<7, S>(__VA_ARGS__); break;              \
+    case 8: FN<8, S>(__VA_ARGS__); break;              \
+    case 9: FN<9, S>(__VA_ARGS__); break;              \
+    case 10: FN<10, S>(__VA_ARGS__); break;            \
+    default: return esth_fail(ESTH_ERR_INVALID, "m"); \
+    }
+
+int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_out, uint32_t N, uint32_t G,
+                     float2* winner, int* perm, uint32_t t, float* bt_w, float* bt_s, float* part, int log_w,
+                     hipStream_t s) {
+    const int lm = log2u(m);
+    switch (S) {
+    case 1: ESTH_DISPATCH_LOGM(launch_sort_t, 1, w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s); break;
+    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_UNI_S, w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s); break;
+    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_RA5_S, w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s); break;
+    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_DH6_S, w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s); break;
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+    }
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+int esth_launch_argmax(int S, const float2* winner, uint32_t G, int log_w, const float* part, const float* sorted,
+                       uint32_t N, uint32_t m, float2* result, double* est, hipStream_t s) {
+    switch (S) {
+    case 1: hipLaunchKernelGGL(k_argmax<1>, dim3(1), dim3(128), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
+    case ESTH_UNI_S: hipLaunchKernelGGL(k_argmax<ESTH_UNI_S>, dim3(1), dim3(128), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
+    case ESTH_RA5_S: hipLaunchKernelGGL(k_argmax<ESTH_RA5_S>, dim3(1), dim3(128), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
+    case ESTH_DH6_S: hipLaunchKernelGGL(k_argmax<ESTH_DH6_S>, dim3(1), dim3(128), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+    }
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+int esth_launch_exchange(float* st, float* w, int S, uint32_t N, uint32_t m, uint32_t G, uint32_t t, int topo,
+                         int dimx, int dimy, const float* bt_w, const float* bt_s, uint32_t* top, hipStream_t s) {
+    if (t == 0 || topo == ESTH_TOPO_NONE) return ESTH_OK;
+    if (topo == ESTH_TOPO_ALL2ALL) {
+        hipLaunchKernelGGL(k_topt, dim3(1), dim3(1024), 0, s, bt_w, G * t, t, top);
+        ESTH_HIP(hipGetLastError());
+    }
+    const uint64_t total = (uint64_t)G * (topo == ESTH_TOPO_TORUS ? 4u : 1u) * t;
+    hipLaunchKernelGGL(k_exchange, dim3(grid_for(total, 256)), dim3(256), 0, s, st, w, S, N, m, G, t, topo, dimx,
+                       dimy, bt_w, bt_s, top);
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w, const float* rnd,
+                         uint32_t N, uint32_t m, uint32_t G, float thr, int adaptive, int log_w, int* sel,
+                         hipStream_t s) {
+    if (resampler == ESTH_RESAMPLE_ALIAS) {
+        switch (S) {
+        case 1: hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr); break;
+        case ESTH_UNI_S: hipLaunchKernelGGL(k_alias<ESTH_UNI_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr); break;
+        case ESTH_RA5_S: hipLaunchKernelGGL(k_alias<ESTH_RA5_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr); break;
+        case ESTH_DH6_S: hipLaunchKernelGGL(k_alias<ESTH_DH6_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr); break;
+        default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+        }
+        ESTH_HIP(hipGetLastError());
+        return ESTH_OK;
+    }
+    const int lm = log2u(m);
+    const int sys = resampler == ESTH_RESAMPLE_SYSTEMATIC;
+    switch (S) {
+    case 1: ESTH_DISPATCH_LOGM(launch_resample_t, 1, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
+    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_UNI_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
+    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_RA5_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
+    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_DH6_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+    }
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+/* ======================================================= device helpers */
+extern "C" int esth_dev_alloc(size_t bytes, void** p) {
+    if (!p) return esth_fail(ESTH_ERR_INVALID, "null out");
+    ESTH_HIP(hipMalloc(p, bytes ? bytes : 4));
+    return ESTH_OK;
+}
+extern "C" int esth_dev_free(void* p) { ESTH_HIP(hipFree(p)); return ESTH_OK; }
+extern "C" int esth_memcpy_htod(void* d, const void* s, size_t n) { ESTH_HIP(hipMemcpy(d, s, n, hipMemcpyHostToDevice)); return ESTH_OK; }
+extern "C" int esth_memcpy_dtoh(void* d, const void* s, size_t n) { ESTH_HIP(hipMemcpy(d, s, n, hipMemcpyDeviceToHost)); return ESTH_OK; }
+extern "C" int esth_memset_d(void* d, int v, size_t n) { ESTH_HIP(hipMemset(d, v, n)); return ESTH_OK; }
+extern "C" int esth_dev_sync(void) { ESTH_HIP(hipDeviceSynchronize()); return ESTH_OK; }
+
+/* ================================================================ stages */
+extern "C" int esth_stage_mtgp(uint32_t* d_status, int nstreams, int first_set, int size, void* d_out, int float_mode) {
+    if (nstreams <= 0 || first_set < 0 || first_set + nstreams > ESTH_MTGP_SETS)
+        return esth_fail(ESTH_ERR_INVALID, "streams %d from set %d", nstreams, first_set);
+    if (size <= 0 || size % ESTH_MTGP_LARGE) return esth_fail(ESTH_ERR_INVALID, "size %d not a multiple of 768", size);
+    std::vector<esth_mtgp_stream> hs(nstreams);
+    for (int j = 0; j < nstreams; ++j) hs[j] = {(uint32_t)(first_set + j), (uint32_t)size, (uint64_t)j * (uint64_t)size};
+    esth_mtgp_stream* ds = nullptr;
+    uint32_t* dp = nullptr;
+    ESTH_HIP(hipMalloc((void**)&ds, sizeof(esth_mtgp_stream) * nstreams));
+    ESTH_HIP(hipMalloc((void**)&dp, sizeof(g_mtgp_table)));
+    ESTH_HIP(hipMemcpy(ds, hs.data(), sizeof(esth_mtgp_stream) * nstreams, hipMemcpyHostToDevice));
+    ESTH_HIP(hipMemcpy(dp, g_mtgp_table, sizeof(g_mtgp_table), hipMemcpyHostToDevice));
+    int rc = esth_launch_mtgp(dp, d_status, ds, nstreams, d_out, float_mode, 0);
+    ESTH_HIP(hipDeviceSynchronize());
+    hipFree(ds);
+    hipFree(dp);
+    return rc;
+}
+
+extern "C" int esth_stage_boxmuller(float* d, int n) {
+    if (n <= 0) return ESTH_OK;
+    hipLaunchKernelGGL(k_boxmuller, dim3(grid_for((uint64_t)(n + 1) / 2, 256)), dim3(256), 0, 0, d, n);
+    ESTH_HIP(hipGetLastError());
+    ESTH_HIP(hipDeviceSynchronize());
+    return ESTH_OK;
+}
+
+extern "C" int esth_stage_sample_weight(int model, float* d_state, uint32_t N, const float* d_random, const float* frame,
+                                        int weights, float* d_w) {
+    if (N == 0 || (N & 1u)) return esth_fail(ESTH_ERR_INVALID, "N=%u must be even and positive", N);
+    if (!frame) return esth_fail(ESTH_ERR_INVALID, "null frame");
+    float* dfr = nullptr;
+    ESTH_HIP(hipMalloc((void**)&dfr, sizeof(float) * ESTH_FRAME_FLOATS));
+    ESTH_HIP(hipMemcpy(dfr, frame, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice));
+    int rc = esth_launch_sample_weight(model, d_state, N, d_random, dfr, weights == ESTH_WEIGHTS_LOG, d_w, 0);
+    ESTH_HIP(hipDeviceSynchronize());
+    hipFree(dfr);
+    return rc;
+}
+
+extern "C" int esth_stage_sort(float* d_w, const float* d_in, float* d_out, int S, uint32_t m, uint32_t G, float* d_winner,
+                               int* d_perm) {
+    if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES) return esth_fail(ESTH_ERR_INVALID, "m=%u", m);
+    if (S != 1 && S != ESTH_UNI_S && S != ESTH_RA5_S && S != ESTH_DH6_S)
+        return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+    const uint32_t N = m * G;
+    float* part = nullptr;
+    float2* win = nullptr;
+    ESTH_HIP(hipMalloc((void**)&part, sizeof(float) * (size_t)G * (2 + S)));
+    if (!d_winner) ESTH_HIP(hipMalloc((void**)&win, sizeof(float2) * G));
+    int rc = esth_launch_sort(S, m, d_w, d_in, d_out, N, G, d_winner ? (float2*)d_winner : win, d_perm, 0, nullptr,
+                              nullptr, part, 0, 0);
+    ESTH_HIP(hipDeviceSynchronize());
+    hipFree(part);
+    if (win) hipFree(win);
+    return rc;
+}
+
+extern "C" int esth_stage_argmax(const float* d_winner, uint32_t G, int weights, float* d_result) {
+    if (G == 0) return esth_fail(ESTH_ERR_INVALID, "G=0");
+    float* part = nullptr;
+    double* est = nullptr;
+    ESTH_HIP(hipMalloc((void**)&part, sizeof(float) * (size_t)G * 3));
+    ESTH_HIP(hipMemset(part, 0, sizeof(float) * (size_t)G * 3));
+    ESTH_HIP(hipMalloc((void**)&est, sizeof(double) * 8));
+    int rc = esth_launch_argmax(1, (const float2*)d_winner, G, weights == ESTH_WEIGHTS_LOG, part, nullptr, 0, 1,
+                                (float2*)d_result, est, 0);
+    ESTH_HIP(hipDeviceSynchronize());
+    hipFree(part);
+    hipFree(est);
+    return rc;
+}
+
+extern "C" int esth_stage_exchange(float* d_state, float* d_w, int S, uint32_t m, uint32_t G, uint32_t t, int topo,
+                                   int dimx, int dimy) {
+    if (t == 0 || topo == ESTH_TOPO_NONE) return ESTH_OK;
+    esth_filter_desc d;
+    esth_filter_desc_default(&d, ESTH_MODEL_ROBOTARM5);
+    d.num_particles = m; d.num_blocks = G; d.num_transfer = t; d.topology = topo; d.torus_dimx = dimx; d.torus_dimy = dimy;
+    int s_, v_;
+    ESTH_TRY(esth_check_desc(&d, &s_, &v_));
+    if (topo == ESTH_TOPO_TORUS && (dimx <= 0 || dimy <= 0)) esth_torus_dims(G, &dimx, &dimy);
+    const uint32_t N = m * G;
+    float *bw = nullptr, *bs = nullptr;
+    uint32_t* top = nullptr;
+    ESTH_HIP(hipMalloc((void**)&bw, sizeof(float) * (size_t)G * t));
+    ESTH_HIP(hipMalloc((void**)&bs, sizeof(float) * (size_t)G * t * S));
+    ESTH_HIP(hipMalloc((void**)&top, sizeof(uint32_t) * t));
+    /* compact the best t of every (already sorted) block */
+    for (uint32_t k = 0; k < t; ++k) {
+        ESTH_HIP(hipMemcpy2D(bw + k, sizeof(float) * t, d_w + k, sizeof(float) * m, sizeof(float), G, hipMemcpyDeviceToDevice));
+        for (int c = 0; c < S; ++c)
+            ESTH_HIP(hipMemcpy2D(bs + (size_t)c * G * t + k, sizeof(float) * t, d_state + (size_t)c * N + k, sizeof(float) * m,
+                                 sizeof(float), G, hipMemcpyDeviceToDevice));
+    }
+    int rc = esth_launch_exchange(d_state, d_w, S, N, m, G, t, topo, dimx, dimy, bw, bs, top, 0);
+    ESTH_HIP(hipDeviceSynchronize());
+    hipFree(bw);
+    hipFree(bs);
+    hipFree(top);
+    return rc;
+}
+
+extern "C" int esth_stage_resample(int resampler, float* dst, const float* src, int S, const float* d_w, const float* d_random,
+                                   uint32_t m, uint32_t G, float thr, int adaptive, int weights, int* d_sel) {
+    esth_filter_desc d;
+    esth_filter_desc_default(&d, ESTH_MODEL_ROBOTARM5);
+    d.num_particles = m; d.num_blocks = G; d.num_transfer = 0; d.resampler = resampler;
+    d.resampling_threshold = thr; d.adaptive = adaptive; d.weights = weights;
+    int s_, v_;
+    ESTH_TRY(esth_check_desc(&d, &s_, &v_));
+    int rc = esth_launch_resample(resampler, S, dst, src, d_w, d_random, m * G, m, G, thr, adaptive,
+                                  weights == ESTH_WEIGHTS_LOG, d_sel, 0);
+    ESTH_HIP(hipDeviceSynchronize());
+    return rc;
+}
+
+extern "C" int esth_stage_alias_table(const float* d_w, uint32_t m, uint32_t G, int weights, float* d_prob, uint32_t* d_alias) {
+    if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES) return esth_fail(ESTH_ERR_INVALID, "m=%u", m);
+    if (!d_prob || !d_alias) return esth_fail(ESTH_ERR_INVALID, "null output");
+    hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, 0, (float*)nullptr, (const float*)nullptr, d_w,
+                       (const float*)nullptr, m * G, m, 1.0f, 0, weights == ESTH_WEIGHTS_LOG, (int*)nullptr, d_prob, d_alias);
+    ESTH_HIP(hipGetLastError());
+    ESTH_HIP(hipDeviceSynchronize());
+    return ESTH_OK;
+}
+
+extern "C" int esth_stage_math(int fn, const float* x, const float* x2, float* out, int n) {
+    if (n <= 0) return ESTH_OK;
+    if (fn == 4 && !x2) return esth_fail(ESTH_ERR_INVALID, "atan2 needs x2");
+    hipLaunchKernelGGL(k_math, dim3(grid_for((uint64_t)n, 256)), dim3(256), 0, 0, fn, x, x2, out, n);
+    ESTH_HIP(hipGetLastError());
+    ESTH_HIP(hipDeviceSynchronize());
+    return ESTH_OK;
+}

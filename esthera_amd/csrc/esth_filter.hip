@@ -1,0 +1,541 @@
+/*
+ * esth_filter.hip - one island-model filter on one GPU: set-up and the step
+ * of particle_filter() (src/partfilt/opencl/pf_generic.c:245-887) as a
+ * sequence of kernels on the filter's stream, with no host synchronisation
+ * inside a step (the reference blocks three times per step: pf_generic.c:231,
+ * 614, 806).
+ */
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "esth_internal.h"
+
+#define ESTH_FRAME_RING 16
+
+struct esth_filter {
+    esth_engine* e = nullptr;
+    esth_filter_desc d{};
+    int S = 0, V = 0, K = 0;
+    uint32_t m = 0, G = 0, t = 0;
+    size_t N = 0;
+    uint64_t needed = 0, size = 0, words = 0;
+    int dimx = 0, dimy = 0;
+    unsigned char params[sizeof(esth_unicycle_params) > sizeof(esth_dh6_params) ? sizeof(esth_unicycle_params)
+                                                                                 : sizeof(esth_dh6_params)];
+    hipStream_t stream = nullptr;
+    /* device */
+    float *state = nullptr, *tmp = nullptr, *w = nullptr, *R = nullptr, *bt_w = nullptr, *bt_s = nullptr,
+          *part = nullptr, *frames = nullptr;
+    float2 *winner = nullptr, *result = nullptr;
+    double* est = nullptr;
+    uint32_t *status = nullptr, *top = nullptr;
+    esth_mtgp_stream* streams = nullptr;
+    std::vector<std::pair<void*, size_t>> allocs;
+    /* host */
+    float* h_frames = nullptr;   /* pinned ring */
+    hipEvent_t frame_ev[ESTH_FRAME_RING];
+    bool frame_used[ESTH_FRAME_RING] = {};
+    int slot = 0;
+    hipEvent_t ev[7];
+    bool timing = false;
+    uint64_t steps = 0;
+    /* graph replay */
+    bool use_graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int gslot[2] = {-1, -1};
+};
+
+static int falloc(esth_filter* f, size_t bytes, void** p) {
+    ESTH_TRY(esth_engine_alloc(f->e, bytes, p));
+    f->allocs.push_back({*p, bytes});
+    return ESTH_OK;
+}
+
+static void filter_free(esth_filter* f) {
+    if (!f) return;
+    hipSetDevice(f->e->device);
+    if (f->stream) hipStreamSynchronize(f->stream);
+    for (int i = 0; i < 2; ++i)
+        if (f->gexec[i]) hipGraphExecDestroy(f->gexec[i]);
+    for (auto& a : f->allocs) esth_engine_release(f->e, a.first, a.second);
+    f->allocs.clear();
+    if (f->h_frames) hipHostFree(f->h_frames);
+    for (int i = 0; i < ESTH_FRAME_RING; ++i) hipEventDestroy(f->frame_ev[i]);
+    for (int i = 0; i < 7; ++i) hipEventDestroy(f->ev[i]);
+    if (f->stream) hipStreamDestroy(f->stream);
+    delete f;
+}
+
+extern "C" int esth_filter_create(esth_engine* e, const esth_filter_desc* d, const void* params, size_t params_size,
+                                  esth_filter** out) {
+    if (!e || !out) return esth_fail(ESTH_ERR_INVALID, "null engine/out");
+    int S, V;
+    ESTH_TRY(esth_check_desc(d, &S, &V));
+    size_t need = 0;
+    ESTH_TRY(esth_params_size(d->model, &need));
+    if (need && (!params || params_size != need))
+        return esth_fail(ESTH_ERR_INVALID, "model %d needs %zu bytes of params, got %zu", d->model, need, params_size);
+    ESTH_HIP(hipSetDevice(e->device));
+    esth_filter* f = new esth_filter();
+    f->e = e;
+    f->d = *d;
+    f->S = S;
+    f->V = V;
+    f->m = d->num_particles;
+    f->G = d->num_blocks;
+    f->t = (d->topology == ESTH_TOPO_NONE) ? 0 : d->num_transfer;
+    f->N = (size_t)f->m * f->G;
+    memset(f->params, 0, sizeof(f->params));
+    if (need) memcpy(f->params, params, need);
+    f->needed = esth_rng_needed(V, d->resampler, d->adaptive, f->m, f->G);
+    f->K = esth_rng_streams(d->rng, f->needed);
+    f->size = esth_rng_stream_size(f->K, f->needed);
+    f->words = f->size * (uint64_t)f->K;
+    f->dimx = d->torus_dimx;
+    f->dimy = d->torus_dimy;
+    if (d->topology == ESTH_TOPO_TORUS && (f->dimx <= 0 || f->dimy <= 0)) esth_torus_dims(f->G, &f->dimx, &f->dimy);
+    for (int i = 0; i < ESTH_FRAME_RING; ++i) hipEventCreateWithFlags(&f->frame_ev[i], hipEventDisableTiming);
+    for (int i = 0; i < 7; ++i) hipEventCreate(&f->ev[i]);
+    int rc = ESTH_OK;
+    const size_t SN = (size_t)S * f->N;
+    const size_t tG = (size_t)(f->t ? f->t : 1) * f->G;
+#define FA(ptr, bytes) if (rc == ESTH_OK) rc = falloc(f, (bytes), (void**)&(ptr))
+    FA(f->state, sizeof(float) * SN);
+    FA(f->tmp, sizeof(float) * SN);
+    FA(f->w, sizeof(float) * f->N);
+    FA(f->R, sizeof(float) * f->words);
+    FA(f->bt_w, sizeof(float) * tG);
+    FA(f->bt_s, sizeof(float) * tG * S);
+    FA(f->part, sizeof(float) * (size_t)f->G * (2 + S));
+    FA(f->frames, sizeof(float) * ESTH_FRAME_FLOATS * ESTH_FRAME_RING);
+    FA(f->winner, sizeof(float2) * f->G);
+    FA(f->result, sizeof(float2));
+    FA(f->est, sizeof(double) * (4 + 2 * ESTH_MAX_STATE_DIM));
+    FA(f->status, sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K);
+    FA(f->top, sizeof(uint32_t) * 64);
+    FA(f->streams, sizeof(esth_mtgp_stream) * (size_t)f->K);
+#undef FA
+    if (rc != ESTH_OK) { filter_free(f); return rc; }
+    hipError_t err = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipHostMalloc((void**)&f->h_frames, sizeof(float) * ESTH_FRAME_FLOATS * ESTH_FRAME_RING, 0);
+    if (err != hipSuccess) { filter_free(f); return esth_fail(ESTH_ERR_DEVICE, "stream/pinned: %s", hipGetErrorString(err)); }
+    /* RNG status: make_kernel_data32 (mtgp32-opencl.c:31-60), stream j seeded seed + j + 1 */
+    std::vector<uint32_t> st((size_t)ESTH_MTGP_N * f->K);
+    std::vector<esth_mtgp_stream> sd(f->K);
+    for (int j = 0; j < f->K; ++j) {
+        esth_mtgp_seed_state(j, d->seed + (uint32_t)j + 1u, st.data() + (size_t)j * ESTH_MTGP_N);
+        sd[j] = {(uint32_t)j, (uint32_t)f->size, (uint64_t)j * f->size};
+    }
+    err = hipMemcpy(f->status, st.data(), st.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(f->streams, sd.data(), sd.size() * sizeof(esth_mtgp_stream), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemset(f->w, 0, sizeof(float) * f->N);
+    if (err == hipSuccess) err = hipMemset(f->est, 0, sizeof(double) * (4 + 2 * ESTH_MAX_STATE_DIM));
+    if (err != hipSuccess) { filter_free(f); return esth_fail(ESTH_ERR_DEVICE, "init upload: %s", hipGetErrorString(err)); }
+    *out = f;
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_destroy(esth_filter* f) {
+    filter_free(f);
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_init_uniform(esth_filter* f, const float* s) {
+    if (!f || !s) return esth_fail(ESTH_ERR_INVALID, "null filter/state");
+    std::vector<float> h((size_t)f->S * f->N);
+    for (int c = 0; c < f->S; ++c) std::fill(h.begin() + (size_t)c * f->N, h.begin() + (size_t)(c + 1) * f->N, s[c]);
+    return esth_filter_init(f, h.data());
+}
+
+extern "C" int esth_filter_init(esth_filter* f, const float* soa) {
+    if (!f || !soa) return esth_fail(ESTH_ERR_INVALID, "null filter/state");
+    ESTH_HIP(hipSetDevice(f->e->device));
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    ESTH_HIP(hipMemcpy(f->state, soa, sizeof(float) * (size_t)f->S * f->N, hipMemcpyHostToDevice));
+    return ESTH_OK;
+}
+
+/* the kernels of one step, reading the frame at d_frame */
+static int enqueue_step(esth_filter* f, const float* d_frame, bool timing) {
+    hipStream_t s = f->stream;
+    const esth_filter_desc& d = f->d;
+    const int log_w = d.weights == ESTH_WEIGHTS_LOG;
+    const uint32_t N = (uint32_t)f->N;
+    if (timing) ESTH_HIP(hipEventRecord(f->ev[0], s));
+    /* generate_normal_random: MTGP words (pf_generic.c:191-232) */
+    ESTH_TRY(esth_launch_mtgp(f->e->d_params, f->status, f->streams, f->K, f->R, 1, s));
+    if (timing) ESTH_HIP(hipEventRecord(f->ev[1], s));
+    /* sampling_importance with Box-Muller fused (pf_generic.c:521-533) */
+    ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, f->R, d_frame, log_w, f->w, s));
+    if (timing) ESTH_HIP(hipEventRecord(f->ev[2], s));
+    /* block_sort (pf_generic.c:572-585) */
+    ESTH_TRY(esth_launch_sort(f->S, f->m, f->w, f->state, f->tmp, N, f->G, f->winner, nullptr, f->t, f->bt_w,
+                              f->bt_s, f->part, log_w, s));
+    if (timing) ESTH_HIP(hipEventRecord(f->ev[3], s));
+    /* reduce (pf_generic.c:604-615), result kept on the device */
+    ESTH_TRY(esth_launch_argmax(f->S, f->winner, f->G, log_w, f->part, f->tmp, N, f->m, f->result, f->est, s));
+    if (timing) ESTH_HIP(hipEventRecord(f->ev[4], s));
+    /* exchange (pf_generic.c:702-715) */
+    ESTH_TRY(esth_launch_exchange(f->tmp, f->w, f->S, N, f->m, f->G, f->t, d.topology, f->dimx, f->dimy, f->bt_w,
+                                  f->bt_s, f->top, s));
+    if (timing) ESTH_HIP(hipEventRecord(f->ev[5], s));
+    /* resampling (pf_generic.c:732-757), uniforms at rand_offset = N*V */
+    ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, f->R + (size_t)N * f->V, N, f->m, f->G,
+                                  d.resampling_threshold, d.adaptive, log_w, nullptr, s));
+    if (timing) ESTH_HIP(hipEventRecord(f->ev[6], s));
+    return ESTH_OK;
+}
+
+static int capture_graph(esth_filter* f, int slot, hipGraphExec_t* out) {
+    hipGraph_t g = nullptr;
+    ESTH_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+    const float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
+    ESTH_HIP(hipMemcpyAsync((void*)dfr, f->h_frames + (size_t)slot * ESTH_FRAME_FLOATS, sizeof(float) * ESTH_FRAME_FLOATS,
+                            hipMemcpyHostToDevice, f->stream));
+    int rc = enqueue_step(f, dfr, false);
+    hipError_t e2 = hipStreamEndCapture(f->stream, &g);
+    if (rc != ESTH_OK) { if (g) hipGraphDestroy(g); return rc; }
+    if (e2 != hipSuccess) return esth_fail(ESTH_ERR_DEVICE, "end capture: %s", hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e3 != hipSuccess) return esth_fail(ESTH_ERR_DEVICE, "instantiate: %s", hipGetErrorString(e3));
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_step(esth_filter* f, const void* control, const void* measurement, float dt) {
+    if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
+    ESTH_HIP(hipSetDevice(f->e->device));
+    /* frame slot: graph mode alternates two fixed slots, eager mode a ring */
+    int slot = f->use_graph ? (int)(f->steps & 1) : f->slot;
+    if (f->frame_used[slot]) ESTH_HIP(hipEventSynchronize(f->frame_ev[slot]));
+    float* hf = f->h_frames + (size_t)slot * ESTH_FRAME_FLOATS;
+    if (esth_build_frame_impl(f->d.model, f->params, control, measurement, dt, hf))
+        return esth_fail(ESTH_ERR_INVALID, "bad control/measurement for model %d", f->d.model);
+    if (f->use_graph) {
+        if (!f->gexec[slot]) ESTH_TRY(capture_graph(f, slot, &f->gexec[slot]));
+        ESTH_HIP(hipGraphLaunch(f->gexec[slot], f->stream));
+    } else {
+        float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
+        ESTH_HIP(hipMemcpyAsync(dfr, hf, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice, f->stream));
+        ESTH_TRY(enqueue_step(f, dfr, f->timing));
+        f->slot = (f->slot + 1) % ESTH_FRAME_RING;
+    }
+    ESTH_HIP(hipEventRecord(f->frame_ev[slot], f->stream));
+    f->frame_used[slot] = true;
+    f->steps++;
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_use_graph(esth_filter* f, int enable) {
+    if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    f->use_graph = enable != 0;
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_sync(esth_filter* f) {
+    if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_stream(esth_filter* f, void** s) {
+    if (!f || !s) return esth_fail(ESTH_ERR_INVALID, "null");
+    *s = (void*)f->stream;
+    return ESTH_OK;
+}
+
+int esth_filter_set_timing(esth_filter* f, int on);
+extern "C" int esth_filter_set_timing(esth_filter* f, int on) {
+    if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
+    f->timing = on != 0;
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_stage_times(esth_filter* f, float us[6]) {
+    if (!f || !us) return esth_fail(ESTH_ERR_INVALID, "null");
+    if (!f->timing || f->use_graph || f->steps == 0) return esth_fail(ESTH_ERR_STATE, "stage timing is off");
+    ESTH_HIP(hipEventSynchronize(f->ev[6]));
+    for (int i = 0; i < 6; ++i) {
+        float ms = 0.0f;
+        ESTH_HIP(hipEventElapsedTime(&ms, f->ev[i], f->ev[i + 1]));
+        us[i] = ms * 1000.0f;
+    }
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_estimate(esth_filter* f, esth_estimate* out) {
+    if (!f || !out) return esth_fail(ESTH_ERR_INVALID, "null");
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    double h[4 + 2 * ESTH_MAX_STATE_DIM];
+    ESTH_HIP(hipMemcpy(h, f->est, sizeof(double) * (4 + 2 * (size_t)f->S), hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(*out));
+    const int S = f->S;
+    out->state_dim = S;
+    out->log_weight_max = h[0];
+    out->weight_sum = h[1];
+    for (int c = 0; c < S; ++c) {
+        out->mean[c] = h[1] > 0.0 ? h[2 + c] / h[1] : NAN;
+        out->best[c] = (float)h[4 + S + c];
+    }
+    out->best_weight = (float)h[2 + S];
+    out->best_block = (int)h[3 + S];
+    out->step = f->steps;
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_read_particles(esth_filter* f, float* soa, float* w) {
+    if (!f) return esth_fail(ESTH_ERR_INVALID, "null");
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    if (soa) ESTH_HIP(hipMemcpy(soa, f->state, sizeof(float) * (size_t)f->S * f->N, hipMemcpyDeviceToHost));
+    if (w) ESTH_HIP(hipMemcpy(w, f->w, sizeof(float) * f->N, hipMemcpyDeviceToHost));
+    return ESTH_OK;
+}
+
+/* snapshot: magic, S, N, K, steps | state | w | status */
+#define ESTH_SNAP_MAGIC 0x45535448u
+extern "C" int esth_filter_snapshot_size(esth_filter* f, size_t* bytes) {
+    if (!f || !bytes) return esth_fail(ESTH_ERR_INVALID, "null");
+    *bytes = 8 * sizeof(uint64_t) + sizeof(float) * ((size_t)f->S * f->N + f->N) + sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K;
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_snapshot(esth_filter* f, void* buf, size_t bytes) {
+    size_t need = 0;
+    ESTH_TRY(esth_filter_snapshot_size(f, &need));
+    if (!buf || bytes < need) return esth_fail(ESTH_ERR_INVALID, "snapshot buffer %zu < %zu", bytes, need);
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    uint64_t* hdr = (uint64_t*)buf;
+    hdr[0] = ESTH_SNAP_MAGIC; hdr[1] = (uint64_t)f->S; hdr[2] = f->N; hdr[3] = (uint64_t)f->K; hdr[4] = f->steps;
+    hdr[5] = f->d.model; hdr[6] = f->m; hdr[7] = f->G;
+    char* p = (char*)buf + 8 * sizeof(uint64_t);
+    ESTH_HIP(hipMemcpy(p, f->state, sizeof(float) * (size_t)f->S * f->N, hipMemcpyDeviceToHost));
+    p += sizeof(float) * (size_t)f->S * f->N;
+    ESTH_HIP(hipMemcpy(p, f->w, sizeof(float) * f->N, hipMemcpyDeviceToHost));
+    p += sizeof(float) * f->N;
+    ESTH_HIP(hipMemcpy(p, f->status, sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K, hipMemcpyDeviceToHost));
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_restore(esth_filter* f, const void* buf, size_t bytes) {
+    size_t need = 0;
+    ESTH_TRY(esth_filter_snapshot_size(f, &need));
+    if (!buf || bytes < need) return esth_fail(ESTH_ERR_INVALID, "snapshot buffer %zu < %zu", bytes, need);
+    const uint64_t* hdr = (const uint64_t*)buf;
+    if (hdr[0] != ESTH_SNAP_MAGIC || hdr[1] != (uint64_t)f->S || hdr[2] != f->N || hdr[3] != (uint64_t)f->K ||
+        hdr[5] != (uint64_t)f->d.model || hdr[6] != f->m || hdr[7] != f->G)
+        return esth_fail(ESTH_ERR_INVALID, "snapshot does not match this filter");
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    const char* p = (const char*)buf + 8 * sizeof(uint64_t);
+    ESTH_HIP(hipMemcpy(f->state, p, sizeof(float) * (size_t)f->S * f->N, hipMemcpyHostToDevice));
+    p += sizeof(float) * (size_t)f->S * f->N;
+    ESTH_HIP(hipMemcpy(f->w, p, sizeof(float) * f->N, hipMemcpyHostToDevice));
+    p += sizeof(float) * f->N;
+    ESTH_HIP(hipMemcpy(f->status, p, sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K, hipMemcpyHostToDevice));
+    f->steps = hdr[4];
+    return ESTH_OK;
+}
+
+/* ================================================= trace I/O, reference */
+/* read_trace, pf_model_host.h:30-54: 20 whitespace-separated floats per row */
+extern "C" int esth_trace_load_robotarm(const char* path, int max_rows, esth_robotarm5_measurement* meas,
+                                        esth_robotarm5_control* ctrl, float* dt, float* truth, int* rows) {
+    if (!path || !rows) return esth_fail(ESTH_ERR_INVALID, "null");
+    FILE* fh = fopen(path, "r");
+    if (!fh) return esth_fail(ESTH_ERR_IO, "could not open %s", path);
+    int n = 0;
+    while (n < max_rows) {
+        float v[20];
+        int k = 0;
+        for (; k < 20; ++k)
+            if (fscanf(fh, "%f", &v[k]) != 1) break;
+        if (k < 20) break;   /* a short or garbled row ends the trace, as read_trace's -1 */
+        if (meas) {
+            meas[n].x = v[0]; meas[n].y = v[1];
+            for (int i = 0; i < 5; ++i) meas[n].angles[i] = v[2 + i];
+        }
+        if (ctrl) for (int i = 0; i < 5; ++i) ctrl[n].angles[i] = v[7 + i];
+        if (dt) dt[n] = v[12];
+        if (truth) {
+            float* tr = truth + (size_t)n * 9;
+            for (int i = 0; i < 5; ++i) tr[i] = v[15 + i];
+            tr[5] = v[13]; tr[6] = v[14]; tr[7] = 0.0f; tr[8] = 0.0f;
+        }
+        ++n;
+    }
+    fclose(fh);
+    *rows = n;
+    return ESTH_OK;
+}
+
+/* estimate_error, pf_model_host.h:56-69 */
+extern "C" float esth_robotarm5_estimate_error(const float* est, const float* act) {
+    float error = 0.0f;
+    error += (est[5] - act[5]) * (est[5] - act[5]) * ESTH_RA5_NOISE_CAMERA_X;
+    error += (est[6] - act[6]) * (est[6] - act[6]) * ESTH_RA5_NOISE_CAMERA_Y;
+    for (int i = 0; i < 5; ++i) error += (est[i] - act[i]) * (est[i] - act[i]) * ESTH_RA5_NOISE_ANGLE_MEASUREMENTS;
+    return error;
+}
+
+/* particle_filter() for the robot arm over a trace (pf_generic.c:245-887) */
+extern "C" int esth_particle_filter(uint32_t m, uint32_t G, uint32_t t, float thr, const char* trace, uint32_t seed,
+                                    int resampler, double* mean_error, int* steps) {
+    const int max_rows = 1 << 20;
+    std::vector<esth_robotarm5_measurement> meas;
+    std::vector<esth_robotarm5_control> ctrl;
+    std::vector<float> dt, truth;
+    int rows = 0;
+    {
+        FILE* fh = fopen(trace, "r");
+        if (!fh) return esth_fail(ESTH_ERR_IO, "could not open %s", trace);
+        int lines = 0, ch;
+        while ((ch = fgetc(fh)) != EOF) lines += ch == '\n';
+        fclose(fh);
+        rows = std::min(lines + 1, max_rows);
+    }
+    meas.resize(rows); ctrl.resize(rows); dt.resize(rows); truth.resize((size_t)rows * 9);
+    ESTH_TRY(esth_trace_load_robotarm(trace, rows, meas.data(), ctrl.data(), dt.data(), truth.data(), &rows));
+    esth_engine* e = nullptr;
+    ESTH_TRY(esth_engine_create(0, 0, &e));
+    esth_filter_desc d;
+    esth_filter_desc_default(&d, ESTH_MODEL_ROBOTARM5);
+    d.num_particles = m; d.num_blocks = G; d.num_transfer = t; d.resampling_threshold = thr;
+    d.resampler = resampler; d.seed = seed; d.topology = t ? ESTH_TOPO_RING : ESTH_TOPO_NONE;
+    esth_filter* f = nullptr;
+    int rc = esth_filter_create(e, &d, nullptr, 0, &f);
+    if (rc != ESTH_OK) { esth_engine_destroy(e); return rc; }
+    /* initial particles, pf_generic.c:346-361 */
+    const float a0 = (float)(0.75f * M_PI);
+    float init[9] = {a0, a0, a0, a0, a0, 7.0f, 0.0f, 0.0f, 0.0f};
+    rc = esth_filter_init_uniform(f, init);
+    double err_sum = 0.0;
+    for (int i = 0; rc == ESTH_OK && i < rows; ++i) {
+        rc = esth_filter_step(f, &ctrl[i], &meas[i], dt[i]);
+        esth_estimate est;
+        if (rc == ESTH_OK) rc = esth_filter_estimate(f, &est);
+        if (rc == ESTH_OK) {
+            float best[9];
+            for (int c = 0; c < 9; ++c) best[c] = est.best[c];
+            best[7] = 0.0f; best[8] = 0.0f;
+            err_sum += esth_robotarm5_estimate_error(best, &truth[(size_t)i * 9]);
+        }
+    }
+    esth_filter_destroy(f);
+    esth_engine_destroy(e);
+    if (rc != ESTH_OK) return rc;
+    if (mean_error) *mean_error = rows ? err_sum / rows : 0.0;
+    if (steps) *steps = rows;
+    return ESTH_OK;
+}
+
+/* ============================================================ scenarios
+ * Seeded host generators: splitmix64 words, uniform doubles in [0,1), normals
+ * by the polar-free Box-Muller in double. */
+struct esth_hostrng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal() {
+        double u1 = 1.0 - uniform(), u2 = uniform();
+        return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    }
+};
+
+static double wrap_pi(double a) {
+    a = fmod(a + M_PI, 2.0 * M_PI);
+    if (a <= 0.0) a += 2.0 * M_PI;
+    return a - M_PI;
+}
+
+extern "C" int esth_unicycle_scenario(uint32_t seed, int L, float box, int steps, float dt, esth_unicycle_params* p,
+                                      esth_unicycle_control* ctrl, esth_unicycle_measurement* meas, float* truth) {
+    if (!p || L < 0 || L > ESTH_UNICYCLE_MAX_LANDMARKS || steps < 0) return esth_fail(ESTH_ERR_INVALID, "scenario args");
+    esth_hostrng rng{0x5eed000000000000ull ^ (uint64_t)seed};
+    memset(p, 0, sizeof(*p));
+    p->num_landmarks = L;
+    p->sigma_x = 0.05f; p->sigma_y = 0.05f; p->sigma_theta = 0.01f;
+    p->sigma_range = 0.1f; p->sigma_bearing = 0.02f;
+    if (L == 4 && box == 0.0f) {
+        const float lx[4] = {10.0f, -10.0f, -10.0f, 10.0f}, ly[4] = {10.0f, 10.0f, -10.0f, -10.0f};
+        for (int j = 0; j < 4; ++j) { p->lx[j] = lx[j]; p->ly[j] = ly[j]; }
+    } else {
+        for (int j = 0; j < L; ++j) {
+            p->lx[j] = (float)(-box + 2.0 * box * rng.uniform());
+            p->ly[j] = (float)(-box + 2.0 * box * rng.uniform());
+        }
+    }
+    double x = 0.0, y = 0.0, th = 0.0;
+    for (int k = 0; k < steps; ++k) {
+        const double v = 1.0, om = 0.2 * sin(0.05 * (double)k);
+        double nth = th;
+        x += v * dt * cos(nth) + p->sigma_x * rng.normal();
+        y += v * dt * sin(nth) + p->sigma_y * rng.normal();
+        th += om * dt + p->sigma_theta * rng.normal();
+        if (ctrl) { ctrl[k].v = (float)v; ctrl[k].omega = (float)om; }
+        if (truth) { truth[3 * k] = (float)x; truth[3 * k + 1] = (float)y; truth[3 * k + 2] = (float)th; }
+        if (meas) {
+            memset(&meas[k], 0, sizeof(meas[k]));
+            for (int j = 0; j < L; ++j) {
+                double dx = p->lx[j] - x, dy = p->ly[j] - y;
+                meas[k].range[j] = (float)(sqrt(dx * dx + dy * dy) + p->sigma_range * rng.normal());
+                meas[k].bearing[j] = (float)wrap_pi(atan2(dy, dx) - th + p->sigma_bearing * rng.normal());
+            }
+        }
+    }
+    return ESTH_OK;
+}
+
+extern "C" int esth_dh6_scenario(uint32_t seed, int steps, float dt, esth_dh6_params* p, esth_dh6_measurement* meas,
+                                 float* truth) {
+    if (!p || steps < 0) return esth_fail(ESTH_ERR_INVALID, "scenario args");
+    esth_hostrng rng{0xd6a0000000000000ull ^ (uint64_t)seed};
+    memset(p, 0, sizeof(*p));
+    const double alpha[6] = {M_PI / 2, 0.0, 0.0, M_PI / 2, -M_PI / 2, 0.0};
+    for (int i = 0; i < 6; ++i) {
+        p->a[i] = (float)(0.2 + 0.4 * rng.uniform());
+        p->d[i] = (float)(-0.1 + 0.2 * rng.uniform());
+        p->alpha[i] = (float)alpha[i];
+    }
+    p->sigma_acc = 0.5f; p->sigma_pos = 0.01f; p->sigma_enc = 0.005f;
+    double q[6], qd[6];
+    for (int i = 0; i < 6; ++i) { q[i] = -0.5 + rng.uniform(); qd[i] = -0.5 + rng.uniform(); }
+    /* truth row 0 holds the initial state; rows 1..steps the states after each step */
+    if (truth) for (int i = 0; i < 6; ++i) { truth[i] = (float)q[i]; truth[6 + i] = (float)qd[i]; }
+    for (int k = 0; k < steps; ++k) {
+        for (int i = 0; i < 6; ++i) {
+            double a = p->sigma_acc * rng.normal();
+            q[i] += qd[i] * dt + 0.5 * dt * dt * a;
+            qd[i] += a * dt;
+        }
+        double T[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+        for (int i = 0; i < 6; ++i) {
+            double cq = cos(q[i]), sq = sin(q[i]), ca = cos((double)p->alpha[i]), sa = sin((double)p->alpha[i]);
+            double A[4][4] = {{cq, -sq * ca, sq * sa, p->a[i] * cq}, {sq, cq * ca, -cq * sa, p->a[i] * sq}, {0, sa, ca, p->d[i]}, {0, 0, 0, 1}};
+            double R[4][4];
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) {
+                    double acc = 0;
+                    for (int kk = 0; kk < 4; ++kk) acc += T[r][kk] * A[kk][c];
+                    R[r][c] = acc;
+                }
+            memcpy(T, R, sizeof(T));
+        }
+        if (meas) {
+            for (int i = 0; i < 3; ++i) meas[k].p[i] = (float)(T[i][3] + p->sigma_pos * rng.normal());
+            for (int i = 0; i < 6; ++i) meas[k].q[i] = (float)(q[i] + p->sigma_enc * rng.normal());
+        }
+        if (truth) {
+            float* tr = truth + (size_t)(k + 1) * 12;
+            for (int i = 0; i < 6; ++i) { tr[i] = (float)q[i]; tr[6 + i] = (float)qd[i]; }
+        }
+    }
+    return ESTH_OK;
+}

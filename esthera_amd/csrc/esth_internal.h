@@ -1,0 +1,68 @@
+/*
+ * esth_internal.h - host-side internals shared by the engine's translation units.
+ */
+#ifndef ESTH_INTERNAL_H
+#define ESTH_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "esthera.h"
+#include "esth_layout.h"
+#include "esth_models.h"
+
+int esth_fail(int code, const char* fmt, ...);
+
+#define ESTH_HIP(x)                                                                        \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return esth_fail(ESTH_ERR_DEVICE, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                             __FILE__, __LINE__);                                          \
+    } while (0)
+
+#define ESTH_TRY(x)               \
+    do {                          \
+        int rc_ = (x);            \
+        if (rc_ != ESTH_OK) return rc_; \
+    } while (0)
+
+struct esth_engine {
+    int device = 0;
+    size_t budget = 0;
+    size_t used = 0;
+    std::mutex mu;
+    uint32_t* d_params = nullptr;   /* 200 x 52 MTGP parameter words */
+};
+
+int esth_engine_alloc(esth_engine* e, size_t bytes, void** p);
+void esth_engine_release(esth_engine* e, void* p, size_t bytes);
+int esth_check_desc(const esth_filter_desc* d, int* S, int* V);
+int esth_params_size(int model, size_t* sz);
+const uint32_t* esth_param_table();
+
+/* launchers shared by filters, batches and stage entry points */
+int esth_launch_mtgp(const uint32_t* d_params, uint32_t* d_status, const esth_mtgp_stream* d_streams,
+                     int nstreams, void* d_out, int float_mode, hipStream_t s);
+int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame,
+                              int log_w, float* w, hipStream_t s);
+int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_out, uint32_t N,
+                     uint32_t G, float2* winner, int* perm, uint32_t t, float* bt_w, float* bt_s,
+                     float* part, int log_w, hipStream_t s);
+int esth_launch_argmax(int S, const float2* winner, uint32_t G, int log_w, const float* part,
+                       const float* sorted, uint32_t N, uint32_t m, float2* result, double* est,
+                       hipStream_t s);
+int esth_launch_exchange(float* st, float* w, int S, uint32_t N, uint32_t m, uint32_t G, uint32_t t,
+                         int topo, int dimx, int dimy, const float* bt_w, const float* bt_s,
+                         uint32_t* top, hipStream_t s);
+int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w,
+                         const float* rnd, uint32_t N, uint32_t m, uint32_t G, float thr,
+                         int adaptive, int log_w, int* sel, hipStream_t s);
+
+#endif

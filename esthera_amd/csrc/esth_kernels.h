@@ -1,0 +1,724 @@
+/*
+ * esth_kernels.h - gfx950 kernels of the filter step.
+ *
+ * Layout in HBM (one filter, N = m*G particles):
+ *   state, tmp : S float planes of N (structure of arrays, plane c at c*N)
+ *   w          : N floats, linear or log weights
+ *   R          : the step's MTGP32 words, K streams x size (float in [1,2))
+ *   winner     : G float2 (best weight, block id)
+ *   bt_w, bt_s : best t of every sub-filter, compact (G*t, S planes of G*t)
+ *   part       : G x (2+S) floats: block max, sum w, sum w*x_c (posterior)
+ *
+ * Execution model: one 64-lane wave per sub-filter for sort and resampling
+ * (E = m/64 elements per lane, lane-major, e = lane*E + r; for m < 64 a wave
+ * holds 64/m sub-filters with one element per lane).  Strides below E are
+ * register compare-exchanges, larger ones cross lanes by xor shuffles, so the
+ * sort needs no LDS and no barrier.  Vose's alias build keeps the reference's
+ * work-group-per-sub-filter shape because its pairing rounds are
+ * data-dependent and share worklists.
+ */
+#ifndef ESTH_KERNELS_H
+#define ESTH_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "esth_layout.h"
+#include "esth_models.h"
+
+#define ESTH_WAVE 64
+
+/* ------------------------------------------------------------ wave ops */
+__device__ __forceinline__ float esth_shfl_xor(float v, int mask) {
+    int idx = ((int)__lane_id() ^ mask) << 2;
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(idx, __builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ int esth_shfl_xor_i(int v, int mask) {
+    int idx = ((int)__lane_id() ^ mask) << 2;
+    return __builtin_amdgcn_ds_bpermute(idx, v);
+}
+__device__ __forceinline__ float esth_shfl_idx(float v, int src_lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src_lane << 2, __builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double esth_shfl_xor_d(double v, int mask) {
+    uint64_t u = __builtin_bit_cast(uint64_t, v);
+    int lo = esth_shfl_xor_i((int)(uint32_t)u, mask);
+    int hi = esth_shfl_xor_i((int)(uint32_t)(u >> 32), mask);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+/* ================================================================= MTGP32
+ * One work-group of 256 per stream, 768-word status ring in LDS; each round
+ * produces 768 words in three 256-wide phases, each one the recursion
+ * x[n] = rec(x[n-351], x[n-350], x[n-351+pos]) for 256 consecutive n
+ * (mtgp32-opencl.cl:186-242).  Output is the sequential stream of the
+ * stream's parameter set, continuing across launches through `status`. */
+__global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ params, uint32_t mask,
+                                              uint32_t* __restrict__ status,
+                                              const esth_mtgp_stream* __restrict__ streams,
+                                              uint32_t* __restrict__ out, int float_mode) {
+    __shared__ uint32_t st[ESTH_MTGP_LARGE];
+    __shared__ uint32_t tbl[16], ttbl[16];
+    const int tid = threadIdx.x;
+    const esth_mtgp_stream sd = streams[blockIdx.x];
+    const uint32_t* p = params + (size_t)sd.set * ESTH_MTGP_PARAM_WORDS;
+    const int pos = (int)p[0];
+    const uint32_t sh1 = p[1], sh2 = p[2];
+    if (tid < 16) {
+        tbl[tid] = p[4 + tid];
+        ttbl[tid] = float_mode ? p[36 + tid] : p[20 + tid];
+    }
+    uint32_t* gst = status + (size_t)blockIdx.x * ESTH_MTGP_N;
+    st[ESTH_MTGP_LARGE - ESTH_MTGP_N + tid] = gst[tid];
+    if (tid < ESTH_MTGP_N - ESTH_MTGP_THREADS)
+        st[ESTH_MTGP_LARGE - ESTH_MTGP_N + ESTH_MTGP_THREADS + tid] = gst[ESTH_MTGP_THREADS + tid];
+    __syncthreads();
+    uint32_t* o = out + sd.out_off;
+    const int size = (int)sd.size;
+    /* start offsets (mod 768) of x[n-351] for the three phases */
+    const int b0 = ESTH_MTGP_LARGE - ESTH_MTGP_N;          /* 417 */
+    const int b1 = 4 * ESTH_MTGP_THREADS - ESTH_MTGP_N;    /* 673 */
+    const int b2 = 2 * ESTH_MTGP_THREADS - ESTH_MTGP_N;    /* 161 */
+    for (int i = 0; i < size; i += ESTH_MTGP_LARGE) {
+#pragma unroll
+        for (int ph = 0; ph < 3; ++ph) {
+            const int b = ph == 0 ? b0 : (ph == 1 ? b1 : b2);
+            int i0 = b + tid, i1 = b + tid + 1, ip = b + tid + pos, it = b + tid + pos - 1;
+            if (ph == 1) { i0 %= ESTH_MTGP_LARGE; i1 %= ESTH_MTGP_LARGE; ip %= ESTH_MTGP_LARGE; it %= ESTH_MTGP_LARGE; }
+            uint32_t X = (st[i0] & mask) ^ st[i1];
+            X ^= X << sh1;
+            uint32_t Y = X ^ (st[ip] >> sh2);
+            uint32_t r = Y ^ tbl[Y & 0x0f];
+            st[ph * ESTH_MTGP_THREADS + tid] = r;
+            uint32_t T = st[it];
+            T ^= T >> 16;
+            T ^= T >> 8;
+            uint32_t v = float_mode ? ((r >> 9) ^ ttbl[T & 0x0f]) : (r ^ ttbl[T & 0x0f]);
+            o[i + ph * ESTH_MTGP_THREADS + tid] = v;
+            __syncthreads();
+        }
+    }
+    gst[tid] = st[ESTH_MTGP_LARGE - ESTH_MTGP_N + tid];
+    if (tid < ESTH_MTGP_N - ESTH_MTGP_THREADS)
+        gst[ESTH_MTGP_THREADS + tid] = st[4 * ESTH_MTGP_THREADS - ESTH_MTGP_N + tid];
+}
+
+/* ============================================================ Box-Muller */
+__global__ void k_boxmuller(float* __restrict__ d, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (2 * i < n) {
+        float z0, z1;
+        esth_boxmuller_pair(d[2 * i], d[2 * i + 1], &z0, &z1);
+        d[2 * i] = z0;
+        d[2 * i + 1] = z1;
+    }
+}
+
+/* ========================================================= sample+weight
+ * One thread per particle pair (p, p+1), p even.  The reference transforms
+ * uniform pairs (2j, 2j+1) of the buffer in place and particle i reads the
+ * normal at k*N + i; with N even the pair at k*N + p gives particle p its
+ * cosine half and particle p+1 its sine half, so the transform is fused here
+ * and the normals never touch HBM. */
+template <int MODEL>
+struct esth_model_traits;
+template <>
+struct esth_model_traits<ESTH_MODEL_ROBOTARM5> {
+    static constexpr int S = ESTH_RA5_S, V = ESTH_RA5_V;
+    __device__ static float iw(float* s, const float* z, const float* fr) { return esth_robotarm5_sampling_iw(s, z, fr); }
+};
+template <>
+struct esth_model_traits<ESTH_MODEL_UNICYCLE> {
+    static constexpr int S = ESTH_UNI_S, V = ESTH_UNI_V;
+    __device__ static float iw(float* s, const float* z, const float* fr) { return esth_unicycle_sampling_iw(s, z, fr); }
+};
+template <>
+struct esth_model_traits<ESTH_MODEL_DH6> {
+    static constexpr int S = ESTH_DH6_S, V = ESTH_DH6_V;
+    __device__ static float iw(float* s, const float* z, const float* fr) { return esth_dh6_sampling_iw(s, z, fr); }
+};
+
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state, uint32_t N,
+                                                       const float* __restrict__ R,
+                                                       const float* __restrict__ frame, int log_w,
+                                                       float* __restrict__ w) {
+    using T = esth_model_traits<MODEL>;
+    const uint32_t p = 2u * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= N) return;
+    float s0[T::S], s1[T::S], z0[T::V], z1[T::V];
+#pragma unroll
+    for (int c = 0; c < T::S; ++c) {
+        float2 v = *reinterpret_cast<const float2*>(state + (size_t)c * N + p);
+        s0[c] = v.x;
+        s1[c] = v.y;
+    }
+#pragma unroll
+    for (int k = 0; k < T::V; ++k) {
+        float2 u = *reinterpret_cast<const float2*>(R + (size_t)k * N + p);
+        esth_boxmuller_pair(u.x, u.y, &z0[k], &z1[k]);
+    }
+    float lw0 = T::iw(s0, z0, frame);
+    float lw1 = T::iw(s1, z1, frame);
+#pragma unroll
+    for (int c = 0; c < T::S; ++c)
+        *reinterpret_cast<float2*>(state + (size_t)c * N + p) = make_float2(s0[c], s1[c]);
+    float2 wo = log_w ? make_float2(lw0, lw1) : make_float2(esth_expf(lw0), esth_expf(lw1));
+    *reinterpret_cast<float2*>(w + p) = wo;
+}
+
+/* ============================================================ block sort
+ * The reference's descending bitonic network (bitonic_sort.cl:28-61 with
+ * dir = 0): for size = 2..m, stride = size/2..1, pair (pos, pos+stride) with
+ * pos & stride == 0 swaps iff (w[pos] > w[pos+stride]) == ddd, where
+ * ddd = (pos & size) != 0 for size < m and 0 in the final merge.  Keys equal
+ * to each other therefore swap in descending segments, exactly as there. */
+template <int LOGM>
+struct esth_tile {
+    static constexpr int M = 1 << LOGM;
+    static constexpr int E = M >= ESTH_WAVE ? M / ESTH_WAVE : 1;   /* elements per lane */
+    static constexpr int LPS = M / E;                              /* lanes per sub-filter */
+    static constexpr int LOGE = LOGM >= 6 ? LOGM - 6 : 0;
+    static constexpr int LOGL = LOGM - LOGE;
+};
+
+template <int LOGM>
+__device__ __forceinline__ void esth_bitonic(float (&key)[esth_tile<LOGM>::E],
+                                             int (&idx)[esth_tile<LOGM>::E], int le) {
+    using TL = esth_tile<LOGM>;
+    constexpr int M = TL::M, E = TL::E;
+#pragma unroll
+    for (int size = 2; size <= M; size <<= 1) {
+#pragma unroll
+        for (int stride = size / 2; stride > 0; stride >>= 1) {
+            if (stride < E) {
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    if ((r & stride) == 0) {
+                        const int a = r, b = r + stride;
+                        const bool d = (size == M) ? false : (((le + a) & size) != 0);
+                        if ((key[a] > key[b]) == d) {
+                            float tk = key[a]; key[a] = key[b]; key[b] = tk;
+                            int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;
+                        }
+                    }
+                }
+            } else {
+                const int lm = stride / E;
+                const bool lower = ((le / E) & lm) == 0;
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    const float other = esth_shfl_xor(key[r], lm);
+                    const int oidx = esth_shfl_xor_i(idx[r], lm);
+                    const bool d = (size == M) ? false : (((le + r) & size) != 0);
+                    const float kl = lower ? key[r] : other;
+                    const float ku = lower ? other : key[r];
+                    if ((kl > ku) == d) { key[r] = other; idx[r] = oidx; }
+                }
+            }
+        }
+    }
+}
+
+/* Sorted weights in place, gathered state to `sorted`, winner (w[0], g),
+ * compact best t, and the posterior partial sums of the sub-filter. */
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float* __restrict__ st_in,
+                                              float* __restrict__ st_out, uint32_t N, uint32_t G,
+                                              float2* __restrict__ winner, int* __restrict__ perm,
+                                              uint32_t t, float* __restrict__ bt_w,
+                                              float* __restrict__ bt_s, float* __restrict__ part,
+                                              int log_w) {
+    using TL = esth_tile<LOGM>;
+    constexpr int M = TL::M, E = TL::E, LPS = TL::LPS;
+    const int lane = threadIdx.x & (ESTH_WAVE - 1);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE;
+    const uint32_t g = wave * (ESTH_WAVE / LPS) + (uint32_t)(lane / LPS);
+    if (g >= G) return;   /* a whole sub-filter leaves together */
+    const int le = (lane % LPS) * E;
+    const size_t base = (size_t)g * M;
+    float key[E];
+    int idx[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) { key[r] = w[base + le + r]; idx[r] = le + r; }
+    esth_bitonic<LOGM>(key, idx, le);
+#pragma unroll
+    for (int r = 0; r < E; ++r) w[base + le + r] = key[r];
+    if (perm) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) perm[base + le + r] = idx[r];
+    }
+    /* gather (particle_set of block_sort, pf_generic.cl:281-283) and
+     * posterior partial sums: block max M_g = key of position 0 */
+    const int lead = lane - (lane % LPS);
+    const float mx = esth_shfl_idx(key[0], lead);
+    float wt[E];
+    float sw = 0.0f;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        wt[r] = log_w ? esth_expf(key[r] - mx) : key[r];
+        sw = sw + wt[r];
+    }
+    float sx[S];
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+        const float* src = st_in + (size_t)c * N + base;
+        float* dst = st_out + (size_t)c * N + base;
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            float v = src[idx[r]];
+            dst[le + r] = v;
+            acc = esth_fma(wt[r], v, acc);
+            if (t > 0 && (uint32_t)(le + r) < t) bt_s[(size_t)c * G * t + (size_t)g * t + le + r] = v;
+        }
+        sx[c] = acc;
+    }
+    if (t > 0) {
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+            if ((uint32_t)(le + r) < t) bt_w[(size_t)g * t + le + r] = key[r];
+    }
+#pragma unroll
+    for (int k = 1; k < LPS; k <<= 1) {
+        sw = sw + esth_shfl_xor(sw, k);
+#pragma unroll
+        for (int c = 0; c < S; ++c) sx[c] = sx[c] + esth_shfl_xor(sx[c], k);
+    }
+    if (le == 0) {
+        winner[g] = make_float2(key[0], (float)g);
+        float* pg = part + (size_t)g * (2 + S);
+        pg[0] = log_w ? mx : 0.0f;
+        pg[1] = sw;
+#pragma unroll
+        for (int c = 0; c < S; ++c) pg[2 + c] = sx[c];
+    }
+}
+
+/* ============================================================== argmax
+ * reduce (reduce.cl:19-55) as one work-group of 128: strided pass (later
+ * element wins ties), then the tree (`other` wins ties).  The same group then
+ * combines the sub-filters' posterior partial sums in double and copies the
+ * best particle (rank 0 of the winning sub-filter, read before any exchange).
+ * est (doubles): [0] max log-weight, [1] sum w, [2..2+S) sum w x, [2+S] best w,
+ * [3+S] best g, [4+S..4+2S) best particle. */
+template <int S>
+__global__ __launch_bounds__(128) void k_argmax(const float2* __restrict__ winner, uint32_t G,
+                                                int log_w, const float* __restrict__ part,
+                                                const float* __restrict__ sorted, uint32_t N,
+                                                uint32_t m, float2* __restrict__ result,
+                                                double* __restrict__ est) {
+    __shared__ float2 sc[128];
+    __shared__ double sd[128][2 + S];
+    __shared__ float smax[128];
+    const int l = threadIdx.x;
+    float2 acc = make_float2(log_w ? -INFINITY : -1.0f, -1.0f);
+    float bmax = -INFINITY;
+    for (uint32_t gi = (uint32_t)l; gi < G; gi += 128) {
+        float2 e = winner[gi];
+        acc = (acc.x > e.x) ? acc : e;
+        float pm = part[(size_t)gi * (2 + S)];
+        bmax = pm > bmax ? pm : bmax;
+    }
+    sc[l] = acc;
+    smax[l] = bmax;
+    __syncthreads();
+    for (int off = 64; off > 0; off >>= 1) {
+        if (l < off) {
+            float2 other = sc[l + off];
+            float2 mine = sc[l];
+            sc[l] = (mine.x > other.x) ? mine : other;
+            float a = smax[l], b = smax[l + off];
+            smax[l] = a > b ? a : b;
+        }
+        __syncthreads();
+    }
+    const float gmax = log_w ? smax[0] : 0.0f;
+    double a[2 + S];
+#pragma unroll
+    for (int c = 0; c < 2 + S; ++c) a[c] = 0.0;
+    for (uint32_t gi = (uint32_t)l; gi < G; gi += 128) {
+        const float* pg = part + (size_t)gi * (2 + S);
+        double scale = log_w ? (double)esth_expf(pg[0] - gmax) : 1.0;
+#pragma unroll
+        for (int c = 1; c < 2 + S; ++c) a[c] += (double)pg[c] * scale;
+    }
+#pragma unroll
+    for (int c = 0; c < 2 + S; ++c) sd[l][c] = a[c];
+    __syncthreads();
+    for (int off = 64; off > 0; off >>= 1) {
+        if (l < off) {
+#pragma unroll
+            for (int c = 1; c < 2 + S; ++c) sd[l][c] += sd[l + off][c];
+        }
+        __syncthreads();
+    }
+    if (l == 0) {
+        result[0] = sc[0];
+        est[0] = (double)gmax;
+#pragma unroll
+        for (int c = 1; c < 2 + S; ++c) est[c] = sd[0][c];
+        est[2 + S] = (double)sc[0].x;
+        est[3 + S] = (double)sc[0].y;
+    }
+    if (sorted && l < S) {
+        int gb = (int)sc[0].y;
+        if (gb >= 0 && (uint32_t)gb < G) est[4 + S + l] = (double)sorted[(size_t)l * N + (size_t)gb * m];
+    }
+}
+
+/* ============================================================= exchange
+ * Receiver-side gather from the compact best-t buffer, so every read sees
+ * the pre-exchange sorted state whatever the launch order:
+ *  ring  : slot m-1-k <- block (b-1) mod G rank k (pf_generic.cl:368-383)
+ *  torus : slot m-1-k-q*t <- rank k of the right (q=0), left (1), lower (2),
+ *          upper (3) neighbour: the receiving side of exchange_2dtorus
+ *          (cuda/pf_generic.cu:352-393) with in-block offsets
+ *  all2all: slot m-1-k <- global top-t candidate k */
+__global__ void k_exchange(float* __restrict__ st, float* __restrict__ w, int S, uint32_t N,
+                           uint32_t m, uint32_t G, uint32_t t, int topo, int dimx, int dimy,
+                           const float* __restrict__ bt_w, const float* __restrict__ bt_s,
+                           const uint32_t* __restrict__ top) {
+    const uint32_t nq = (topo == ESTH_TOPO_TORUS) ? 4u : 1u;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = (uint64_t)G * nq * t;
+    if (i >= total) return;
+    const uint32_t k = (uint32_t)(i % t);
+    const uint32_t q = (uint32_t)((i / t) % nq);
+    const uint32_t b = (uint32_t)(i / ((uint64_t)t * nq));
+    size_t src;
+    if (topo == ESTH_TOPO_RING) {
+        src = (size_t)((b + G - 1) % G) * t + k;
+    } else if (topo == ESTH_TOPO_ALL2ALL) {
+        src = top[k];
+    } else {
+        int x = (int)(b % (uint32_t)dimx), y = (int)(b / (uint32_t)dimx);
+        int sx = x, sy = y;
+        if (q == 0) sx = (x + 1) % dimx;
+        else if (q == 1) sx = (x - 1 + dimx) % dimx;
+        else if (q == 2) sy = (y + 1) % dimy;
+        else sy = (y - 1 + dimy) % dimy;
+        src = (size_t)(sy * dimx + sx) * t + k;
+    }
+    const size_t dst = (size_t)b * m + (m - 1 - k - q * t);
+    w[dst] = bt_w[src];
+    const size_t Gt = (size_t)G * t;
+    for (int c = 0; c < S; ++c) st[(size_t)c * N + dst] = bt_s[(size_t)c * Gt + src];
+}
+
+/* Global top-t of the G*t candidates by (weight desc, index asc): t rounds,
+ * each selecting the best candidate strictly after the previous one. */
+__global__ __launch_bounds__(1024) void k_topt(const float* __restrict__ bt_w, uint32_t n, uint32_t t,
+                                               uint32_t* __restrict__ top) {
+    __shared__ float sw[1024];
+    __shared__ uint32_t si[1024];
+    const int l = threadIdx.x;
+    float pw = INFINITY;
+    uint32_t pi = 0;
+    bool have_prev = false;
+    for (uint32_t k = 0; k < t; ++k) {
+        float bw = -INFINITY;
+        uint32_t bi = 0xffffffffu;
+        for (uint32_t c = (uint32_t)l; c < n; c += 1024) {
+            float v = bt_w[c];
+            bool after = !have_prev || (v < pw) || (v == pw && c > pi);
+            bool better = (bi == 0xffffffffu) || (v > bw) || (v == bw && c < bi);
+            if (after && better) { bw = v; bi = c; }
+        }
+        sw[l] = bw;
+        si[l] = bi;
+        __syncthreads();
+        for (int off = 512; off > 0; off >>= 1) {
+            if (l < off) {
+                float ow = sw[l + off];
+                uint32_t oi = si[l + off];
+                bool take = (oi != 0xffffffffu) &&
+                            ((si[l] == 0xffffffffu) || ow > sw[l] || (ow == sw[l] && oi < si[l]));
+                if (take) { sw[l] = ow; si[l] = oi; }
+            }
+            __syncthreads();
+        }
+        pw = sw[0];
+        pi = si[0];
+        have_prev = true;
+        if (l == 0) top[k] = pi;
+        __syncthreads();
+    }
+}
+
+/* ============================================ resampling (prefix sum)
+ * resampling (pf_generic.cl:385-432): gate, exclusive scan of the sub-filter's
+ * weights, selection, binary search, gather.  The scan reproduces the
+ * Blelloch tree's additions exactly: the up-sweep node over [j 2^k, (j+1) 2^k)
+ * is the sum of its two halves, and the exclusive prefix of element e is
+ * 0 + L_top + ... + L_0 taken top-down over the levels k where bit k of e is
+ * set, L_k being the left sibling's node.  Levels inside a lane come from a
+ * register tree, levels across lanes from xor butterflies (a + b == b + a in
+ * IEEE arithmetic, so both lanes of a pair hold the same node bits). */
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
+                                                  const float* __restrict__ st_tmp,
+                                                  const float* __restrict__ w,
+                                                  const float* __restrict__ rnd, uint32_t N, uint32_t G,
+                                                  float thr, int systematic, int log_w,
+                                                  int* __restrict__ sel) {
+    using TL = esth_tile<LOGM>;
+    constexpr int M = TL::M, E = TL::E, LPS = TL::LPS, LOGE = TL::LOGE, LOGL = TL::LOGL;
+    __shared__ float cdf_s[4][ESTH_WAVE * E];
+    const int lane = threadIdx.x & (ESTH_WAVE - 1);
+    const int wv = threadIdx.x / ESTH_WAVE;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE;
+    uint32_t g = wave * (ESTH_WAVE / LPS) + (uint32_t)(lane / LPS);
+    const bool valid = g < G;
+    if (!valid) g = G - 1;   /* keep the lane busy; its writes are dropped */
+    const int laneg = lane % LPS;
+    const int le = laneg * E;
+    const size_t base = (size_t)g * M;
+    const bool go = valid && !((rnd[N + g] - 1.0f) > thr);   /* gate, pf_generic.cl:403 */
+    float x[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) x[r] = w[base + le + r];
+    if (log_w) {
+        float mx = x[0];
+#pragma unroll
+        for (int r = 1; r < E; ++r) mx = x[r] > mx ? x[r] : mx;
+#pragma unroll
+        for (int k = 1; k < LPS; k <<= 1) {
+            float o = esth_shfl_xor(mx, k);
+            mx = o > mx ? o : mx;
+        }
+#pragma unroll
+        for (int r = 0; r < E; ++r) x[r] = esth_expf(x[r] - mx);
+    }
+    /* register tree: lvl[k][j], k = 0..LOGE */
+    float lvl[LOGE + 1][E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) lvl[0][r] = x[r];
+#pragma unroll
+    for (int k = 1; k <= LOGE; ++k)
+#pragma unroll
+        for (int j = 0; j < (E >> k); ++j) lvl[k][j] = lvl[k - 1][2 * j] + lvl[k - 1][2 * j + 1];
+    float node = lvl[LOGE][0];
+    float lx[LOGL > 0 ? LOGL : 1];
+#pragma unroll
+    for (int k = 0; k < LOGL; ++k) {
+        float sib = esth_shfl_xor(node, 1 << k);
+        lx[k] = sib;
+        node = node + sib;
+    }
+    const float total = node;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = LOGL - 1; k >= 0; --k)
+        if ((laneg >> k) & 1) acc = acc + lx[k];
+    float* cdf = &cdf_s[wv][(lane - laneg) * E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        float a = acc;
+#pragma unroll
+        for (int k = LOGE - 1; k >= 0; --k)
+            if ((r >> k) & 1) a = a + lvl[k][(r >> k) - 1];
+        cdf[le + r] = a;
+    }
+    __syncthreads();
+    const float u0 = rnd[base] - 1.0f;
+    const float step = total / (float)M;
+    int pick[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        float s;
+        if (systematic) s = ((float)(le + r) + u0) * step;
+        else s = (rnd[base + le + r] - 1.0f) * total;
+        int mn = 0, mxi = M - 1;          /* binary_search, pf_generic.cl:182-201 */
+        do {
+            int cur = mn + (mxi - mn + 1) / 2;
+            if (cdf[cur] > s) mxi = cur - 1;
+            else mn = cur;
+        } while (mn < mxi);
+        pick[r] = mn;
+    }
+    if (go) {
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+            const float* src = st_tmp + (size_t)c * N + base;
+            float* dst = st_main + (size_t)c * N + base;
+#pragma unroll
+            for (int r = 0; r < E; ++r) dst[le + r] = src[pick[r]];
+        }
+    }
+    if (sel && valid) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) sel[base + le + r] = go ? pick[r] : -1;
+    }
+}
+
+/* =========================================================== Vose alias
+ * resampling_valias (resampling-valias.cl:214-304): one work-group of m per
+ * sub-filter.  Normalisation sums with sum_reduce4's exact tree (:85-104);
+ * the worklists are filled in ascending work-item order (ballot ranks), the
+ * canonical order of the reference's local atomics (:124-126,175-178). */
+__device__ __forceinline__ float esth_sum_reduce4(float* arr, uint32_t n, int l) {
+    uint32_t offset = n;
+    while (offset > 2) {
+        offset >>= 2;
+        if ((uint32_t)l < offset)
+            arr[l] += arr[l + offset] + arr[l + offset + offset] + arr[l + offset + offset + offset];
+        __syncthreads();
+    }
+    if (offset > 1) {
+        if (l == 0) arr[0] += arr[1];
+        __syncthreads();
+    }
+    return arr[0];
+}
+
+/* rank of this lane among lanes with `pred` in ascending local id, and the
+ * total, over the whole work-group (cnt: LDS scratch of 16 uint). */
+__device__ __forceinline__ uint32_t esth_block_rank(bool pred, uint32_t* cnt, int l, int nw,
+                                                    uint32_t* total) {
+    const uint64_t bal = __ballot(pred);
+    const int lane = l & 63, wid = l >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t r = (uint32_t)__popcll(bal & lt);
+    if (lane == 0) cnt[wid] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int i = 0; i < nw; ++i) {
+        uint32_t c = cnt[i];
+        if (i < wid) before += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + r;
+}
+
+template <int S>
+__global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
+                                                const float* __restrict__ st_tmp,
+                                                const float* __restrict__ w,
+                                                const float* __restrict__ rnd, uint32_t N, uint32_t M,
+                                                float thr, int adaptive, int log_w,
+                                                int* __restrict__ sel, float* __restrict__ prob_out,
+                                                uint32_t* __restrict__ alias_out) {
+    __shared__ float prob[ESTH_MAX_NUM_PARTICLES];
+    __shared__ uint32_t alias[ESTH_MAX_NUM_PARTICLES];
+    __shared__ uint32_t sl[ESTH_MAX_NUM_PARTICLES];
+    __shared__ float scratch[ESTH_MAX_NUM_PARTICLES];
+    __shared__ uint32_t cnt[16];
+    __shared__ float red[16];
+    const int l = threadIdx.x;
+    const int nw = (int)((M + 63) / 64);
+    const uint32_t g = blockIdx.x;
+    const size_t idx = (size_t)g * M + (size_t)l;
+    const bool table_only = prob_out != nullptr;
+    if (!table_only && !adaptive) {
+        if ((rnd[N + g] - 1.0f) > thr) {
+            if (sel) sel[idx] = -1;
+            return;
+        }
+    }
+    float p = w[idx];
+    if (log_w) {
+        float mx = p;
+        for (int k = 1; k < 64; k <<= 1) {
+            float o = esth_shfl_xor(mx, k);
+            if ((l ^ k) < (int)M) mx = o > mx ? o : mx;
+        }
+        if ((l & 63) == 0) red[l >> 6] = mx;
+        __syncthreads();
+        mx = red[0];
+        for (int i = 1; i < nw; ++i) mx = red[i] > mx ? red[i] : mx;
+        p = esth_expf(p - mx);
+    }
+    scratch[l] = p;
+    __syncthreads();
+    float sum = esth_sum_reduce4(scratch, M, l);
+    if (sum > 0.0f) p = p / sum;
+    else p = 1.0f / (float)M;
+    prob[l] = p;
+    __syncthreads();   /* every item has read the sum before scratch is reused */
+    if (adaptive && !table_only) {
+        scratch[l] = p * p;
+        __syncthreads();
+        float recip_ess = esth_sum_reduce4(scratch, M, l);
+        if (!(recip_ess > thr)) {
+            if (sel) sel[idx] = -1;
+            return;
+        }
+    }
+    /* vose_alias_init, resampling-valias.cl:111-195 */
+    float pl = prob[l] * (float)M;
+    prob[l] = pl;
+    alias[l] = (uint32_t)l;
+    uint32_t nsmall;
+    const bool small = pl <= 1.0f;
+    uint32_t rk = esth_block_rank(small, cnt, l, nw, &nsmall);
+    uint32_t slot = small ? rk : (M - 1u - ((uint32_t)l - rk));
+    sl[slot] = (uint32_t)l;
+    uint32_t small_end = nsmall, small_begin = 0;
+    for (;;) {
+        __syncthreads();
+        uint32_t a = small_end - small_begin, b = M - small_end;
+        uint32_t nactive = a < b ? a : b;
+        if (nactive == 0) break;
+        const bool act = (uint32_t)l < nactive;
+        uint32_t si = 0, li = 0;
+        float pS = 0.0f, pL = 0.0f;
+        if (act) {
+            si = sl[small_begin + l];
+            li = sl[small_end + l];
+            pS = prob[si];
+            alias[si] = li;
+            pL = prob[li];
+        }
+        __syncthreads();
+        bool bs = false;
+        if (act) {
+            pL = (pL + pS) - 1.0f;
+            bs = pL <= 1.0f;
+        }
+        uint32_t nbs;
+        uint32_t rs = esth_block_rank(act && bs, cnt, l, nw, &nbs);
+        if (act) {
+            uint32_t s2 = bs ? (small_end + rs) : (small_end - 1u + nactive - ((uint32_t)l - rs));
+            sl[s2] = li;
+            prob[li] = pL;
+        }
+        small_end += nbs;
+        small_begin += nactive;
+    }
+    __syncthreads();
+    if (table_only) {
+        prob_out[idx] = prob[l];
+        alias_out[idx] = alias[l];
+        return;
+    }
+    /* vose_alias_generate_next (:198-202) on d_random2[idx] - 1 (:288-295) */
+    const float r0 = rnd[2 * idx] - 1.0f, r1 = rnd[2 * idx + 1] - 1.0f;
+    const uint32_t col = (uint32_t)floorf(r0 * (float)M);
+    const uint32_t pick = (r1 < prob[col]) ? col : alias[col];
+    const size_t sb = (size_t)g * M;
+#pragma unroll
+    for (int c = 0; c < S; ++c) st_main[(size_t)c * N + idx] = st_tmp[(size_t)c * N + sb + pick];
+    if (sel) sel[idx] = (int)pick;
+}
+
+/* ================================================================ math */
+__global__ void k_math(int fn, const float* __restrict__ x, const float* __restrict__ x2,
+                       float* __restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = x[i], r = 0.0f, s, c;
+    switch (fn) {
+    case 0: r = esth_expf(v); break;
+    case 1: r = esth_logf(v); break;
+    case 2: esth_sincosf(v, &s, &c); r = s; break;
+    case 3: esth_sincosf(v, &s, &c); r = c; break;
+    case 4: r = esth_atan2f(v, x2[i]); break;
+    default: r = 0.0f;
+    }
+    out[i] = r;
+}
+
+#endif /* ESTH_KERNELS_H */

@@ -1,0 +1,261 @@
+/*
+ * esth_models.h - model plug-ins: the per-particle sample+weight hook and the
+ * per-step frame builder, shared by the gfx950 kernels and the host.
+ *
+ * Hook contract (reference: src/models/robotarm/opencl/pf_model.cl:33-131):
+ *   sampling_iw(state, noise, frame) propagates one particle in place with
+ *   its standard normals and returns its log importance weight.  The engine
+ *   turns that into exp(lw) for the reference's linear weights
+ *   (pf_model.cl:129) or keeps it in the log domain.
+ *   particle_set (pf_model.cl:33-48) is the engine's generic per-plane copy;
+ *   state is structure-of-arrays, one float plane per state variable.
+ *
+ * Frame: everything a step's hook needs that is uniform over particles
+ * (dt, control, measurement, model constants), precomputed once per step on
+ * the host into ESTH_FRAME_FLOATS floats, read by the kernels with scalar
+ * loads.  Arithmetic is written operation by operation with explicit fmaf
+ * where a fused multiply-add is meant, so host and device agree bit for bit.
+ */
+#ifndef ESTH_MODELS_H
+#define ESTH_MODELS_H
+
+#include "esth_math.h"
+#include "esthera.h"
+
+/* ------------------------------------------------------------ robotarm5
+ * src/models/robotarm/opencl/pf_model.cl:52-131, pf_model.h:26-32.
+ * State planes: 0..4 joint angles, 5 x, 6 y, 7 vx, 8 vy.
+ * Noise k (d_random[k*N + idx], pf_model.cl:128-158): 0 x, 1 y, 2 vx, 3 vy,
+ * 4..8 joint angles. */
+#define ESTH_RA5_S 9
+#define ESTH_RA5_V 9
+/* frame: [0] dt, [1..5] control angles, [6..10] measured angles, [11] x, [12] y */
+#define ESTH_RA5_NOISE_ANGLES 0.2f   /* pf_model.cl:3 */
+#define ESTH_RA5_NOISE_X      0.2f
+#define ESTH_RA5_NOISE_Y      0.2f
+#define ESTH_RA5_NOISE_V_X    2.0f
+#define ESTH_RA5_NOISE_V_Y    2.0f
+#define ESTH_RA5_NOISE_ANGLE_MEASUREMENTS 0.1f  /* pf_model.h:28-30 */
+#define ESTH_RA5_NOISE_CAMERA_X 0.1f
+#define ESTH_RA5_NOISE_CAMERA_Y 0.1f
+
+ESTH_HD float esth_ra5_arm_length(int i) { return i == 0 ? 0.0f : 3.0f; } /* pf_model.cl:13 */
+
+ESTH_HD float esth_robotarm5_sampling_iw(float* s, const float* z, const float* fr) {
+    const float dt = fr[0];
+    float value = 0.0f;
+    /* position / velocity random walk, pf_model.cl:70-74 */
+    float x = s[5], y = s[6], vx = s[7], vy = s[8];
+    x = x + ((vx * dt) + (z[0] * ESTH_RA5_NOISE_X));
+    y = y + ((vy * dt) + (z[1] * ESTH_RA5_NOISE_Y));
+    vx = vx + (z[2] * ESTH_RA5_NOISE_V_X);
+    vy = vy + (z[3] * ESTH_RA5_NOISE_V_Y);
+    float camX, camY, camZ;
+    /* joint 0, pf_model.cl:76-90 */
+    {
+        float a = s[0] + ((fr[1] * dt) + (z[4] * ESTH_RA5_NOISE_ANGLES));
+        float sf, cf;
+        esth_sincosf(a, &sf, &cf);
+        camX = (x * cf) + (y * sf);
+        camY = ((-x) * sf) + (y * cf);
+        camZ = 0.0f;
+        float e = a - fr[6];
+        value = value + ((e * e) * ESTH_RA5_NOISE_ANGLE_MEASUREMENTS);
+        s[0] = a;
+    }
+    /* joints 1..4, pf_model.cl:91-117 */
+    for (int i = 1; i < 5; ++i) {
+        float a = s[i] + ((fr[1 + i] * dt) + (z[4 + i] * ESTH_RA5_NOISE_ANGLES));
+        float sf, cf;
+        esth_sincosf(a, &sf, &cf);
+        float e = a - fr[6 + i];
+        value = value + ((e * e) * ESTH_RA5_NOISE_ANGLE_MEASUREMENTS);
+        float nB = (camY * cf) - (camZ * sf);
+        float nC = (camY * sf) + (camZ * cf);
+        camY = nB;
+        camZ = nC - esth_ra5_arm_length(i);
+        s[i] = a;
+    }
+    s[5] = x; s[6] = y; s[7] = vx; s[8] = vy;
+    /* camera error, pf_model.cl:124-125 */
+    value = value + (((camX - fr[11]) * (camX - fr[11])) * ESTH_RA5_NOISE_CAMERA_X);
+    value = value + (((camY - fr[12]) * (camY - fr[12])) * ESTH_RA5_NOISE_CAMERA_Y);
+    return -value;
+}
+
+/* ------------------------------------------------------------- unicycle
+ * x += v dt cos(theta), y += v dt sin(theta), theta += omega dt, each plus
+ * additive Gaussian noise; measurement: range and bearing to each landmark.
+ * log w = -1/2 sum_j [ ((zr_j - r_j)/sigma_r)^2 + (wrap(zb_j - b_j)/sigma_b)^2 ],
+ * b_j = atan2(ly_j - y, lx_j - x) - theta.
+ * The bearing residual is taken as the angle of the landmark vector rotated
+ * by -(theta + zb_j), which lands in [-pi, pi] without a separate wrap.
+ * State planes: 0 x, 1 y, 2 theta.  Noise: 0 x, 1 y, 2 theta. */
+#define ESTH_UNI_S 3
+#define ESTH_UNI_V 3
+/* frame: [0] dt, [1] v*dt, [2] omega*dt, [3] sx, [4] sy, [5] stheta,
+ * [6] 0.5/sr^2, [7] 0.5/sb^2, [8] L, then at 16+5j: lx, ly, zr, cos zb, sin zb */
+#define ESTH_UNI_HDR 16
+
+ESTH_HD float esth_unicycle_sampling_iw(float* s, const float* z, const float* fr) {
+    float ct, st;
+    esth_sincosf(s[2], &st, &ct);
+    float x = s[0] + fr[1] * ct;
+    x = x + fr[3] * z[0];
+    float y = s[1] + fr[1] * st;
+    y = y + fr[4] * z[1];
+    float th = s[2] + fr[2];
+    th = th + fr[5] * z[2];
+    s[0] = x; s[1] = y; s[2] = th;
+    esth_sincosf(th, &st, &ct);
+    const int L = (int)fr[8];
+    float acc_r = 0.0f, acc_b = 0.0f;
+    for (int j = 0; j < L; ++j) {
+        const float* lm = fr + ESTH_UNI_HDR + 5 * j;
+        float dx = lm[0] - x;
+        float dy = lm[1] - y;
+        float r = sqrtf(esth_fma(dx, dx, dy * dy));
+        float er = lm[2] - r;
+        acc_r = esth_fma(er, er, acc_r);
+        float C = esth_fma(ct, lm[3], -(st * lm[4]));   /* cos(theta + zb) */
+        float S = esth_fma(st, lm[3], ct * lm[4]);      /* sin(theta + zb) */
+        float u = esth_fma(dx, C, dy * S);
+        float v = esth_fma(dy, C, -(dx * S));
+        float eb = esth_atan2f(v, u);
+        acc_b = esth_fma(eb, eb, acc_b);
+    }
+    return -esth_fma(acc_r, fr[6], acc_b * fr[7]);
+}
+
+/* ------------------------------------------------------------------ dh6
+ * 6-joint serial arm, Denavit-Hartenberg links A_i = Rz(q_i) Tz(d_i) Tx(a_i) Rx(alpha_i).
+ * Constant-velocity dynamics with Gaussian joint acceleration noise:
+ *   acc = sigma_acc z,  q += qd dt + acc dt^2/2,  qd += acc dt.
+ * Measurement: end-effector position (FK) with sigma_pos, joint encoders with sigma_enc.
+ * State planes: 0..5 q, 6..11 qd.  Noise: 0..5 joint accelerations. */
+#define ESTH_DH6_S 12
+#define ESTH_DH6_V 6
+/* frame: [0] dt, [1] dt^2/2, [2] sigma_acc, [3] 0.5/sp^2, [4] 0.5/se^2,
+ * [5..7] measured p, [8..13] measured q, then at 16+4i: a_i, d_i, cos alpha_i, sin alpha_i */
+#define ESTH_DH6_HDR 16
+
+ESTH_HD float esth_dh6_sampling_iw(float* s, const float* z, const float* fr) {
+    const float dt = fr[0];
+    for (int i = 0; i < 6; ++i) {
+        float acc = fr[2] * z[i];
+        float q = esth_fma(s[6 + i], dt, s[i]);
+        s[i] = esth_fma(fr[1], acc, q);
+        s[6 + i] = esth_fma(dt, acc, s[6 + i]);
+    }
+    /* end-effector position: p = A_0 (A_1 ( ... (A_5 0))) evaluated inside out */
+    float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+    for (int i = 5; i >= 0; --i) {
+        const float* lk = fr + ESTH_DH6_HDR + 4 * i;
+        float sq, cq;
+        esth_sincosf(s[i], &sq, &cq);
+        float w = esth_fma(lk[2], v1, -(lk[3] * v2));   /* ca v1 - sa v2 */
+        float u = v0 + lk[0];
+        float n2 = esth_fma(lk[3], v1, esth_fma(lk[2], v2, lk[1]));
+        v0 = esth_fma(cq, u, -(sq * w));
+        v1 = esth_fma(sq, u, cq * w);
+        v2 = n2;
+    }
+    float ep0 = fr[5] - v0, ep1 = fr[6] - v1, ep2 = fr[7] - v2;
+    float acc_p = esth_fma(ep0, ep0, esth_fma(ep1, ep1, ep2 * ep2));
+    float acc_q = 0.0f;
+    for (int i = 0; i < 6; ++i) {
+        float e = fr[8 + i] - s[i];
+        acc_q = esth_fma(e, e, acc_q);
+    }
+    return -esth_fma(acc_p, fr[3], acc_q * fr[4]);
+}
+
+/* ---------------------------------------------------------- frame build */
+ESTH_HD int esth_model_dims(int model, int* S, int* V) {
+    switch (model) {
+    case ESTH_MODEL_ROBOTARM5: *S = ESTH_RA5_S; *V = ESTH_RA5_V; return 0;
+    case ESTH_MODEL_UNICYCLE:  *S = ESTH_UNI_S; *V = ESTH_UNI_V; return 0;
+    case ESTH_MODEL_DH6:       *S = ESTH_DH6_S; *V = ESTH_DH6_V; return 0;
+    default: return -1;
+    }
+}
+
+ESTH_HD int esth_build_frame_impl(int model, const void* params, const void* control,
+                                  const void* measurement, float dt, float* fr) {
+    for (int i = 0; i < ESTH_FRAME_FLOATS; ++i) fr[i] = 0.0f;
+    fr[0] = dt;
+    if (model == ESTH_MODEL_ROBOTARM5) {
+        const esth_robotarm5_control* c = (const esth_robotarm5_control*)control;
+        const esth_robotarm5_measurement* m = (const esth_robotarm5_measurement*)measurement;
+        if (!c || !m) return -1;
+        for (int i = 0; i < 5; ++i) { fr[1 + i] = c->angles[i]; fr[6 + i] = m->angles[i]; }
+        fr[11] = m->x;
+        fr[12] = m->y;
+        return 0;
+    }
+    if (model == ESTH_MODEL_UNICYCLE) {
+        const esth_unicycle_params* p = (const esth_unicycle_params*)params;
+        const esth_unicycle_control* c = (const esth_unicycle_control*)control;
+        const esth_unicycle_measurement* m = (const esth_unicycle_measurement*)measurement;
+        if (!p || !c || !m) return -1;
+        if (p->num_landmarks < 0 || p->num_landmarks > ESTH_UNICYCLE_MAX_LANDMARKS) return -1;
+        if (!(p->sigma_range > 0.0f) || !(p->sigma_bearing > 0.0f)) return -1;
+        fr[1] = c->v * dt;
+        fr[2] = c->omega * dt;
+        fr[3] = p->sigma_x;
+        fr[4] = p->sigma_y;
+        fr[5] = p->sigma_theta;
+        fr[6] = 0.5f / (p->sigma_range * p->sigma_range);
+        fr[7] = 0.5f / (p->sigma_bearing * p->sigma_bearing);
+        fr[8] = (float)p->num_landmarks;
+        for (int j = 0; j < p->num_landmarks; ++j) {
+            float* lm = fr + ESTH_UNI_HDR + 5 * j;
+            float sb, cb;
+            esth_sincosf(m->bearing[j], &sb, &cb);
+            lm[0] = p->lx[j];
+            lm[1] = p->ly[j];
+            lm[2] = m->range[j];
+            lm[3] = cb;
+            lm[4] = sb;
+        }
+        return 0;
+    }
+    if (model == ESTH_MODEL_DH6) {
+        const esth_dh6_params* p = (const esth_dh6_params*)params;
+        const esth_dh6_measurement* m = (const esth_dh6_measurement*)measurement;
+        if (!p || !m) return -1;
+        if (!(p->sigma_pos > 0.0f) || !(p->sigma_enc > 0.0f)) return -1;
+        fr[1] = 0.5f * (dt * dt);
+        fr[2] = p->sigma_acc;
+        fr[3] = 0.5f / (p->sigma_pos * p->sigma_pos);
+        fr[4] = 0.5f / (p->sigma_enc * p->sigma_enc);
+        for (int i = 0; i < 3; ++i) fr[5 + i] = m->p[i];
+        for (int i = 0; i < 6; ++i) fr[8 + i] = m->q[i];
+        for (int i = 0; i < 6; ++i) {
+            float* lk = fr + ESTH_DH6_HDR + 4 * i;
+            float sa, ca;
+            esth_sincosf(p->alpha[i], &sa, &ca);
+            lk[0] = p->a[i];
+            lk[1] = p->d[i];
+            lk[2] = ca;
+            lk[3] = sa;
+        }
+        return 0;
+    }
+    return -1;
+}
+
+/* Box-Muller of one uniform pair in [1,2) (pf_generic.cl:465-483):
+ * u1 = 2 - x0 in (0,1], u2 = 2 - x1, r = sqrt(-2 ln u1), phi = 2 pi u2. */
+ESTH_HD void esth_boxmuller_pair(float x0, float x1, float* z0, float* z1) {
+    float u1 = 2.0f - x0;
+    float u2 = 2.0f - x1;
+    float r = sqrtf(-2.0f * esth_logf(u1));
+    float phi = ESTH_2PI * u2;
+    float sp, cp;
+    esth_sincosf(phi, &sp, &cp);
+    *z0 = r * cp;
+    *z1 = r * sp;
+}
+
+#endif /* ESTH_MODELS_H */

@@ -1,0 +1,349 @@
+/*
+ * esthera.h - C ABI of the MI355X (gfx950) particle-filter engine.
+ *
+ * The engine runs the island-model filter step of the reference
+ * (src/partfilt/opencl/pf_generic.c:438-808):
+ *
+ *   MTGP32 words -> Box-Muller -> sample+weight (model hook) -> per-sub-filter
+ *   bitonic sort + gather -> argmax of sub-filter winners -> exchange of best
+ *   particles (ring / 2-D torus / all-to-all) -> per-sub-filter resampling
+ *   (prefix-sum multinomial, systematic, or Vose alias)
+ *
+ * as hand-written HIP kernels.  Everything crossing this boundary is plain C:
+ * integers, floats, host pointers and device pointers obtained from
+ * esth_dev_alloc().  Every function returns an int status (ESTH_OK == 0) and
+ * never exits the process; esth_last_error() returns a thread-local message.
+ *
+ * Threading: a filter, batch or server handle is used by one host thread at a
+ * time; distinct handles may be driven from different threads.
+ *
+ * Citations name the reference interface each entry point replaces.
+ */
+#ifndef ESTHERA_H
+#define ESTHERA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESTH_VERSION 100
+
+/* ---------------------------------------------------------------- status */
+#define ESTH_OK               0
+#define ESTH_ERR_INVALID     -1  /* bad argument or descriptor */
+#define ESTH_ERR_NOMEM       -2  /* host or device allocation failed */
+#define ESTH_ERR_DEVICE      -3  /* HIP runtime error */
+#define ESTH_ERR_BUDGET      -4  /* request exceeds the engine's HBM budget */
+#define ESTH_ERR_UNSUPPORTED -5  /* combination not built */
+#define ESTH_ERR_IO          -6  /* trace file error */
+#define ESTH_ERR_STATE       -7  /* handle in the wrong state for the call */
+#define ESTH_ERR_AGAIN       -8  /* nothing ready yet (polling) */
+
+/* Thread-local message for the last failing call of this thread. */
+const char* esth_last_error(void);
+int esth_version(void);
+
+/* ------------------------------------------------------------ limits
+ * Reference limits: pf_model.h:22-24 (MAX_NUM_PARTICLES 1024,
+ * MAX_NUM_BLOCKS 65536, MAX_NUM_TRANSFER = 512), checked at pf_generic.c:1333-1342. */
+#define ESTH_MAX_NUM_PARTICLES 1024
+#define ESTH_MAX_NUM_BLOCKS    65536
+#define ESTH_MAX_NUM_TRANSFER  512
+#define ESTH_MAX_STATE_DIM     16
+#define ESTH_FRAME_FLOATS      336   /* per-step model frame, see esth_model_build_frame */
+
+/* ---------------------------------------------------------------- models
+ * Compiled-in model plug-ins.  Each provides the reference's two hooks:
+ * particle_set (copy one particle, pf_model.cl:33-48) and sampling_iw
+ * (propagate with noise + importance weight, pf_model.cl:52-131).
+ *
+ *  robotarm5 : the reference's planar 5-joint arm on a moving base
+ *              (src/models/robotarm/opencl/pf_model.{h,cl}); state
+ *              a0..a4, x, y, vx, vy (9); 9 normals per particle.
+ *  unicycle  : unicycle with range+bearing landmarks; state x, y, theta (3);
+ *              3 normals per particle.
+ *  dh6       : 6-joint Denavit-Hartenberg arm, constant-velocity dynamics;
+ *              state q0..q5, qd0..qd5 (12); 6 normals per particle.
+ */
+#define ESTH_MODEL_ROBOTARM5 0
+#define ESTH_MODEL_UNICYCLE  1
+#define ESTH_MODEL_DH6       2
+#define ESTH_NUM_MODELS      3
+
+/* robotarm5 (pf_model.h:34-56) */
+typedef struct { float angles[5]; } esth_robotarm5_control;
+typedef struct { float angles[5]; float x, y; } esth_robotarm5_measurement;
+/* robotarm5 takes no parameters (its constants are fixed, pf_model.cl:1-13). */
+
+/* unicycle */
+#define ESTH_UNICYCLE_MAX_LANDMARKS 64
+typedef struct {
+    int   num_landmarks;
+    float sigma_x, sigma_y, sigma_theta;   /* additive process noise */
+    float sigma_range, sigma_bearing;      /* measurement noise */
+    float lx[ESTH_UNICYCLE_MAX_LANDMARKS];
+    float ly[ESTH_UNICYCLE_MAX_LANDMARKS];
+} esth_unicycle_params;
+typedef struct { float v, omega; } esth_unicycle_control;
+typedef struct {
+    float range[ESTH_UNICYCLE_MAX_LANDMARKS];
+    float bearing[ESTH_UNICYCLE_MAX_LANDMARKS];
+} esth_unicycle_measurement;
+
+/* dh6 */
+typedef struct {
+    float a[6], d[6], alpha[6];            /* DH link length, offset, twist */
+    float sigma_acc, sigma_pos, sigma_enc;
+} esth_dh6_params;
+typedef struct { float unused; } esth_dh6_control;
+typedef struct { float p[3]; float q[6]; } esth_dh6_measurement;
+
+/* Model id for "robotarm5" | "unicycle" | "dh6"; -1 if unknown. */
+int esth_model_lookup(const char* name);
+/* Sizes for a model: state floats per particle, normals per particle, and
+ * the byte sizes of its params / control / measurement structs. */
+int esth_model_info(int model, int* state_dim, int* noise_dim, size_t* params_size,
+                    size_t* control_size, size_t* measurement_size);
+/* Build the per-step model frame (ESTH_FRAME_FLOATS floats) that the kernels
+ * read: the model's params, control, measurement and dt folded into
+ * precomputed constants.  Exposed for the stage-level entry points. */
+int esth_model_build_frame(int model, const void* params, const void* control,
+                           const void* measurement, float dt, float* frame_out);
+
+/* -------------------------------------------------------------- filters */
+#define ESTH_TOPO_NONE     0
+#define ESTH_TOPO_RING     1   /* exchange_ring, pf_generic.cl:368-383 */
+#define ESTH_TOPO_TORUS    2   /* exchange_2dtorus, cuda/pf_generic.cu:352-393 (slot offsets fixed) */
+#define ESTH_TOPO_ALL2ALL  3   /* global top-t, cuda/pf_generic2_all2all.cu:404-419,1007 */
+
+#define ESTH_RESAMPLE_MULTINOMIAL 0  /* resampling, pf_generic.cl:385-432 */
+#define ESTH_RESAMPLE_SYSTEMATIC  1  /* same scan, one uniform per sub-filter */
+#define ESTH_RESAMPLE_ALIAS       2  /* resampling_valias, resampling-valias.cl:214-304 */
+
+#define ESTH_WEIGHTS_LINEAR 0  /* weight = exp(-value) as the reference (pf_model.cl:129) */
+#define ESTH_WEIGHTS_LOG    1  /* log-weight; resampling uses exp(lw - max) */
+
+/* RNG ownership.  Both produce MTGP32-11213 words with the reference's
+ * layout rule (stream j owns [j*size, (j+1)*size), seed+j+1, parameter set j,
+ * pf_generic.c:208-210,282; mtgp32-opencl.c:46-48):
+ *   COMPAT     : 96 streams, exactly the reference's buffer.
+ *   PER_FILTER : stream count chosen from the filter's own size, so a filter's
+ *                random numbers never depend on other filters in a batch. */
+#define ESTH_RNG_MTGP_COMPAT     0
+#define ESTH_RNG_MTGP_PER_FILTER 1
+
+typedef struct {
+    int      model;                /* ESTH_MODEL_* */
+    uint32_t num_particles;        /* m: particles per sub-filter, power of two in [2, 1024] */
+    uint32_t num_blocks;           /* G: sub-filters, in [1, 65536] */
+    uint32_t num_transfer;         /* t: particles sent per neighbour (0 = no exchange) */
+    int      topology;             /* ESTH_TOPO_* */
+    int      torus_dimx;           /* 0: reference rule dimx = 2^floor(log2 sqrt G) */
+    int      torus_dimy;           /* 0: G / dimx */
+    int      resampler;            /* ESTH_RESAMPLE_* */
+    float    resampling_threshold; /* gate: resample sub-filter g iff u_g <= thr (pf_generic.cl:403) */
+    int      adaptive;             /* alias only: resample iff sum p^2 > thr (resampling-valias.cl:273-284) */
+    int      weights;              /* ESTH_WEIGHTS_* */
+    int      rng;                  /* ESTH_RNG_* */
+    uint32_t seed;                 /* MTGP seed; stream j is seeded with seed + j + 1 */
+} esth_filter_desc;
+
+/* Fill a descriptor with the reference's defaults (pf_generic.c:1072-1084):
+ * m=256, G=16, t=1, ring, multinomial, threshold 1.0, linear weights. */
+void esth_filter_desc_default(esth_filter_desc* d, int model);
+
+typedef struct {
+    int      state_dim;
+    double   mean[ESTH_MAX_STATE_DIM];   /* posterior mean sum(w x)/sum(w) before resampling */
+    float    best[ESTH_MAX_STATE_DIM];   /* best particle of the best sub-filter (pf_generic.c:634-668) */
+    float    best_weight;                /* its (log-)weight */
+    int      best_block;                 /* its sub-filter */
+    double   log_weight_max;             /* max log-weight (log mode; 0 in linear mode) */
+    double   weight_sum;                 /* sum of weights scaled by exp(-log_weight_max) */
+    uint64_t step;                       /* steps completed */
+} esth_estimate;
+
+typedef struct esth_engine esth_engine;
+typedef struct esth_filter esth_filter;
+
+/* One engine per GPU (device ordinal).  hbm_budget bytes bounds the device
+ * memory the engine hands to filters, batches and servers (0 = 90% of free). */
+int esth_engine_create(int device, size_t hbm_budget, esth_engine** out);
+int esth_engine_destroy(esth_engine* e);
+int esth_engine_memory(const esth_engine* e, size_t* used, size_t* budget);
+
+/* Replaces the set-up half of particle_filter() (pf_generic.c:245-436). */
+int esth_filter_create(esth_engine* e, const esth_filter_desc* desc, const void* params,
+                       size_t params_size, esth_filter** out);
+int esth_filter_destroy(esth_filter* f);
+/* Set every particle to `state` (state_dim floats), as pf_generic.c:346-364 does. */
+int esth_filter_init_uniform(esth_filter* f, const float* state);
+/* Set all particles from host SoA data: state_dim planes of m*G floats. */
+int esth_filter_init(esth_filter* f, const float* state_soa);
+/* One filter step (pf_generic.c:438-808), enqueued on the filter's stream;
+ * control/measurement are the model's structs (copied before return). */
+int esth_filter_step(esth_filter* f, const void* control, const void* measurement, float dt);
+int esth_filter_sync(esth_filter* f);
+/* Estimate of the last completed step (waits for it). */
+int esth_filter_estimate(esth_filter* f, esth_estimate* out);
+/* Current particles (state_dim planes of m*G floats) and per-particle
+ * weights as left by the sort/exchange stages (sorted per sub-filter). */
+int esth_filter_read_particles(esth_filter* f, float* state_soa, float* weights);
+/* Per-stage device time of the last step in microseconds (hipEvent based):
+ * [rng, sample_weight, sort, argmax, exchange, resample]; needs
+ * esth_filter_set_timing(f, 1) and eager (non-graph) steps. */
+int esth_filter_set_timing(esth_filter* f, int on);
+int esth_filter_stage_times(esth_filter* f, float us[6]);
+/* Checkpoint / resume: particles, weights, RNG status and step counter. */
+int esth_filter_snapshot_size(esth_filter* f, size_t* bytes);
+int esth_filter_snapshot(esth_filter* f, void* buf, size_t bytes);
+int esth_filter_restore(esth_filter* f, const void* buf, size_t bytes);
+/* The HIP stream (hipStream_t) the filter's work is enqueued on. */
+int esth_filter_stream(esth_filter* f, void** hip_stream);
+/* Capture one step into a HIP graph and replay it for later steps with the
+ * same control/measurement layout (the frame buffer is updated per call). */
+int esth_filter_use_graph(esth_filter* f, int enable);
+
+/* -------------------------------------------------- reference driver
+ * int particle_filter(m, G, t, thr, file) (pf_generic.c:245-887) for the
+ * robotarm5 model over a recorded trace (pf_model_host.h:30-54).  Writes the
+ * mean over steps of estimate_error(best particle, truth) (pf_model_host.h:56-69),
+ * the DEBUG_ESTIMATE summary of pf_generic.c:814, to *mean_error. */
+int esth_particle_filter(uint32_t m, uint32_t G, uint32_t t, float thr, const char* trace_path,
+                         uint32_t seed, int resampler, double* mean_error, int* steps);
+
+/* Trace rows of the robot-arm format (20 columns): meas x y, 5 meas angles,
+ * 5 controls, dt, truth x y, 5 truth angles (pf_model_host.h:30-54).
+ * truth receives 9 floats per row (a0..a4, x, y, vx=0, vy=0). */
+int esth_trace_load_robotarm(const char* path, int max_rows, esth_robotarm5_measurement* meas,
+                             esth_robotarm5_control* ctrl, float* dt, float* truth, int* rows);
+/* estimate_error (pf_model_host.h:56-69) on robotarm5 states (9 floats). */
+float esth_robotarm5_estimate_error(const float* estimate, const float* actual);
+
+/* Seeded synthetic scenarios.
+ * unicycle: dt, v=1, omega_t = 0.2 sin(0.05 t); landmarks at (+-10,+-10) when
+ * num_landmarks == 4 and box == 0, else uniform in [-box, box]^2 from `seed`.
+ * truth: steps*3 floats (state after each step, starting from x0 = 0,0,0);
+ * measurements use the given noise levels; process noise is added to truth. */
+int esth_unicycle_scenario(uint32_t seed, int num_landmarks, float box, int steps, float dt,
+                           esth_unicycle_params* params, esth_unicycle_control* controls,
+                           esth_unicycle_measurement* meas, float* truth);
+/* dh6: DH parameters drawn from `seed` (a in [0.2,0.6], d in [-0.1,0.1],
+ * twists pi/2, 0, 0, pi/2, -pi/2, 0); truth (steps+1)*12 floats: row 0 the
+ * initial q, qd (also drawn from `seed`), row k the state after step k. */
+int esth_dh6_scenario(uint32_t seed, int steps, float dt, esth_dh6_params* params,
+                      esth_dh6_measurement* meas, float* truth);
+
+/* ------------------------------------------------------------ batches
+ * Many independent filters stepped together: every stage is one launch over
+ * all filters' sub-filters, the whole step captured in one HIP graph.
+ * Filters must use ESTH_RNG_MTGP_PER_FILTER and m >= 64. */
+typedef struct esth_batch esth_batch;
+int esth_batch_create(esth_engine* e, int num_filters, const esth_filter_desc* descs,
+                      const void* params, size_t params_stride, esth_batch** out);
+int esth_batch_destroy(esth_batch* b);
+int esth_batch_init_uniform(esth_batch* b, const float* states /* num_filters x state_dim */);
+/* controls / measurements: arrays of the model structs with the given strides. */
+int esth_batch_step(esth_batch* b, const void* controls, size_t control_stride,
+                    const void* measurements, size_t measurement_stride, const float* dts);
+int esth_batch_sync(esth_batch* b);
+int esth_batch_estimate(esth_batch* b, int filter, esth_estimate* out);
+int esth_batch_read_particles(esth_batch* b, int filter, float* state_soa, float* weights);
+int esth_batch_use_graph(esth_batch* b, int enable);
+/* Device time of the last step in microseconds. */
+int esth_batch_step_time(esth_batch* b, float* us);
+
+/* ------------------------------------------------------------ serving
+ * A request loop with continuous batching: tracks (filters) are opened and
+ * updated by requests; every admitted request's step runs in the next launch
+ * wave together with all others that arrived; device state stays resident;
+ * admission is bounded by the engine's HBM budget. */
+typedef struct esth_server esth_server;
+typedef struct {
+    uint64_t request_id;
+    uint64_t track_id;
+    int      status;         /* ESTH_OK, or why the request failed */
+    int      kind;           /* 0 open, 1 update, 2 close */
+    double   submit_us;      /* host clock at submit */
+    double   done_us;        /* host clock when the result was collected */
+    esth_estimate estimate;  /* for updates */
+} esth_result;
+int esth_server_create(esth_engine* e, int max_tracks, esth_server** out);
+int esth_server_destroy(esth_server* s);
+int esth_server_open(esth_server* s, const esth_filter_desc* desc, const void* params,
+                     size_t params_size, const float* init_state, uint64_t* track_id,
+                     uint64_t* request_id);
+int esth_server_update(esth_server* s, uint64_t track_id, const void* control,
+                       const void* measurement, float dt, uint64_t* request_id);
+int esth_server_close(esth_server* s, uint64_t track_id, uint64_t* request_id);
+/* Launch all queued work that fits (non-blocking), collect finished results. */
+int esth_server_pump(esth_server* s);
+int esth_server_poll(esth_server* s, esth_result* out, int max_results, int* n_out);
+int esth_server_drain(esth_server* s);
+int esth_server_stats(esth_server* s, size_t* hbm_used, int* open_tracks, int* queued);
+
+/* ------------------------------------------------- device memory helpers */
+int esth_set_device(int device);
+int esth_dev_alloc(size_t bytes, void** dptr);
+int esth_dev_free(void* dptr);
+int esth_memcpy_htod(void* dst, const void* src, size_t bytes);
+int esth_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int esth_memset_d(void* dst, int value, size_t bytes);
+int esth_dev_sync(void);
+
+/* ------------------------------------------------------------- stages
+ * Each stage on caller-provided device buffers, on the null stream, for
+ * stage-by-stage parity tests and benchmarks.  N = m*G. */
+
+/* MTGP32-11213 parameter table: 200 sets of 52 words
+ * (pos, sh1, sh2, mask, tbl[16], tmp_tbl[16], flt_tmp_tbl[16]). */
+const uint32_t* esth_mtgp_params(void);
+/* mtgp32_init_state (mtgp32-fast.c:119-137) for parameter set `set`: 351 words. */
+int esth_mtgp_seed_state(int set, uint32_t seed, uint32_t* state351);
+/* mtgp32_single_kernel / mtgp32_uint32_kernel (mtgp32-opencl.cl:125-242):
+ * streams j = 0..nstreams-1 use parameter set first_set + j, state
+ * d_status[j*351 .. +351], write `size` words to d_out[j*size ..];
+ * size must be a multiple of 768.  float_mode: 1 floats in [1,2), 0 uint32. */
+int esth_stage_mtgp(uint32_t* d_status, int nstreams, int first_set, int size, void* d_out,
+                    int float_mode);
+/* boxmuller (pf_generic.cl:465-483), in place on the first n floats. */
+int esth_stage_boxmuller(float* d_data, int n);
+/* sampling_importance (pf_generic.cl:206-250) with Box-Muller fused: particle
+ * i uses normals d_random[k*N + i] after the reference's in-place transform of
+ * the uniform pairs.  d_state: state_dim planes of N; d_w: N (log or linear). */
+int esth_stage_sample_weight(int model, float* d_state, uint32_t N, const float* d_random,
+                             const float* frame, int weights, float* d_w);
+/* block_sort (pf_generic.cl:252-291): sorts d_w per sub-filter descending in
+ * place with the reference's bitonic network, gathers state planes
+ * d_state_in -> d_state_out, writes d_winner[g] = (w[0], g) as float2 and,
+ * if d_perm != NULL, the permutation. */
+int esth_stage_sort(float* d_w, const float* d_state_in, float* d_state_out, int state_dim,
+                    uint32_t m, uint32_t G, float* d_winner, int* d_perm);
+/* reduce (reduce.cl:19-55): argmax over G float2 winners -> d_result float2. */
+int esth_stage_argmax(const float* d_winner, uint32_t G, int weights, float* d_result);
+/* exchange of best particles on sorted sub-filters (in place on d_state/d_w). */
+int esth_stage_exchange(float* d_state, float* d_w, int state_dim, uint32_t m, uint32_t G,
+                        uint32_t t, int topology, int dimx, int dimy);
+/* resampling (pf_generic.cl:385-432) / systematic: gate uniform
+ * d_random[N + g], selection uniforms d_random[i] (values in [1,2)).
+ * d_state_src: sorted state; d_state_dst: main state; d_sel (optional):
+ * selected source index per particle within its sub-filter. */
+int esth_stage_resample(int resampler, float* d_state_dst, const float* d_state_src,
+                        int state_dim, const float* d_w, const float* d_random, uint32_t m,
+                        uint32_t G, float threshold, int adaptive, int weights, int* d_sel);
+/* Vose alias tables only (resampling-valias.cl:111-195) for parity tests:
+ * per sub-filter prob[m] (scaled, after pairing) and alias[m]. */
+int esth_stage_alias_table(const float* d_w, uint32_t m, uint32_t G, int weights, float* d_prob,
+                           uint32_t* d_alias);
+/* Deterministic math functions evaluated on the device (n inputs): fn 0 exp,
+ * 1 log, 2 sin, 3 cos, 4 atan2(y=x[i], x=x2[i]). */
+int esth_stage_math(int fn, const float* d_x, const float* d_x2, float* d_out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESTHERA_H */

@@ -1,0 +1,158 @@
+"""Whole-step parity: the engine's filter step (all kernels, fused Box-Muller,
+device-resident state) against the oracle's restatement of the reference's
+step loop (pf_generic.c:438-808), on the same seed and inputs.  Particle
+states after every step are compared bit for bit; posterior means against the
+FP64 twin within the 1e-4 relative bar."""
+import os
+
+import numpy as np
+import pytest
+
+import esthera_amd as ea
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def make_case(model, steps):
+    if model == ea.MODEL_UNICYCLE:
+        p, ctrl, meas, truth = ea.unicycle_scenario(1234, 4, 0.0, steps)
+        return p, ctrl, meas, np.zeros(3, np.float32), truth, 0.1
+    if model == ea.MODEL_DH6:
+        p, meas, truth = ea.dh6_scenario(7, steps)
+        ctrl = [ea.DH6Control() for _ in range(steps)]
+        return p, ctrl, meas, truth[0].copy(), truth[1:], 0.01
+    meas, ctrl, dts, truth = ea.load_robotarm_trace(os.path.join(HERE, "golden", "robotarm_input5.txt"))
+    a0 = np.float32(0.75 * np.pi)
+    init = np.array([a0] * 5 + [7.0, 0.0, 0.0, 0.0], np.float32)
+    return None, ctrl, meas, init, truth, dts
+
+
+def run_pair(engine, desc, steps, check_every=1):
+    model = desc.model
+    params, ctrl, meas, init, truth, dt = make_case(model, steps)
+    f = ea.Filter(engine, desc, params)
+    f.init_uniform(init)
+    o = O.Filter(desc, params, 0 if params is None else __import__("ctypes").sizeof(params))
+    o.init_uniform(init)
+    S = ea.STATE_DIM[model]
+    worst = 0.0
+    for k in range(steps):
+        d = float(dt[k]) if np.ndim(dt) else dt
+        f.step(ctrl[k], meas[k], d)
+        o.step(ctrl[k], meas[k], d)
+        if k % check_every == 0 or k == steps - 1:
+            gst, gw = f.particles()
+            ost, ow = o.read(S)
+            assert np.array_equal(gst.view(np.uint32), ost.view(np.uint32)), f"state differs at step {k}"
+            assert np.array_equal(gw.view(np.uint32), ow.view(np.uint32)), f"weights differ at step {k}"
+            est = f.estimate()
+            m32, m64, best = o.estimate(S)
+            mean = np.array(est.mean[:S])
+            scale = np.maximum(np.abs(m64), 1.0)
+            worst = max(worst, float(np.max(np.abs(mean - m64) / scale)))
+            np.testing.assert_allclose(mean, m32, rtol=1e-6, atol=1e-6)
+            assert est.best_weight == best[0] and est.best_block == int(best[1])
+    return worst
+
+
+CASES = [
+    ("unicycle-multinomial-ring-log", dict(model=ea.MODEL_UNICYCLE, m=256, G=16, t=1, weights=ea.WEIGHTS_LOG)),
+    ("unicycle-systematic-torus-log", dict(model=ea.MODEL_UNICYCLE, m=256, G=16, t=1, topology=ea.TOPO_TORUS,
+                                           resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG)),
+    ("unicycle-alias-torus-log", dict(model=ea.MODEL_UNICYCLE, m=512, G=16, t=4, topology=ea.TOPO_TORUS,
+                                      resampler=ea.RESAMPLE_ALIAS, weights=ea.WEIGHTS_LOG)),
+    ("unicycle-all2all-gate", dict(model=ea.MODEL_UNICYCLE, m=128, G=32, t=2, topology=ea.TOPO_ALL2ALL,
+                                   threshold=0.5, weights=ea.WEIGHTS_LOG)),
+    ("robotarm-reference-linear", dict(model=ea.MODEL_ROBOTARM5, m=64, G=16, t=1)),
+    ("robotarm-alias-adaptive", dict(model=ea.MODEL_ROBOTARM5, m=256, G=8, t=1, resampler=ea.RESAMPLE_ALIAS,
+                                     adaptive=1, threshold=0.01)),
+    ("dh6-torus-log", dict(model=ea.MODEL_DH6, m=512, G=8, t=1, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG)),
+    ("unicycle-small-m", dict(model=ea.MODEL_UNICYCLE, m=16, G=12, t=2, weights=ea.WEIGHTS_LOG,
+                              rng=ea.RNG_MTGP_PER_FILTER)),
+]
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_filter_steps_bit_exact(name, kw, engine):
+    desc = ea.filter_desc(seed=1234, **kw)
+    worst = run_pair(engine, desc, 20)
+    assert worst < 1e-4, f"posterior mean vs FP64 oracle: {worst:.3e}"
+
+
+def test_graph_replay_equals_eager(engine):
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=512, G=64, t=4, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG,
+                          seed=42)
+    params, ctrl, meas, init, truth, dt = make_case(ea.MODEL_UNICYCLE, 12)
+    a, b = ea.Filter(engine, desc, params), ea.Filter(engine, desc, params)
+    a.init_uniform(init)
+    b.init_uniform(init)
+    b.use_graph(True)
+    for k in range(12):
+        a.step(ctrl[k], meas[k], dt)
+        b.step(ctrl[k], meas[k], dt)
+        if k % 3 == 2:
+            sa, wa = a.particles()
+            sb, wb = b.particles()
+            assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+            assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+
+
+def test_snapshot_restore_resumes_identically(engine):
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=16, t=1, weights=ea.WEIGHTS_LOG, seed=9)
+    params, ctrl, meas, init, truth, dt = make_case(ea.MODEL_UNICYCLE, 10)
+    a = ea.Filter(engine, desc, params)
+    a.init_uniform(init)
+    for k in range(4):
+        a.step(ctrl[k], meas[k], dt)
+    snap = a.snapshot()
+    for k in range(4, 10):
+        a.step(ctrl[k], meas[k], dt)
+    ref, _ = a.particles()
+    b = ea.Filter(engine, desc, params)
+    b.restore(snap)
+    for k in range(4, 10):
+        b.step(ctrl[k], meas[k], dt)
+    got, _ = b.particles()
+    assert np.array_equal(ref.view(np.uint32), got.view(np.uint32))
+
+
+def test_reference_driver_robotarm_input5(engine):
+    err, steps = ea.particle_filter(64, 16, 1, 1.0, os.path.join(HERE, "golden", "robotarm_input5.txt"))
+    assert steps == 100
+    assert np.isfinite(err) and err > 0
+
+
+def test_unicycle_tracks_truth(engine):
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=16, t=1, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG,
+                          seed=1234)
+    params, ctrl, meas, init, truth, dt = make_case(ea.MODEL_UNICYCLE, 100)
+    f = ea.Filter(engine, desc, params)
+    f.init_uniform(init)
+    err = []
+    for k in range(100):
+        f.step(ctrl[k], meas[k], dt)
+        est = f.estimate()
+        err.append(np.hypot(est.mean[0] - truth[k, 0], est.mean[1] - truth[k, 1]))
+    rmse = float(np.sqrt(np.mean(np.square(err))))
+    assert rmse < 0.2, rmse
+
+
+def test_c1_scale_properties(engine):
+    """The 2M-particle configuration (64x64 torus of 512): one step, checked
+    through properties that do not need the oracle at this size."""
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=512, G=4096, t=4, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG,
+                          seed=42)
+    params, ctrl, meas, truth = ea.unicycle_scenario(42, 32, 50.0, 3)
+    f = ea.Filter(engine, desc, params)
+    f.init_uniform(np.zeros(3, np.float32))
+    for k in range(3):
+        f.step(ctrl[k], meas[k], 0.1)
+    st, w = f.particles()
+    assert np.all(np.isfinite(st))
+    w = w.reshape(4096, 512)
+    head = w[:, : 512 - 16]
+    assert np.all(head[:, :-1] >= head[:, 1:])   # sorted per sub-filter up to the received slots
+    est = f.estimate()
+    assert np.hypot(est.mean[0] - truth[2, 0], est.mean[1] - truth[2, 1]) < 0.5
