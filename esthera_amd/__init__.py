@@ -167,6 +167,7 @@ _SIGS = {
                              C.c_int, _P], C.c_int),
     "esth_stage_alias_table": ([_P, C.c_uint32, C.c_uint32, C.c_int, _P, _P], C.c_int),
     "esth_stage_math": ([C.c_int, _P, _P, _P, C.c_int], C.c_int),
+    "esth_stage_copy": ([_P, _P, C.c_size_t], C.c_int),
 }
 
 

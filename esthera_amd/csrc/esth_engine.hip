@@ -484,6 +484,17 @@ extern "C" int esth_stage_alias_table(const float* d_w, uint32_t m, uint32_t G, 
     return ESTH_OK;
 }
 
+extern "C" int esth_stage_copy(void* dst, const void* src, size_t bytes) {
+    if (bytes % 16) return esth_fail(ESTH_ERR_INVALID, "copy size %zu not a multiple of 16", bytes);
+    const size_t n = bytes / 16;
+    int grid = (int)std::min<size_t>((n + 255) / 256, 256 * 64);
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, (float4*)dst, (const float4*)src, n);
+    ESTH_HIP(hipGetLastError());
+    ESTH_HIP(hipDeviceSynchronize());
+    return ESTH_OK;
+}
+
 extern "C" int esth_stage_math(int fn, const float* x, const float* x2, float* out, int n) {
     if (n <= 0) return ESTH_OK;
     if (fn == 4 && !x2) return esth_fail(ESTH_ERR_INVALID, "atan2 needs x2");

@@ -14,6 +14,7 @@
 #include "esth_internal.h"
 
 #define ESTH_FRAME_RING 16
+#define ESTH_TIMING_STEPS 4096
 
 struct esth_filter {
     esth_engine* e = nullptr;
@@ -41,6 +42,9 @@ struct esth_filter {
     int slot = 0;
     hipEvent_t ev[7];
     bool timing = false;
+    /* per-step stage events while timing is on: 7 per step, up to ESTH_TIMING_STEPS steps */
+    std::vector<hipEvent_t> tev;
+    int tsteps = 0;
     uint64_t steps = 0;
     /* graph replay */
     bool use_graph = false;
@@ -65,6 +69,7 @@ static void filter_free(esth_filter* f) {
     if (f->h_frames) hipHostFree(f->h_frames);
     for (int i = 0; i < ESTH_FRAME_RING; ++i) hipEventDestroy(f->frame_ev[i]);
     for (int i = 0; i < 7; ++i) hipEventDestroy(f->ev[i]);
+    for (auto ev : f->tev) hipEventDestroy(ev);
     if (f->stream) hipStreamDestroy(f->stream);
     delete f;
 }
@@ -159,33 +164,34 @@ extern "C" int esth_filter_init(esth_filter* f, const float* soa) {
 }
 
 /* the kernels of one step, reading the frame at d_frame */
-static int enqueue_step(esth_filter* f, const float* d_frame, bool timing) {
+static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
+    const bool timing = ev != nullptr;
     hipStream_t s = f->stream;
     const esth_filter_desc& d = f->d;
     const int log_w = d.weights == ESTH_WEIGHTS_LOG;
     const uint32_t N = (uint32_t)f->N;
-    if (timing) ESTH_HIP(hipEventRecord(f->ev[0], s));
+    if (timing) ESTH_HIP(hipEventRecord(ev[0], s));
     /* generate_normal_random: MTGP words (pf_generic.c:191-232) */
     ESTH_TRY(esth_launch_mtgp(f->e->d_params, f->status, f->streams, f->K, f->R, 1, s));
-    if (timing) ESTH_HIP(hipEventRecord(f->ev[1], s));
+    if (timing) ESTH_HIP(hipEventRecord(ev[1], s));
     /* sampling_importance with Box-Muller fused (pf_generic.c:521-533) */
     ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, f->R, d_frame, log_w, f->w, s));
-    if (timing) ESTH_HIP(hipEventRecord(f->ev[2], s));
+    if (timing) ESTH_HIP(hipEventRecord(ev[2], s));
     /* block_sort (pf_generic.c:572-585) */
     ESTH_TRY(esth_launch_sort(f->S, f->m, f->w, f->state, f->tmp, N, f->G, f->winner, nullptr, f->t, f->bt_w,
                               f->bt_s, f->part, log_w, s));
-    if (timing) ESTH_HIP(hipEventRecord(f->ev[3], s));
+    if (timing) ESTH_HIP(hipEventRecord(ev[3], s));
     /* reduce (pf_generic.c:604-615), result kept on the device */
     ESTH_TRY(esth_launch_argmax(f->S, f->winner, f->G, log_w, f->part, f->tmp, N, f->m, f->result, f->est, s));
-    if (timing) ESTH_HIP(hipEventRecord(f->ev[4], s));
+    if (timing) ESTH_HIP(hipEventRecord(ev[4], s));
     /* exchange (pf_generic.c:702-715) */
     ESTH_TRY(esth_launch_exchange(f->tmp, f->w, f->S, N, f->m, f->G, f->t, d.topology, f->dimx, f->dimy, f->bt_w,
                                   f->bt_s, f->top, s));
-    if (timing) ESTH_HIP(hipEventRecord(f->ev[5], s));
+    if (timing) ESTH_HIP(hipEventRecord(ev[5], s));
     /* resampling (pf_generic.c:732-757), uniforms at rand_offset = N*V */
     ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, f->R + (size_t)N * f->V, N, f->m, f->G,
                                   d.resampling_threshold, d.adaptive, log_w, nullptr, s));
-    if (timing) ESTH_HIP(hipEventRecord(f->ev[6], s));
+    if (timing) ESTH_HIP(hipEventRecord(ev[6], s));
     return ESTH_OK;
 }
 
@@ -195,7 +201,7 @@ static int capture_graph(esth_filter* f, int slot, hipGraphExec_t* out) {
     const float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
     ESTH_HIP(hipMemcpyAsync((void*)dfr, f->h_frames + (size_t)slot * ESTH_FRAME_FLOATS, sizeof(float) * ESTH_FRAME_FLOATS,
                             hipMemcpyHostToDevice, f->stream));
-    int rc = enqueue_step(f, dfr, false);
+    int rc = enqueue_step(f, dfr, nullptr);
     hipError_t e2 = hipStreamEndCapture(f->stream, &g);
     if (rc != ESTH_OK) { if (g) hipGraphDestroy(g); return rc; }
     if (e2 != hipSuccess) return esth_fail(ESTH_ERR_DEVICE, "end capture: %s", hipGetErrorString(e2));
@@ -220,7 +226,19 @@ extern "C" int esth_filter_step(esth_filter* f, const void* control, const void*
     } else {
         float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
         ESTH_HIP(hipMemcpyAsync(dfr, hf, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice, f->stream));
-        ESTH_TRY(enqueue_step(f, dfr, f->timing));
+        hipEvent_t* ev = nullptr;
+        if (f->timing && f->tsteps < ESTH_TIMING_STEPS) {
+            if ((size_t)(f->tsteps + 1) * 7 > f->tev.size()) {
+                for (int i = 0; i < 7; ++i) {
+                    hipEvent_t e;
+                    ESTH_HIP(hipEventCreate(&e));
+                    f->tev.push_back(e);
+                }
+            }
+            ev = &f->tev[(size_t)f->tsteps * 7];
+            f->tsteps++;
+        }
+        ESTH_TRY(enqueue_step(f, dfr, ev));
         f->slot = (f->slot + 1) % ESTH_FRAME_RING;
     }
     ESTH_HIP(hipEventRecord(f->frame_ev[slot], f->stream));
@@ -248,22 +266,29 @@ extern "C" int esth_filter_stream(esth_filter* f, void** s) {
     return ESTH_OK;
 }
 
-int esth_filter_set_timing(esth_filter* f, int on);
 extern "C" int esth_filter_set_timing(esth_filter* f, int on) {
     if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
+    ESTH_HIP(hipStreamSynchronize(f->stream));
     f->timing = on != 0;
+    f->tsteps = 0;
     return ESTH_OK;
 }
 
+/* mean per-stage device time over the steps recorded since timing was switched on */
 extern "C" int esth_filter_stage_times(esth_filter* f, float us[6]) {
     if (!f || !us) return esth_fail(ESTH_ERR_INVALID, "null");
-    if (!f->timing || f->use_graph || f->steps == 0) return esth_fail(ESTH_ERR_STATE, "stage timing is off");
-    ESTH_HIP(hipEventSynchronize(f->ev[6]));
-    for (int i = 0; i < 6; ++i) {
-        float ms = 0.0f;
-        ESTH_HIP(hipEventElapsedTime(&ms, f->ev[i], f->ev[i + 1]));
-        us[i] = ms * 1000.0f;
+    if (f->tsteps == 0) return esth_fail(ESTH_ERR_STATE, "no timed steps (timing off, or graph replay)");
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < f->tsteps; ++k) {
+        hipEvent_t* ev = &f->tev[(size_t)k * 7];
+        for (int i = 0; i < 6; ++i) {
+            float ms = 0.0f;
+            ESTH_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            acc[i] += ms;
+        }
     }
+    for (int i = 0; i < 6; ++i) us[i] = (float)(acc[i] * 1000.0 / f->tsteps);
     return ESTH_OK;
 }
 

@@ -704,6 +704,12 @@ __global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
     if (sel) sel[idx] = (int)pick;
 }
 
+/* ================================================================ copy */
+__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 /* ================================================================ math */
 __global__ void k_math(int fn, const float* __restrict__ x, const float* __restrict__ x2,
                        float* __restrict__ out, int n) {

@@ -192,9 +192,10 @@ int esth_filter_estimate(esth_filter* f, esth_estimate* out);
 /* Current particles (state_dim planes of m*G floats) and per-particle
  * weights as left by the sort/exchange stages (sorted per sub-filter). */
 int esth_filter_read_particles(esth_filter* f, float* state_soa, float* weights);
-/* Per-stage device time of the last step in microseconds (hipEvent based):
- * [rng, sample_weight, sort, argmax, exchange, resample]; needs
- * esth_filter_set_timing(f, 1) and eager (non-graph) steps. */
+/* Stage timing with hipEvents recorded between the kernels on the filter's
+ * stream.  set_timing(f, 1) starts a new measurement (eager steps only, up to
+ * 4096 steps); stage_times returns the mean device time per step in
+ * microseconds of [rng, sample_weight, sort, argmax, exchange, resample]. */
 int esth_filter_set_timing(esth_filter* f, int on);
 int esth_filter_stage_times(esth_filter* f, float us[6]);
 /* Checkpoint / resume: particles, weights, RNG status and step counter. */
@@ -338,6 +339,9 @@ int esth_stage_resample(int resampler, float* d_state_dst, const float* d_state_
  * per sub-filter prob[m] (scaled, after pairing) and alias[m]. */
 int esth_stage_alias_table(const float* d_w, uint32_t m, uint32_t G, int weights, float* d_prob,
                            uint32_t* d_alias);
+/* Plain device copy of `bytes` (multiple of 16): a kernel of known traffic
+ * used to calibrate HBM byte counters. */
+int esth_stage_copy(void* d_dst, const void* d_src, size_t bytes);
 /* Deterministic math functions evaluated on the device (n inputs): fn 0 exp,
  * 1 log, 2 sin, 3 cos, 4 atan2(y=x[i], x=x2[i]). */
 int esth_stage_math(int fn, const float* d_x, const float* d_x2, float* d_out, int n);

@@ -630,6 +630,7 @@ struct or_filter {
     double* lw64;
     unsigned char mparams[2048];
     int dimx, dimy;
+    int f64;
 };
 
 or_filter* or_filter_create(const esth_filter_desc* d, const void* params, size_t params_size,
@@ -660,6 +661,7 @@ or_filter* or_filter_create(const esth_filter_desc* d, const void* params, size_
     f->dimy = d->torus_dimy;
     if (d->topology == ESTH_TOPO_TORUS && (f->dimx <= 0 || f->dimy <= 0))
         esth_torus_dims(d->num_blocks, &f->dimx, &f->dimy);
+    f->f64 = 1;
     return f;
 }
 
@@ -701,16 +703,18 @@ int or_filter_step(or_filter* f, const void* ctrl, const void* meas, float dt) {
     or_mtgp_fill(f->params, f->K, 0, f->status, (int)f->size, (uint32_t*)f->R, 1);
     or_boxmuller(f->R, (int64_t)N * f->V);
     /* FP64 twin from the same prior and normals (lock-step posterior) */
+    if (f->f64) {
     double* pre = f->st64;
     float* prior = (float*)malloc(sizeof(float) * (size_t)f->S * N);
     memcpy(prior, f->state, sizeof(float) * (size_t)f->S * N);
     or_sample_weight_f64(d->model, prior, (uint32_t)N, f->R, f->mparams, ctrl, meas, dt, pre, f->lw64);
     posterior_f64(pre, f->lw64, N, f->S, f->mean64);
     free(prior);
+    }
     /* sampling_importance, pf_generic.c:521-533 */
     or_sample_weight(d->model, f->state, (uint32_t)N, f->R, frame, d->weights, f->w);
     /* FP64 posterior of the FP32 path's particles and weights */
-    {
+    if (f->f64) {
         double* lw = f->lw64;
         double* st = f->st64;
         for (size_t i = 0; i < N; ++i)
@@ -747,6 +751,9 @@ void or_filter_estimate(or_filter* f, double* m32, double* m64, float* best) {
 }
 
 uint64_t or_filter_rng_words(or_filter* f) { return f->words; }
+
+/* 0: time only the FP32 restatement (the CPU baseline), 1: also the FP64 twin */
+void or_filter_set_f64(or_filter* f, int on) { f->f64 = on; }
 
 void or_set_threads(int n) {
 #ifdef _OPENMP

@@ -62,6 +62,7 @@ void or_filter_read(or_filter* f, float* state, float* w);
  * with the same normals (mean64); best = (w, g) of the argmax. */
 void or_filter_estimate(or_filter* f, double* mean_f32path, double* mean_f64path, float* best);
 uint64_t or_filter_rng_words(or_filter* f);
+void or_filter_set_f64(or_filter* f, int on);
 void or_set_threads(int n);
 
 #ifdef __cplusplus

@@ -65,6 +65,7 @@ def lib():
         L.or_filter_rng_words.argtypes = [C.c_void_p]
         L.or_filter_rng_words.restype = C.c_uint64
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_filter_set_f64.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -193,6 +194,9 @@ class Filter:
         if getattr(self, "h", None):
             lib().or_filter_destroy(self.h)
             self.h = None
+
+    def set_f64(self, on):
+        lib().or_filter_set_f64(self.h, 1 if on else 0)
 
     def init_uniform(self, state):
         lib().or_filter_init_uniform(self.h, np.ascontiguousarray(state, np.float32))
