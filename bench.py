@@ -32,26 +32,29 @@ HBM_PEAK_GBS = 8000.0
 # + state write 12 + log-weight write 4
 SW_BYTES_PER_PARTICLE = 40
 M, G, T, L = 512, 4096, 4, 32
-STAGES = ["rng", "sample_weight", "sort", "argmax", "exchange", "resample"]
+STAGES = ["rng_wait", "sample_weight", "sort", "argmax", "exchange_or_topt", "resample"]
 
 
-def run_plain(ea, eng, resampler, steps, warmup):
+def run_plain(ea, eng, resampler, steps, warmup, graph=False, timing=1):
     desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=G, t=T, topology=ea.TOPO_TORUS, resampler=resampler,
                           weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
     params, ctrl, meas, truth = ea.unicycle_scenario(42, L, 50.0, warmup + steps)
     f = ea.Filter(eng, desc, params)
     import numpy as np
     f.init_uniform(np.zeros(3, np.float32))
+    if graph:
+        f.use_graph(True)
     for k in range(warmup):
         f.step(ctrl[k], meas[k], 0.1)
     ea.device_sync()
-    f.set_timing(True)
+    if not graph:
+        f.set_timing(timing)
     t0 = time.perf_counter()
     for k in range(warmup, warmup + steps):
         f.step(ctrl[k], meas[k], 0.1)
     ea.device_sync()
     t1 = time.perf_counter()
-    stage = [float(x) for x in f.stage_times()]
+    stage = [float(x) for x in f.stage_times()] if not graph else [float("nan")] * 6
     est = f.estimate()
     k = warmup + steps - 1
     err = float(np.hypot(est.mean[0] - truth[k, 0], est.mean[1] - truth[k, 1]))
@@ -105,6 +108,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-alias", action="store_true", help="skip the Vose alias variant")
+    ap.add_argument("--graph-only", action="store_true", help="only the HIP-graph replay run (profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -124,8 +128,15 @@ def main():
         b.free()
 
     N = M * G
+    if args.graph_only:
+        gsecs, _, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, graph=True)
+        print(json.dumps({"graph_ms_per_step": gsecs * 1e3 / args.steps}), flush=True)
+        return
+    # timed region: eager steps, hipEvents bracketing only the sample+weight kernel
     secs, stage, err = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup)
     value = N * args.steps / secs
+    # separate short pass with events around every stage (adds idle gaps; breakdown only)
+    _, stage_all, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 100), args.warmup, timing=2)
     sw_us = stage[1]
     achieved = SW_BYTES_PER_PARTICLE * N / (sw_us * 1e-6) / 1e9
     out = {
@@ -149,14 +160,16 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
                      "kernel": "k_sample_weight<unicycle>", "kernel_us": sw_us,
                      "bytes_per_launch": SW_BYTES_PER_PARTICLE * N},
-        "stages_us": dict(zip(STAGES, stage)),
+        "stages_us": {k: v for k, v in zip(STAGES, stage_all) if v == v},
         "position_error_last_step": err,
     }
+    gsecs, _, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, graph=True)
+    out["graph_replay"] = {"value": N * args.steps / gsecs, "ms_per_step": gsecs * 1e3 / args.steps}
     if not args.no_alias:
-        asecs, astage, aerr = run_plain(ea, eng, ea.RESAMPLE_ALIAS, max(1, args.steps // 2), args.warmup)
+        asecs, astage, aerr = run_plain(ea, eng, ea.RESAMPLE_ALIAS, max(1, args.steps // 2), args.warmup, timing=2)
         out["alias"] = {"value": N * max(1, args.steps // 2) / asecs, "unit": "particles/s",
                         "ms_per_step": asecs * 1e3 / max(1, args.steps // 2),
-                        "stages_us": dict(zip(STAGES, astage))}
+                        "stages_us": {k: v for k, v in zip(STAGES, astage) if v == v}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     eng.close()

@@ -123,6 +123,7 @@ _SIGS = {
     "esth_filter_restore": ([_P, _P, C.c_size_t], C.c_int),
     "esth_filter_stream": ([_P, _P], C.c_int),
     "esth_filter_use_graph": ([_P, C.c_int], C.c_int),
+    "esth_filter_set_prefetch": ([_P, C.c_int], C.c_int),
     "esth_particle_filter": ([C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p, C.c_uint32, C.c_int,
                               _P, _P], C.c_int),
     "esth_trace_load_robotarm": ([C.c_char_p, C.c_int, _P, _P, _P, _P, _P], C.c_int),
@@ -138,6 +139,10 @@ _SIGS = {
     "esth_batch_read_particles": ([_P, C.c_int, _P, _P], C.c_int),
     "esth_batch_use_graph": ([_P, C.c_int], C.c_int),
     "esth_batch_step_time": ([_P, _P], C.c_int),
+    "esth_batch_step_times": ([_P, _P, C.c_int, _P], C.c_int),
+    "esth_batch_info": ([_P, _P, _P, _P], C.c_int),
+    "esth_autotune": ([_P, _P, _P, C.c_size_t, _P, _P, C.c_float, _P], C.c_int),
+    "esth_autotune_pick": ([_P, C.c_uint32, C.c_uint32, _P], C.c_int),
     "esth_server_create": ([_P, C.c_int, _P], C.c_int),
     "esth_server_destroy": ([_P], C.c_int),
     "esth_server_open": ([_P, _P, _P, C.c_size_t, _P, _P, _P], C.c_int),
@@ -330,13 +335,17 @@ class Filter:
         check(lib().esth_filter_read_particles(self.h, ptr(st), ptr(w)))
         return st, w
 
-    def set_timing(self, on=True):
-        check(lib().esth_filter_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, level=1):
+        """0 off, 1 time the sample+weight kernel, 2 time every stage."""
+        check(lib().esth_filter_set_timing(self.h, int(level)))
 
     def stage_times(self):
         us = np.zeros(6, np.float32)
         check(lib().esth_filter_stage_times(self.h, ptr(us)))
         return us
+
+    def set_prefetch(self, on=True):
+        check(lib().esth_filter_set_prefetch(self.h, 1 if on else 0))
 
     def use_graph(self, on=True):
         check(lib().esth_filter_use_graph(self.h, 1 if on else 0))
@@ -361,6 +370,93 @@ class Filter:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many filters stepped together (one graph launch per step)."""
+
+    def __init__(self, engine, descs, params=None):
+        self.engine = engine
+        self.n = len(descs)
+        arr = (FilterDesc * self.n)(*descs)
+        self.descs = list(descs)
+        self.S = STATE_DIM[descs[0].model]
+        self._params = params
+        h = C.c_void_p()
+        if params is not None:
+            parr = (type(params[0]) * self.n)(*params)
+            self._parr = parr
+            check(lib().esth_batch_create(engine.h, self.n, arr, C.cast(parr, C.c_void_p), C.sizeof(type(params[0])),
+                                          C.byref(h)))
+        else:
+            check(lib().esth_batch_create(engine.h, self.n, arr, None, 0, C.byref(h)))
+        self.h = h
+
+    def init_uniform(self, states):
+        s = np.ascontiguousarray(states, dtype=np.float32).reshape(self.n, self.S)
+        check(lib().esth_batch_init_uniform(self.h, ptr(s)))
+
+    def step(self, controls, measurements, dts):
+        """controls / measurements: ctypes arrays of the model structs (one per filter)."""
+        d = np.ascontiguousarray(dts, dtype=np.float32)
+        cptr = None if controls is None else C.cast(controls, C.c_void_p)
+        cs = 0 if controls is None else C.sizeof(controls._type_)
+        check(lib().esth_batch_step(self.h, cptr, cs, C.cast(measurements, C.c_void_p),
+                                    C.sizeof(measurements._type_), ptr(d)))
+
+    def sync(self):
+        check(lib().esth_batch_sync(self.h))
+
+    def use_graph(self, on=True):
+        check(lib().esth_batch_use_graph(self.h, 1 if on else 0))
+
+    def estimate(self, i):
+        e = Estimate()
+        check(lib().esth_batch_estimate(self.h, i, C.byref(e)))
+        return e
+
+    def particles(self, i):
+        N = self.descs[i].num_particles * self.descs[i].num_blocks
+        st = np.empty(self.S * N, np.float32)
+        w = np.empty(N, np.float32)
+        check(lib().esth_batch_read_particles(self.h, i, ptr(st), ptr(w)))
+        return st, w
+
+    def step_times(self, max_steps=8192):
+        us = np.zeros(max_steps, np.float32)
+        n = C.c_int()
+        check(lib().esth_batch_step_times(self.h, ptr(us), max_steps, C.byref(n)))
+        return us[: n.value]
+
+    def info(self):
+        p, s, g = C.c_uint64(), C.c_int(), C.c_uint32()
+        check(lib().esth_batch_info(self.h, C.byref(p), C.byref(s), C.byref(g)))
+        return p.value, s.value, g.value
+
+    def close(self):
+        if self.h:
+            lib().esth_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(engine, desc, params, control, measurement, dt):
+    us = np.zeros(5, np.float32)
+    psize = C.sizeof(params) if params is not None else 0
+    check(lib().esth_autotune(engine.h, ptr(desc), ptr(params), psize, ptr(control), ptr(measurement), dt, ptr(us)))
+    return us
+
+
+def autotune_pick(us_per_mparticle, particles, min_m=256):
+    us = np.ascontiguousarray(us_per_mparticle, dtype=np.float32)
+    m = C.c_uint32()
+    check(lib().esth_autotune_pick(ptr(us), particles, min_m, C.byref(m)))
+    return m.value
 
 
 def particle_filter(m, G, t, threshold, trace_path, seed=1234, resampler=RESAMPLE_MULTINOMIAL):

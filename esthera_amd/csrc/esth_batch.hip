@@ -1,1 +1,576 @@
+/*
+ * esth_batch.hip - many independent filters stepped together.
+ *
+ * All filters of a batch live in shared device buffers, each in its own
+ * single-filter layout (S state planes of N_f, N_f weights, its own MTGP32
+ * stream set of K_f streams seeded seed_f + j + 1).  One step is:
+ *   RNG stream : one MTGP32 launch over every filter's streams (next step)
+ *   main stream: frame upload, sample+weight, sort (one launch per sub-filter
+ *                size class), argmax (one work-group per filter), exchange,
+ *                resampling (per size class)
+ * and the main-stream part is captured once per step parity into a HIP graph
+ * and replayed.  Because every kernel runs the single-filter per-sub-filter
+ * code on each filter's own buffers and random words, a filter's particles are
+ * bit-identical to the same filter stepped alone with ESTH_RNG_MTGP_PER_FILTER.
+ */
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
 #include "esth_internal.h"
+#include "esth_kernels.h"
+
+#define ESTH_BATCH_TIMING_STEPS 8192
+
+struct esth_bclass {
+    int logm = 0;
+    uint32_t n = 0;
+    uint2* d_sub = nullptr;
+};
+
+struct esth_batch {
+    esth_engine* e = nullptr;
+    int n = 0;
+    esth_filter_desc d{};   /* shared settings (model, topology, t, resampler, ...) */
+    int S = 0, V = 0;
+    int nq = 1;
+    size_t frame_stride = 0;
+    std::vector<esth_filter_desc> descs;
+    std::vector<std::vector<unsigned char>> params;
+    std::vector<esth_bfilter> hf;   /* host copy of the table */
+    std::vector<uint64_t> soff, woff, stream0;
+    std::vector<int> K;
+    std::vector<uint64_t> rsize;
+    uint64_t Ntot = 0, words = 0;
+    int nstreams = 0;
+    uint32_t nsub = 0;
+    hipStream_t stream = nullptr, rng_stream = nullptr;
+    /* device */
+    esth_bfilter* d_F = nullptr;
+    uint2* d_sub_all = nullptr;
+    std::vector<esth_bclass> cls;
+    float *state = nullptr, *tmp = nullptr, *w = nullptr, *part = nullptr, *bt_w = nullptr, *bt_s = nullptr;
+    float* R[2] = {nullptr, nullptr};
+    float* frames[2] = {nullptr, nullptr};
+    uint32_t* status[2] = {nullptr, nullptr};
+    esth_mtgp_stream* streams = nullptr;
+    float2 *winner = nullptr, *result = nullptr;
+    double* est = nullptr;
+    uint32_t* top = nullptr;
+    std::vector<std::pair<void*, size_t>> allocs;
+    /* host */
+    float* h_frames[2] = {nullptr, nullptr};
+    hipEvent_t frame_ev[2];
+    bool frame_used[2] = {false, false};
+    bool rng_ready = false, prefetch = true;
+    hipEvent_t ev_rng_done[2], ev_consumed[2], ev_fork;
+    bool use_graph = true;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    uint64_t steps = 0;
+    /* step latency: an event pair per step */
+    std::vector<hipEvent_t> tev;
+    int tsteps = 0;
+};
+
+static int balloc(esth_batch* b, size_t bytes, void** p) {
+    ESTH_TRY(esth_engine_alloc(b->e, bytes ? bytes : 4, p));
+    b->allocs.push_back({*p, bytes ? bytes : 4});
+    return ESTH_OK;
+}
+
+static void batch_free(esth_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->e->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->rng_stream) hipStreamSynchronize(b->rng_stream);
+    for (int i = 0; i < 2; ++i) {
+        if (b->gexec[i]) hipGraphExecDestroy(b->gexec[i]);
+        if (b->h_frames[i]) hipHostFree(b->h_frames[i]);
+        hipEventDestroy(b->frame_ev[i]);
+        hipEventDestroy(b->ev_rng_done[i]);
+        hipEventDestroy(b->ev_consumed[i]);
+    }
+    hipEventDestroy(b->ev_fork);
+    for (auto ev : b->tev) hipEventDestroy(ev);
+    for (auto& a : b->allocs) esth_engine_release(b->e, a.first, a.second);
+    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->rng_stream) hipStreamDestroy(b->rng_stream);
+    delete b;
+}
+
+static size_t frame_floats(int model, const void* params) {
+    if (model == ESTH_MODEL_UNICYCLE) {
+        const esth_unicycle_params* p = (const esth_unicycle_params*)params;
+        return (size_t)ESTH_UNI_HDR + 5u * (size_t)p->num_landmarks;
+    }
+    if (model == ESTH_MODEL_DH6) return (size_t)ESTH_DH6_HDR + 24u;
+    return 16u;
+}
+
+extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* descs, const void* params,
+                                 size_t params_stride, esth_batch** out) {
+    if (!e || !descs || n <= 0 || !out) return esth_fail(ESTH_ERR_INVALID, "batch: bad arguments");
+    const esth_filter_desc& d0 = descs[0];
+    int S, V;
+    size_t psize = 0;
+    ESTH_TRY(esth_params_size(d0.model, &psize));
+    if (psize && (!params || params_stride < psize)) return esth_fail(ESTH_ERR_INVALID, "batch: params stride %zu < %zu", params_stride, psize);
+    for (int i = 0; i < n; ++i) {
+        const esth_filter_desc& d = descs[i];
+        ESTH_TRY(esth_check_desc(&d, &S, &V));
+        if (d.model != d0.model || d.topology != d0.topology || d.num_transfer != d0.num_transfer ||
+            d.resampler != d0.resampler || d.resampling_threshold != d0.resampling_threshold ||
+            d.adaptive != d0.adaptive || d.weights != d0.weights)
+            return esth_fail(ESTH_ERR_INVALID, "batch: filter %d differs from filter 0 in model/topology/t/resampler/weights", i);
+        if (d.rng != ESTH_RNG_MTGP_PER_FILTER)
+            return esth_fail(ESTH_ERR_INVALID, "batch: filter %d must use ESTH_RNG_MTGP_PER_FILTER", i);
+        if (d.num_particles < 64) return esth_fail(ESTH_ERR_UNSUPPORTED, "batch: sub-filters need m >= 64 (filter %d)", i);
+    }
+    ESTH_HIP(hipSetDevice(e->device));
+    esth_batch* b = new esth_batch();
+    b->e = e;
+    b->n = n;
+    b->d = d0;
+    b->S = S;
+    b->V = V;
+    b->nq = d0.topology == ESTH_TOPO_TORUS ? 4 : 1;
+    for (int i = 0; i < 2; ++i) {
+        hipEventCreateWithFlags(&b->frame_ev[i], hipEventDisableTiming);
+        hipEventCreateWithFlags(&b->ev_rng_done[i], hipEventDisableTiming);
+        hipEventCreateWithFlags(&b->ev_consumed[i], hipEventDisableTiming);
+    }
+    hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    b->descs.assign(descs, descs + n);
+    b->params.resize(n);
+    b->hf.resize(n);
+    b->soff.resize(n);
+    b->woff.resize(n);
+    b->stream0.resize(n);
+    b->K.resize(n);
+    b->rsize.resize(n);
+    /* layout */
+    uint64_t Gtot = 0, tGtot = 0, fl = 0;
+    for (int i = 0; i < n; ++i) {
+        const esth_filter_desc& d = descs[i];
+        const unsigned char* pp = psize ? (const unsigned char*)params + (size_t)i * params_stride : nullptr;
+        if (psize) b->params[i].assign(pp, pp + psize);
+        b->frame_stride = std::max(b->frame_stride, frame_floats(d.model, pp));
+        const uint64_t N = (uint64_t)d.num_particles * d.num_blocks;
+        const uint64_t need = esth_rng_needed(V, d.resampler, d.adaptive, d.num_particles, d.num_blocks);
+        b->K[i] = esth_rng_streams(ESTH_RNG_MTGP_PER_FILTER, need);
+        b->rsize[i] = esth_rng_stream_size(b->K[i], need);
+        b->soff[i] = b->Ntot * (uint64_t)S;
+        b->woff[i] = b->Ntot;
+        b->stream0[i] = (uint64_t)b->nstreams;
+        esth_bfilter& f = b->hf[i];
+        memset(&f, 0, sizeof(f));
+        f.N = (uint32_t)N;
+        f.G = d.num_blocks;
+        f.m = d.num_particles;
+        f.t = d.topology == ESTH_TOPO_NONE ? 0 : d.num_transfer;
+        f.dimx = d.torus_dimx;
+        f.dimy = d.torus_dimy;
+        if (d.topology == ESTH_TOPO_TORUS && (f.dimx <= 0 || f.dimy <= 0)) esth_torus_dims(f.G, &f.dimx, &f.dimy);
+        if (f.dimx <= 0) f.dimx = 1;
+        if (f.dimy <= 0) f.dimy = 1;
+        f.uoff = (uint32_t)(N * (uint64_t)V);
+        f.r_off = b->words;
+        b->Ntot += N;
+        b->words += b->rsize[i] * (uint64_t)b->K[i];
+        b->nstreams += b->K[i];
+        Gtot += d.num_blocks;
+        tGtot += (uint64_t)(f.t ? f.t : 1) * d.num_blocks;
+        (void)fl;
+    }
+    for (int i = 0; i < n; ++i) b->hf[i].f_off = (uint64_t)i * b->frame_stride;
+    b->nsub = (uint32_t)Gtot;
+    int rc = ESTH_OK;
+#define BA(ptr, bytes) if (rc == ESTH_OK) rc = balloc(b, (bytes), (void**)&(ptr))
+    BA(b->state, sizeof(float) * b->Ntot * S);
+    BA(b->tmp, sizeof(float) * b->Ntot * S);
+    BA(b->w, sizeof(float) * b->Ntot);
+    BA(b->R[0], sizeof(float) * b->words);
+    BA(b->R[1], sizeof(float) * b->words);
+    BA(b->status[0], sizeof(uint32_t) * ESTH_MTGP_N * (size_t)b->nstreams);
+    BA(b->status[1], sizeof(uint32_t) * ESTH_MTGP_N * (size_t)b->nstreams);
+    BA(b->streams, sizeof(esth_mtgp_stream) * (size_t)b->nstreams);
+    BA(b->part, sizeof(float) * Gtot * (2 + S));
+    BA(b->winner, sizeof(float2) * Gtot);
+    BA(b->bt_w, sizeof(float) * tGtot);
+    BA(b->bt_s, sizeof(float) * tGtot * S);
+    BA(b->result, sizeof(float2) * n);
+    BA(b->est, sizeof(double) * (4 + 2 * ESTH_MAX_STATE_DIM) * n);
+    BA(b->top, sizeof(uint32_t) * 64 * n);
+    BA(b->frames[0], sizeof(float) * b->frame_stride * n);
+    BA(b->frames[1], sizeof(float) * b->frame_stride * n);
+    BA(b->d_F, sizeof(esth_bfilter) * n);
+    BA(b->d_sub_all, sizeof(uint2) * Gtot);
+#undef BA
+    if (rc != ESTH_OK) { batch_free(b); return rc; }
+    /* filter table, sub-filter tables, RNG streams */
+    uint64_t goff = 0, tgoff = 0;
+    std::vector<std::vector<uint2>> by_class(11);
+    std::vector<uint2> all;
+    all.reserve(Gtot);
+    std::vector<esth_mtgp_stream> hs((size_t)b->nstreams);
+    std::vector<uint32_t> st((size_t)ESTH_MTGP_N * b->nstreams);
+    for (int i = 0; i < n; ++i) {
+        esth_bfilter& f = b->hf[i];
+        f.state = b->state + b->soff[i];
+        f.tmp = b->tmp + b->soff[i];
+        f.w = b->w + b->woff[i];
+        f.part = b->part + goff * (2 + S);
+        f.winner = (void*)(b->winner + goff);
+        f.bt_w = b->bt_w + tgoff;
+        f.bt_s = b->bt_s + tgoff * S;
+        f.result = (void*)(b->result + i);
+        f.est = b->est + (size_t)i * (4 + 2 * ESTH_MAX_STATE_DIM);
+        f.top = b->top + (size_t)i * 64;
+        int lm = 0;
+        while ((1u << lm) < f.m) ++lm;
+        for (uint32_t g = 0; g < f.G; ++g) {
+            by_class[lm].push_back(make_uint2((uint32_t)i, g));
+            all.push_back(make_uint2((uint32_t)i, g));
+        }
+        for (int j = 0; j < b->K[i]; ++j) {
+            const size_t sj = (size_t)b->stream0[i] + (size_t)j;
+            hs[sj] = {(uint32_t)j, (uint32_t)b->rsize[i], f.r_off + (uint64_t)j * b->rsize[i]};
+            esth_mtgp_seed_state(j, descs[i].seed + (uint32_t)j + 1u, st.data() + sj * ESTH_MTGP_N);
+        }
+        goff += f.G;
+        tgoff += (uint64_t)(f.t ? f.t : 1) * f.G;
+    }
+    hipError_t err = hipMemcpy(b->d_F, b->hf.data(), sizeof(esth_bfilter) * n, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(b->d_sub_all, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(b->streams, hs.data(), sizeof(esth_mtgp_stream) * hs.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(b->status[0], st.data(), sizeof(uint32_t) * st.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemset(b->w, 0, sizeof(float) * b->Ntot);
+    if (err == hipSuccess) err = hipMemset(b->est, 0, sizeof(double) * (4 + 2 * ESTH_MAX_STATE_DIM) * n);
+    if (err != hipSuccess) { batch_free(b); return esth_fail(ESTH_ERR_DEVICE, "batch upload: %s", hipGetErrorString(err)); }
+    for (int lm = 0; lm <= 10; ++lm) {
+        if (by_class[lm].empty()) continue;
+        esth_bclass c;
+        c.logm = lm;
+        c.n = (uint32_t)by_class[lm].size();
+        rc = balloc(b, sizeof(uint2) * c.n, (void**)&c.d_sub);
+        if (rc == ESTH_OK && hipMemcpy(c.d_sub, by_class[lm].data(), sizeof(uint2) * c.n, hipMemcpyHostToDevice) != hipSuccess)
+            rc = esth_fail(ESTH_ERR_DEVICE, "batch class upload");
+        if (rc != ESTH_OK) { batch_free(b); return rc; }
+        b->cls.push_back(c);
+    }
+    err = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&b->rng_stream, hipStreamNonBlocking);
+    for (int i = 0; i < 2 && err == hipSuccess; ++i)
+        err = hipHostMalloc((void**)&b->h_frames[i], sizeof(float) * b->frame_stride * n, 0);
+    if (err != hipSuccess) { batch_free(b); return esth_fail(ESTH_ERR_DEVICE, "batch streams: %s", hipGetErrorString(err)); }
+    *out = b;
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_destroy(esth_batch* b) {
+    batch_free(b);
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_init_uniform(esth_batch* b, const float* states) {
+    if (!b || !states) return esth_fail(ESTH_ERR_INVALID, "null");
+    ESTH_HIP(hipSetDevice(b->e->device));
+    ESTH_HIP(hipStreamSynchronize(b->stream));
+    std::vector<float> h((size_t)b->Ntot * b->S);
+    for (int i = 0; i < b->n; ++i) {
+        const esth_bfilter& f = b->hf[i];
+        for (int c = 0; c < b->S; ++c)
+            std::fill(h.begin() + b->soff[i] + (size_t)c * f.N, h.begin() + b->soff[i] + (size_t)(c + 1) * f.N,
+                      states[(size_t)i * b->S + c]);
+    }
+    ESTH_HIP(hipMemcpy(b->state, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    return ESTH_OK;
+}
+
+/* ------------------------------------------------------------ launches */
+template <int LOGM, int S>
+static void launch_kb_sort(esth_batch* b, const esth_bclass& c, int log_w) {
+    hipLaunchKernelGGL((kb_sort<LOGM, S>), dim3((c.n + 3) / 4), dim3(256), 0, b->stream, b->d_F, c.d_sub, c.n, log_w);
+}
+template <int LOGM, int S>
+static void launch_kb_resample(esth_batch* b, const esth_bclass& c, const float* R, int log_w) {
+    hipLaunchKernelGGL((kb_resample<LOGM, S>), dim3((c.n + 3) / 4), dim3(256), 0, b->stream, b->d_F, c.d_sub, c.n, R,
+                       b->d.resampling_threshold, b->d.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0, log_w);
+}
+
+#define ESTH_BCLASS(FN, S, ...)                        \
+    switch (c.logm) {                                  \
+    case 6: FN<6, S>(__VA_ARGS__); break;              \
+    case 7: FN<7, S>(__VA_ARGS__); break;              \
+    case 8: FN<8, S>(__VA_ARGS__); break;              \
+    case 9: FN<9, S>(__VA_ARGS__); break;              \
+    case 10: FN<10, S>(__VA_ARGS__); break;            \
+    default: return esth_fail(ESTH_ERR_INVALID, "m"); \
+    }
+
+template <int S>
+static int batch_kernels_s(esth_batch* b, int p) {
+    const int log_w = b->d.weights == ESTH_WEIGHTS_LOG;
+    const float* R = b->R[p];
+    const float* frames = b->frames[p];
+    hipStream_t s = b->stream;
+    const uint32_t nsub = b->nsub;
+    const int grid_w = (int)(((uint64_t)nsub * ESTH_WAVE + 255) / 256);
+    switch (b->d.model) {
+    case ESTH_MODEL_ROBOTARM5:
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
+        break;
+    case ESTH_MODEL_UNICYCLE:
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
+        break;
+    default:
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
+        break;
+    }
+    for (const esth_bclass& c : b->cls) ESTH_BCLASS(launch_kb_sort, S, b, c, log_w);
+    hipLaunchKernelGGL(kb_argmax<S>, dim3(b->n), dim3(256), 0, s, b->d_F, log_w);
+    const uint32_t t = b->d.topology == ESTH_TOPO_NONE ? 0 : b->d.num_transfer;
+    if (t > 0) {
+        if (b->d.topology == ESTH_TOPO_ALL2ALL) hipLaunchKernelGGL(kb_topt, dim3(b->n), dim3(1024), 0, s, b->d_F);
+        const uint32_t nslots = (uint32_t)b->nq * t;
+        const uint64_t total = (uint64_t)nsub * nslots;
+        hipLaunchKernelGGL(kb_exchange, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub,
+                           S, nslots, b->d.topology);
+    }
+    if (b->d.resampler == ESTH_RESAMPLE_ALIAS) {
+        for (const esth_bclass& c : b->cls)
+            hipLaunchKernelGGL(kb_alias<S>, dim3(c.n), dim3(1u << c.logm), 0, s, b->d_F, c.d_sub, R,
+                               b->d.resampling_threshold, b->d.adaptive, log_w);
+    } else {
+        for (const esth_bclass& c : b->cls) ESTH_BCLASS(launch_kb_resample, S, b, c, R, log_w);
+    }
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+static int batch_kernels(esth_batch* b, int p) {
+    switch (b->S) {
+    case ESTH_UNI_S: return batch_kernels_s<ESTH_UNI_S>(b, p);
+    case ESTH_RA5_S: return batch_kernels_s<ESTH_RA5_S>(b, p);
+    case ESTH_DH6_S: return batch_kernels_s<ESTH_DH6_S>(b, p);
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", b->S);
+    }
+}
+
+static int launch_rng_b(esth_batch* b, uint64_t k, hipStream_t s) {
+    const int p = (int)(k & 1);
+    return esth_launch_mtgp(b->e->d_params, b->status[p], b->status[p ^ 1], b->streams, b->nstreams, b->R[p], 1, s);
+}
+
+static int capture_batch(esth_batch* b, int p) {
+    hipGraph_t g = nullptr;
+    ESTH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    ESTH_HIP(hipMemcpyAsync(b->frames[p], b->h_frames[p], sizeof(float) * b->frame_stride * b->n, hipMemcpyHostToDevice,
+                            b->stream));
+    int rc = batch_kernels(b, p);
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (rc != ESTH_OK) { if (g) hipGraphDestroy(g); return rc; }
+    if (e2 != hipSuccess) return esth_fail(ESTH_ERR_DEVICE, "batch capture: %s", hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(&b->gexec[p], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e3 != hipSuccess) return esth_fail(ESTH_ERR_DEVICE, "batch instantiate: %s", hipGetErrorString(e3));
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_step(esth_batch* b, const void* controls, size_t control_stride, const void* measurements,
+                               size_t meas_stride, const float* dts) {
+    if (!b || !measurements || !dts) return esth_fail(ESTH_ERR_INVALID, "null");
+    ESTH_HIP(hipSetDevice(b->e->device));
+    const int p = (int)(b->steps & 1);
+    if (b->frame_used[p]) ESTH_HIP(hipEventSynchronize(b->frame_ev[p]));
+    float* hf = b->h_frames[p];
+    float fr[ESTH_FRAME_FLOATS];
+    for (int i = 0; i < b->n; ++i) {
+        const void* c = controls ? (const unsigned char*)controls + (size_t)i * control_stride : nullptr;
+        const void* m = (const unsigned char*)measurements + (size_t)i * meas_stride;
+        const void* pp = b->params[i].empty() ? nullptr : b->params[i].data();
+        if (b->d.model == ESTH_MODEL_DH6 && !c) {
+            static const esth_dh6_control none = {0.0f};
+            c = &none;
+        }
+        if (esth_build_frame_impl(b->d.model, pp, c, m, dts[i], fr))
+            return esth_fail(ESTH_ERR_INVALID, "batch: bad control/measurement for filter %d", i);
+        memcpy(hf + (size_t)i * b->frame_stride, fr, sizeof(float) * b->frame_stride);
+    }
+    if (b->use_graph && !b->gexec[p]) ESTH_TRY(capture_batch(b, p));
+    hipEvent_t* ev = nullptr;
+    if (b->tsteps < ESTH_BATCH_TIMING_STEPS) {
+        if ((size_t)(b->tsteps + 1) * 2 > b->tev.size()) {
+            for (int i = 0; i < 2; ++i) {
+                hipEvent_t e;
+                ESTH_HIP(hipEventCreate(&e));
+                b->tev.push_back(e);
+            }
+        }
+        ev = &b->tev[(size_t)b->tsteps * 2];
+        b->tsteps++;
+    }
+    /* RNG: this step's words (if not prefetched) and the next step's on the RNG stream */
+    hipStream_t s = b->stream;
+    const uint64_t n = b->steps;
+    bool forked = false;
+    if (!b->rng_ready) {
+        ESTH_TRY(launch_rng_b(b, n, s));
+        if (b->prefetch) {
+            ESTH_HIP(hipEventRecord(b->ev_fork, s));
+            ESTH_HIP(hipStreamWaitEvent(b->rng_stream, b->ev_fork, 0));
+            forked = true;
+        }
+    } else {
+        ESTH_HIP(hipStreamWaitEvent(s, b->ev_rng_done[p], 0));
+    }
+    if (b->prefetch) {
+        if (!forked && n > 0) ESTH_HIP(hipStreamWaitEvent(b->rng_stream, b->ev_consumed[p ^ 1], 0));
+        ESTH_TRY(launch_rng_b(b, n + 1, b->rng_stream));
+        ESTH_HIP(hipEventRecord(b->ev_rng_done[p ^ 1], b->rng_stream));
+    }
+    if (ev) ESTH_HIP(hipEventRecord(ev[0], s));
+    if (b->use_graph) {
+        ESTH_HIP(hipGraphLaunch(b->gexec[p], s));
+    } else {
+        ESTH_HIP(hipMemcpyAsync(b->frames[p], hf, sizeof(float) * b->frame_stride * b->n, hipMemcpyHostToDevice, s));
+        ESTH_TRY(batch_kernels(b, p));
+    }
+    if (ev) ESTH_HIP(hipEventRecord(ev[1], s));
+    if (b->prefetch) ESTH_HIP(hipEventRecord(b->ev_consumed[p], s));
+    b->rng_ready = b->prefetch;
+    ESTH_HIP(hipEventRecord(b->frame_ev[p], s));
+    b->frame_used[p] = true;
+    b->steps++;
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_sync(esth_batch* b) {
+    if (!b) return esth_fail(ESTH_ERR_INVALID, "null");
+    ESTH_HIP(hipStreamSynchronize(b->stream));
+    ESTH_HIP(hipStreamSynchronize(b->rng_stream));
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_use_graph(esth_batch* b, int enable) {
+    if (!b) return esth_fail(ESTH_ERR_INVALID, "null");
+    ESTH_TRY(esth_batch_sync(b));
+    b->use_graph = enable != 0;
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_estimate(esth_batch* b, int i, esth_estimate* out) {
+    if (!b || !out || i < 0 || i >= b->n) return esth_fail(ESTH_ERR_INVALID, "batch estimate: bad filter %d", i);
+    ESTH_HIP(hipStreamSynchronize(b->stream));
+    const int S = b->S;
+    double h[4 + 2 * ESTH_MAX_STATE_DIM];
+    ESTH_HIP(hipMemcpy(h, b->est + (size_t)i * (4 + 2 * ESTH_MAX_STATE_DIM), sizeof(double) * (4 + 2 * (size_t)S),
+                       hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(*out));
+    out->state_dim = S;
+    out->log_weight_max = h[0];
+    out->weight_sum = h[1];
+    for (int c = 0; c < S; ++c) {
+        out->mean[c] = h[1] > 0.0 ? h[2 + c] / h[1] : NAN;
+        out->best[c] = (float)h[4 + S + c];
+    }
+    out->best_weight = (float)h[2 + S];
+    out->best_block = (int)h[3 + S];
+    out->step = b->steps;
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_read_particles(esth_batch* b, int i, float* soa, float* w) {
+    if (!b || i < 0 || i >= b->n) return esth_fail(ESTH_ERR_INVALID, "batch read: bad filter %d", i);
+    ESTH_HIP(hipStreamSynchronize(b->stream));
+    const esth_bfilter& f = b->hf[i];
+    if (soa) ESTH_HIP(hipMemcpy(soa, f.state, sizeof(float) * (size_t)f.N * b->S, hipMemcpyDeviceToHost));
+    if (w) ESTH_HIP(hipMemcpy(w, f.w, sizeof(float) * f.N, hipMemcpyDeviceToHost));
+    return ESTH_OK;
+}
+
+/* device time of the steps since the last call (or creation), microseconds */
+extern "C" int esth_batch_step_times(esth_batch* b, float* us, int max, int* count) {
+    if (!b || !count) return esth_fail(ESTH_ERR_INVALID, "null");
+    ESTH_HIP(hipStreamSynchronize(b->stream));
+    int k = std::min(b->tsteps, max);
+    for (int i = 0; i < k; ++i) {
+        float ms = 0.0f;
+        ESTH_HIP(hipEventElapsedTime(&ms, b->tev[(size_t)i * 2], b->tev[(size_t)i * 2 + 1]));
+        if (us) us[i] = ms * 1000.0f;
+    }
+    *count = k;
+    b->tsteps = 0;
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_step_time(esth_batch* b, float* us) {
+    if (!b || !us) return esth_fail(ESTH_ERR_INVALID, "null");
+    if (b->tsteps == 0) return esth_fail(ESTH_ERR_STATE, "no timed step");
+    ESTH_HIP(hipStreamSynchronize(b->stream));
+    float ms = 0.0f;
+    ESTH_HIP(hipEventElapsedTime(&ms, b->tev[(size_t)(b->tsteps - 1) * 2], b->tev[(size_t)(b->tsteps - 1) * 2 + 1]));
+    *us = ms * 1000.0f;
+    return ESTH_OK;
+}
+
+extern "C" int esth_batch_info(esth_batch* b, uint64_t* particles, int* streams, uint32_t* sub_filters) {
+    if (!b) return esth_fail(ESTH_ERR_INVALID, "null");
+    if (particles) *particles = b->Ntot;
+    if (streams) *streams = b->nstreams;
+    if (sub_filters) *sub_filters = b->nsub;
+    return ESTH_OK;
+}
+
+/* ============================================================ autotuner
+ * Device cost of the step's kernels (sample+weight, sort, argmax, exchange,
+ * resampling) per million particles for each sub-filter size 64..1024, measured
+ * on a 2^20-particle filter of the given model and settings. */
+extern "C" int esth_autotune(esth_engine* e, const esth_filter_desc* base, const void* params, size_t params_size,
+                             const void* control, const void* measurement, float dt, float us_per_mparticle[5]) {
+    if (!e || !base || !us_per_mparticle) return esth_fail(ESTH_ERR_INVALID, "null");
+    for (int k = 0; k < 5; ++k) {
+        esth_filter_desc d = *base;
+        d.num_particles = 64u << k;
+        d.num_blocks = (1u << 20) / d.num_particles;
+        d.torus_dimx = 0;
+        d.torus_dimy = 0;
+        d.rng = ESTH_RNG_MTGP_PER_FILTER;
+        esth_filter* f = nullptr;
+        ESTH_TRY(esth_filter_create(e, &d, params, params_size, &f));
+        int S, V;
+        esth_model_dims(d.model, &S, &V);
+        std::vector<float> zero(S, 0.0f);
+        int rc = esth_filter_init_uniform(f, zero.data());
+        for (int i = 0; rc == ESTH_OK && i < 3; ++i) rc = esth_filter_step(f, control, measurement, dt);
+        if (rc == ESTH_OK) rc = esth_filter_sync(f);
+        if (rc == ESTH_OK) rc = esth_filter_set_timing(f, 2);
+        for (int i = 0; rc == ESTH_OK && i < 10; ++i) rc = esth_filter_step(f, control, measurement, dt);
+        float us[6];
+        if (rc == ESTH_OK) rc = esth_filter_stage_times(f, us);
+        esth_filter_destroy(f);
+        if (rc != ESTH_OK) return rc;
+        us_per_mparticle[k] = us[1] + us[2] + us[3] + us[4] + us[5];
+    }
+    return ESTH_OK;
+}
+
+/* the sub-filter size for a filter of `particles` particles: among the powers
+ * of two in [max(64, min_m), 1024] that divide it, the one with the lowest
+ * measured cost (ties: the larger); if none is >= min_m, the largest divisor. */
+extern "C" int esth_autotune_pick(const float us_per_mparticle[5], uint32_t particles, uint32_t min_m, uint32_t* m_out) {
+    if (!us_per_mparticle || !m_out || particles % 64) return esth_fail(ESTH_ERR_INVALID, "particles must be a multiple of 64");
+    uint32_t best = 0;
+    float bc = 0.0f;
+    uint32_t largest = 64;
+    for (int k = 0; k < 5; ++k) {
+        const uint32_t m = 64u << k;
+        if (particles % m) continue;
+        largest = m;
+        if (m < min_m) continue;
+        if (best == 0 || us_per_mparticle[k] <= bc) { best = m; bc = us_per_mparticle[k]; }
+    }
+    *m_out = best ? best : largest;
+    return ESTH_OK;
+}

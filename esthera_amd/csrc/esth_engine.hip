@@ -211,18 +211,18 @@ void esth_engine_release(esth_engine* e, void* p, size_t bytes) {
 /* ============================================================ launchers */
 static int grid_for(uint64_t threads, int block) { return (int)((threads + block - 1) / block); }
 
-int esth_launch_mtgp(const uint32_t* d_params, uint32_t* d_status, const esth_mtgp_stream* d_streams,
-                     int nstreams, void* d_out, int float_mode, hipStream_t s) {
+int esth_launch_mtgp(const uint32_t* d_params, const uint32_t* d_status_in, uint32_t* d_status_out,
+                     const esth_mtgp_stream* d_streams, int nstreams, void* d_out, int float_mode, hipStream_t s) {
     if (nstreams <= 0) return ESTH_OK;
     const uint32_t mask = g_mtgp_table[3];
-    hipLaunchKernelGGL(k_mtgp, dim3(nstreams), dim3(ESTH_MTGP_THREADS), 0, s, d_params, mask, d_status,
-                       d_streams, (uint32_t*)d_out, float_mode);
+    hipLaunchKernelGGL(k_mtgp, dim3(nstreams), dim3(ESTH_MTGP_THREADS), 0, s, d_params, mask, d_status_in,
+                       d_status_out, d_streams, (uint32_t*)d_out, float_mode);
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }
 
-int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame,
-                              int log_w, float* w, hipStream_t s) {
+int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame, int log_w,
+                              float* w, hipStream_t s) {
     const int grid = grid_for(N / 2, 256);
     switch (model) {
     case ESTH_MODEL_ROBOTARM5:
@@ -279,13 +279,15 @@ int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_
                      float2* winner, int* perm, uint32_t t, float* bt_w, float* bt_s, float* part, int log_w,
                      hipStream_t s) {
     const int lm = log2u(m);
+#define ESTH_SORT_ARGS w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s
     switch (S) {
-    case 1: ESTH_DISPATCH_LOGM(launch_sort_t, 1, w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s); break;
-    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_UNI_S, w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s); break;
-    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_RA5_S, w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s); break;
-    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_DH6_S, w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s); break;
+    case 1: ESTH_DISPATCH_LOGM(launch_sort_t, 1, ESTH_SORT_ARGS); break;
+    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_UNI_S, ESTH_SORT_ARGS); break;
+    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_RA5_S, ESTH_SORT_ARGS); break;
+    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_DH6_S, ESTH_SORT_ARGS); break;
     default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
     }
+#undef ESTH_SORT_ARGS
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }
@@ -293,12 +295,18 @@ int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_
 int esth_launch_argmax(int S, const float2* winner, uint32_t G, int log_w, const float* part, const float* sorted,
                        uint32_t N, uint32_t m, float2* result, double* est, hipStream_t s) {
     switch (S) {
-    case 1: hipLaunchKernelGGL(k_argmax<1>, dim3(1), dim3(128), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
-    case ESTH_UNI_S: hipLaunchKernelGGL(k_argmax<ESTH_UNI_S>, dim3(1), dim3(128), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
-    case ESTH_RA5_S: hipLaunchKernelGGL(k_argmax<ESTH_RA5_S>, dim3(1), dim3(128), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
-    case ESTH_DH6_S: hipLaunchKernelGGL(k_argmax<ESTH_DH6_S>, dim3(1), dim3(128), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
+    case 1: hipLaunchKernelGGL(k_argmax<1>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
+    case ESTH_UNI_S: hipLaunchKernelGGL(k_argmax<ESTH_UNI_S>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
+    case ESTH_RA5_S: hipLaunchKernelGGL(k_argmax<ESTH_RA5_S>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
+    case ESTH_DH6_S: hipLaunchKernelGGL(k_argmax<ESTH_DH6_S>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
     default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
     }
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+int esth_launch_topt(const float* bt_w, uint32_t n, uint32_t t, uint32_t* top, hipStream_t s) {
+    hipLaunchKernelGGL(k_topt, dim3(1), dim3(1024), 0, s, bt_w, n, t, top);
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }
@@ -369,7 +377,7 @@ extern "C" int esth_stage_mtgp(uint32_t* d_status, int nstreams, int first_set, 
     ESTH_HIP(hipMalloc((void**)&dp, sizeof(g_mtgp_table)));
     ESTH_HIP(hipMemcpy(ds, hs.data(), sizeof(esth_mtgp_stream) * nstreams, hipMemcpyHostToDevice));
     ESTH_HIP(hipMemcpy(dp, g_mtgp_table, sizeof(g_mtgp_table), hipMemcpyHostToDevice));
-    int rc = esth_launch_mtgp(dp, d_status, ds, nstreams, d_out, float_mode, 0);
+    int rc = esth_launch_mtgp(dp, d_status, d_status, ds, nstreams, d_out, float_mode, 0);
     ESTH_HIP(hipDeviceSynchronize());
     hipFree(ds);
     hipFree(dp);

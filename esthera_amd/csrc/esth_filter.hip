@@ -27,12 +27,20 @@ struct esth_filter {
     unsigned char params[sizeof(esth_unicycle_params) > sizeof(esth_dh6_params) ? sizeof(esth_unicycle_params)
                                                                                  : sizeof(esth_dh6_params)];
     hipStream_t stream = nullptr;
+    hipStream_t rng_stream = nullptr;   /* MTGP of the next step runs here, overlapped */
     /* device */
-    float *state = nullptr, *tmp = nullptr, *w = nullptr, *R = nullptr, *bt_w = nullptr, *bt_s = nullptr,
+    float *state = nullptr, *tmp = nullptr, *w = nullptr, *bt_w = nullptr, *bt_s = nullptr,
           *part = nullptr, *frames = nullptr;
+    /* random words: R[k & 1] holds step k's words, generated from status[k & 1]
+     * into status[(k + 1) & 1] */
+    float* R[2] = {nullptr, nullptr};
+    uint32_t* status[2] = {nullptr, nullptr};
+    bool rng_ready = false;             /* R[steps & 1] already holds the next step's words */
+    bool prefetch = true;
+    hipEvent_t ev_rng_done[2], ev_consumed[2], ev_fork;
     float2 *winner = nullptr, *result = nullptr;
     double* est = nullptr;
-    uint32_t *status = nullptr, *top = nullptr;
+    uint32_t* top = nullptr;
     esth_mtgp_stream* streams = nullptr;
     std::vector<std::pair<void*, size_t>> allocs;
     /* host */
@@ -41,7 +49,7 @@ struct esth_filter {
     bool frame_used[ESTH_FRAME_RING] = {};
     int slot = 0;
     hipEvent_t ev[7];
-    bool timing = false;
+    int timing = 0;   /* 0 off, 1 sample+weight only, 2 every stage */
     /* per-step stage events while timing is on: 7 per step, up to ESTH_TIMING_STEPS steps */
     std::vector<hipEvent_t> tev;
     int tsteps = 0;
@@ -69,6 +77,9 @@ static void filter_free(esth_filter* f) {
     if (f->h_frames) hipHostFree(f->h_frames);
     for (int i = 0; i < ESTH_FRAME_RING; ++i) hipEventDestroy(f->frame_ev[i]);
     for (int i = 0; i < 7; ++i) hipEventDestroy(f->ev[i]);
+    for (int i = 0; i < 2; ++i) { hipEventDestroy(f->ev_rng_done[i]); hipEventDestroy(f->ev_consumed[i]); }
+    hipEventDestroy(f->ev_fork);
+    if (f->rng_stream) hipStreamDestroy(f->rng_stream);
     for (auto ev : f->tev) hipEventDestroy(ev);
     if (f->stream) hipStreamDestroy(f->stream);
     delete f;
@@ -104,6 +115,11 @@ extern "C" int esth_filter_create(esth_engine* e, const esth_filter_desc* d, con
     if (d->topology == ESTH_TOPO_TORUS && (f->dimx <= 0 || f->dimy <= 0)) esth_torus_dims(f->G, &f->dimx, &f->dimy);
     for (int i = 0; i < ESTH_FRAME_RING; ++i) hipEventCreateWithFlags(&f->frame_ev[i], hipEventDisableTiming);
     for (int i = 0; i < 7; ++i) hipEventCreate(&f->ev[i]);
+    for (int i = 0; i < 2; ++i) {
+        hipEventCreateWithFlags(&f->ev_rng_done[i], hipEventDisableTiming);
+        hipEventCreateWithFlags(&f->ev_consumed[i], hipEventDisableTiming);
+    }
+    hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming);
     int rc = ESTH_OK;
     const size_t SN = (size_t)S * f->N;
     const size_t tG = (size_t)(f->t ? f->t : 1) * f->G;
@@ -111,7 +127,8 @@ extern "C" int esth_filter_create(esth_engine* e, const esth_filter_desc* d, con
     FA(f->state, sizeof(float) * SN);
     FA(f->tmp, sizeof(float) * SN);
     FA(f->w, sizeof(float) * f->N);
-    FA(f->R, sizeof(float) * f->words);
+    FA(f->R[0], sizeof(float) * f->words);
+    FA(f->R[1], sizeof(float) * f->words);
     FA(f->bt_w, sizeof(float) * tG);
     FA(f->bt_s, sizeof(float) * tG * S);
     FA(f->part, sizeof(float) * (size_t)f->G * (2 + S));
@@ -119,12 +136,15 @@ extern "C" int esth_filter_create(esth_engine* e, const esth_filter_desc* d, con
     FA(f->winner, sizeof(float2) * f->G);
     FA(f->result, sizeof(float2));
     FA(f->est, sizeof(double) * (4 + 2 * ESTH_MAX_STATE_DIM));
-    FA(f->status, sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K);
+    FA(f->status[0], sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K);
+    FA(f->status[1], sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K);
     FA(f->top, sizeof(uint32_t) * 64);
+
     FA(f->streams, sizeof(esth_mtgp_stream) * (size_t)f->K);
 #undef FA
     if (rc != ESTH_OK) { filter_free(f); return rc; }
     hipError_t err = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&f->rng_stream, hipStreamNonBlocking);
     if (err == hipSuccess) err = hipHostMalloc((void**)&f->h_frames, sizeof(float) * ESTH_FRAME_FLOATS * ESTH_FRAME_RING, 0);
     if (err != hipSuccess) { filter_free(f); return esth_fail(ESTH_ERR_DEVICE, "stream/pinned: %s", hipGetErrorString(err)); }
     /* RNG status: make_kernel_data32 (mtgp32-opencl.c:31-60), stream j seeded seed + j + 1 */
@@ -134,9 +154,10 @@ extern "C" int esth_filter_create(esth_engine* e, const esth_filter_desc* d, con
         esth_mtgp_seed_state(j, d->seed + (uint32_t)j + 1u, st.data() + (size_t)j * ESTH_MTGP_N);
         sd[j] = {(uint32_t)j, (uint32_t)f->size, (uint64_t)j * f->size};
     }
-    err = hipMemcpy(f->status, st.data(), st.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    err = hipMemcpy(f->status[0], st.data(), st.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(f->streams, sd.data(), sd.size() * sizeof(esth_mtgp_stream), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(f->w, 0, sizeof(float) * f->N);
+
     if (err == hipSuccess) err = hipMemset(f->est, 0, sizeof(double) * (4 + 2 * ESTH_MAX_STATE_DIM));
     if (err != hipSuccess) { filter_free(f); return esth_fail(ESTH_ERR_DEVICE, "init upload: %s", hipGetErrorString(err)); }
     *out = f;
@@ -163,38 +184,81 @@ extern "C" int esth_filter_init(esth_filter* f, const float* soa) {
     return ESTH_OK;
 }
 
-/* the kernels of one step, reading the frame at d_frame */
+/* MTGP words for step k into R[k & 1] */
+static int launch_rng(esth_filter* f, uint64_t k, hipStream_t s) {
+    const int p = (int)(k & 1);
+    return esth_launch_mtgp(f->e->d_params, f->status[p], f->status[p ^ 1], f->streams, f->K, f->R[p], 1, s);
+}
+
+/* RNG side of step n = f->steps, before its kernels: if the step's words are
+ * not ready they are generated first on the main stream; with prefetch on,
+ * step n+1's words are generated on the RNG stream while step n runs (their
+ * buffer was last read by step n-1: the ev_consumed event orders that). */
+static int rng_before_step(esth_filter* f) {
+    hipStream_t s = f->stream;
+    const uint64_t n = f->steps;
+    const int p = (int)(n & 1);
+    bool forked = false;
+    if (!f->rng_ready) {
+        ESTH_TRY(launch_rng(f, n, s));
+        if (f->prefetch) {   /* step n+1's words continue from this status */
+            ESTH_HIP(hipEventRecord(f->ev_fork, s));
+            ESTH_HIP(hipStreamWaitEvent(f->rng_stream, f->ev_fork, 0));
+            forked = true;
+        }
+    } else {
+        ESTH_HIP(hipStreamWaitEvent(s, f->ev_rng_done[p], 0));
+    }
+    if (f->prefetch) {
+        if (!forked && n > 0) ESTH_HIP(hipStreamWaitEvent(f->rng_stream, f->ev_consumed[p ^ 1], 0));
+        ESTH_TRY(launch_rng(f, n + 1, f->rng_stream));
+        ESTH_HIP(hipEventRecord(f->ev_rng_done[p ^ 1], f->rng_stream));
+    }
+    return ESTH_OK;
+}
+
+static int rng_after_step(esth_filter* f) {
+    if (f->prefetch) ESTH_HIP(hipEventRecord(f->ev_consumed[f->steps & 1], f->stream));
+    f->rng_ready = f->prefetch;
+    return ESTH_OK;
+}
+
+/* The main-stream kernels of step n = f->steps, reading the frame at d_frame
+ * and the step's words in R[n & 1].  ev (optional): timing events. */
 static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
     const bool timing = ev != nullptr;
+    const bool all = timing && f->timing >= 2;
     hipStream_t s = f->stream;
     const esth_filter_desc& d = f->d;
     const int log_w = d.weights == ESTH_WEIGHTS_LOG;
     const uint32_t N = (uint32_t)f->N;
-    if (timing) ESTH_HIP(hipEventRecord(ev[0], s));
-    /* generate_normal_random: MTGP words (pf_generic.c:191-232) */
-    ESTH_TRY(esth_launch_mtgp(f->e->d_params, f->status, f->streams, f->K, f->R, 1, s));
+    const int p = (int)(f->steps & 1);
+    float* R = f->R[p];
     if (timing) ESTH_HIP(hipEventRecord(ev[1], s));
     /* sampling_importance with Box-Muller fused (pf_generic.c:521-533) */
-    ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, f->R, d_frame, log_w, f->w, s));
+    ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, R, d_frame, log_w, f->w, s));
     if (timing) ESTH_HIP(hipEventRecord(ev[2], s));
     /* block_sort (pf_generic.c:572-585) */
     ESTH_TRY(esth_launch_sort(f->S, f->m, f->w, f->state, f->tmp, N, f->G, f->winner, nullptr, f->t, f->bt_w,
                               f->bt_s, f->part, log_w, s));
-    if (timing) ESTH_HIP(hipEventRecord(ev[3], s));
+    if (all) ESTH_HIP(hipEventRecord(ev[3], s));
     /* reduce (pf_generic.c:604-615), result kept on the device */
     ESTH_TRY(esth_launch_argmax(f->S, f->winner, f->G, log_w, f->part, f->tmp, N, f->m, f->result, f->est, s));
-    if (timing) ESTH_HIP(hipEventRecord(ev[4], s));
+    if (all) ESTH_HIP(hipEventRecord(ev[4], s));
     /* exchange (pf_generic.c:702-715) */
     ESTH_TRY(esth_launch_exchange(f->tmp, f->w, f->S, N, f->m, f->G, f->t, d.topology, f->dimx, f->dimy, f->bt_w,
                                   f->bt_s, f->top, s));
-    if (timing) ESTH_HIP(hipEventRecord(ev[5], s));
+    if (all) ESTH_HIP(hipEventRecord(ev[5], s));
     /* resampling (pf_generic.c:732-757), uniforms at rand_offset = N*V */
-    ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, f->R + (size_t)N * f->V, N, f->m, f->G,
+    ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m, f->G,
                                   d.resampling_threshold, d.adaptive, log_w, nullptr, s));
-    if (timing) ESTH_HIP(hipEventRecord(ev[6], s));
+    if (all) ESTH_HIP(hipEventRecord(ev[6], s));
     return ESTH_OK;
 }
 
+/* Capture the main-stream part of the step of parity `slot`: the frame upload
+ * from the pinned slot and the step's kernels (the R buffer is fixed by the
+ * parity).  The RNG runs outside the graph on its own stream. */
 static int capture_graph(esth_filter* f, int slot, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     ESTH_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
@@ -214,33 +278,37 @@ static int capture_graph(esth_filter* f, int slot, hipGraphExec_t* out) {
 extern "C" int esth_filter_step(esth_filter* f, const void* control, const void* measurement, float dt) {
     if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
     ESTH_HIP(hipSetDevice(f->e->device));
-    /* frame slot: graph mode alternates two fixed slots, eager mode a ring */
+    /* frame slot: graphs use the step parity, eager steps a ring */
     int slot = f->use_graph ? (int)(f->steps & 1) : f->slot;
     if (f->frame_used[slot]) ESTH_HIP(hipEventSynchronize(f->frame_ev[slot]));
     float* hf = f->h_frames + (size_t)slot * ESTH_FRAME_FLOATS;
     if (esth_build_frame_impl(f->d.model, f->params, control, measurement, dt, hf))
         return esth_fail(ESTH_ERR_INVALID, "bad control/measurement for model %d", f->d.model);
+    if (f->use_graph && !f->gexec[slot]) ESTH_TRY(capture_graph(f, slot, &f->gexec[slot]));
+    hipEvent_t* ev = nullptr;
+    if (!f->use_graph && f->timing && f->tsteps < ESTH_TIMING_STEPS) {
+        if ((size_t)(f->tsteps + 1) * 7 > f->tev.size()) {
+            for (int i = 0; i < 7; ++i) {
+                hipEvent_t e;
+                ESTH_HIP(hipEventCreate(&e));
+                f->tev.push_back(e);
+            }
+        }
+        ev = &f->tev[(size_t)f->tsteps * 7];
+        f->tsteps++;
+    }
+    float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
+    if (!f->use_graph)   /* the frame copy goes ahead of the wait for the step's words */
+        ESTH_HIP(hipMemcpyAsync(dfr, hf, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice, f->stream));
+    if (ev && f->timing >= 2) ESTH_HIP(hipEventRecord(ev[0], f->stream));
+    ESTH_TRY(rng_before_step(f));
     if (f->use_graph) {
-        if (!f->gexec[slot]) ESTH_TRY(capture_graph(f, slot, &f->gexec[slot]));
         ESTH_HIP(hipGraphLaunch(f->gexec[slot], f->stream));
     } else {
-        float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
-        ESTH_HIP(hipMemcpyAsync(dfr, hf, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice, f->stream));
-        hipEvent_t* ev = nullptr;
-        if (f->timing && f->tsteps < ESTH_TIMING_STEPS) {
-            if ((size_t)(f->tsteps + 1) * 7 > f->tev.size()) {
-                for (int i = 0; i < 7; ++i) {
-                    hipEvent_t e;
-                    ESTH_HIP(hipEventCreate(&e));
-                    f->tev.push_back(e);
-                }
-            }
-            ev = &f->tev[(size_t)f->tsteps * 7];
-            f->tsteps++;
-        }
         ESTH_TRY(enqueue_step(f, dfr, ev));
         f->slot = (f->slot + 1) % ESTH_FRAME_RING;
     }
+    ESTH_TRY(rng_after_step(f));
     ESTH_HIP(hipEventRecord(f->frame_ev[slot], f->stream));
     f->frame_used[slot] = true;
     f->steps++;
@@ -250,13 +318,26 @@ extern "C" int esth_filter_step(esth_filter* f, const void* control, const void*
 extern "C" int esth_filter_use_graph(esth_filter* f, int enable) {
     if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
     ESTH_HIP(hipStreamSynchronize(f->stream));
+    ESTH_HIP(hipStreamSynchronize(f->rng_stream));
     f->use_graph = enable != 0;
+    return ESTH_OK;
+}
+
+/* Overlap the next step's MTGP words with the current step (default on). */
+extern "C" int esth_filter_set_prefetch(esth_filter* f, int enable) {
+    if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    ESTH_HIP(hipStreamSynchronize(f->rng_stream));
+    for (int i = 0; i < 2; ++i)
+        if (f->gexec[i]) { hipGraphExecDestroy(f->gexec[i]); f->gexec[i] = nullptr; }
+    f->prefetch = enable != 0;
     return ESTH_OK;
 }
 
 extern "C" int esth_filter_sync(esth_filter* f) {
     if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
     ESTH_HIP(hipStreamSynchronize(f->stream));
+    ESTH_HIP(hipStreamSynchronize(f->rng_stream));
     return ESTH_OK;
 }
 
@@ -269,7 +350,7 @@ extern "C" int esth_filter_stream(esth_filter* f, void** s) {
 extern "C" int esth_filter_set_timing(esth_filter* f, int on) {
     if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
     ESTH_HIP(hipStreamSynchronize(f->stream));
-    f->timing = on != 0;
+    f->timing = on < 0 ? 0 : (on > 2 ? 2 : on);
     f->tsteps = 0;
     return ESTH_OK;
 }
@@ -283,12 +364,14 @@ extern "C" int esth_filter_stage_times(esth_filter* f, float us[6]) {
     for (int k = 0; k < f->tsteps; ++k) {
         hipEvent_t* ev = &f->tev[(size_t)k * 7];
         for (int i = 0; i < 6; ++i) {
+            if (f->timing < 2 && i != 1) continue;
             float ms = 0.0f;
             ESTH_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
             acc[i] += ms;
         }
     }
-    for (int i = 0; i < 6; ++i) us[i] = (float)(acc[i] * 1000.0 / f->tsteps);
+    for (int i = 0; i < 6; ++i)
+        us[i] = (f->timing < 2 && i != 1) ? NAN : (float)(acc[i] * 1000.0 / f->tsteps);
     return ESTH_OK;
 }
 
@@ -333,6 +416,7 @@ extern "C" int esth_filter_snapshot(esth_filter* f, void* buf, size_t bytes) {
     ESTH_TRY(esth_filter_snapshot_size(f, &need));
     if (!buf || bytes < need) return esth_fail(ESTH_ERR_INVALID, "snapshot buffer %zu < %zu", bytes, need);
     ESTH_HIP(hipStreamSynchronize(f->stream));
+    ESTH_HIP(hipStreamSynchronize(f->rng_stream));
     uint64_t* hdr = (uint64_t*)buf;
     hdr[0] = ESTH_SNAP_MAGIC; hdr[1] = (uint64_t)f->S; hdr[2] = f->N; hdr[3] = (uint64_t)f->K; hdr[4] = f->steps;
     hdr[5] = f->d.model; hdr[6] = f->m; hdr[7] = f->G;
@@ -341,7 +425,8 @@ extern "C" int esth_filter_snapshot(esth_filter* f, void* buf, size_t bytes) {
     p += sizeof(float) * (size_t)f->S * f->N;
     ESTH_HIP(hipMemcpy(p, f->w, sizeof(float) * f->N, hipMemcpyDeviceToHost));
     p += sizeof(float) * f->N;
-    ESTH_HIP(hipMemcpy(p, f->status, sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K, hipMemcpyDeviceToHost));
+    /* the status that generates the next step's words (any prefetched words are regenerated on restore) */
+    ESTH_HIP(hipMemcpy(p, f->status[f->steps & 1], sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K, hipMemcpyDeviceToHost));
     return ESTH_OK;
 }
 
@@ -354,13 +439,15 @@ extern "C" int esth_filter_restore(esth_filter* f, const void* buf, size_t bytes
         hdr[5] != (uint64_t)f->d.model || hdr[6] != f->m || hdr[7] != f->G)
         return esth_fail(ESTH_ERR_INVALID, "snapshot does not match this filter");
     ESTH_HIP(hipStreamSynchronize(f->stream));
+    ESTH_HIP(hipStreamSynchronize(f->rng_stream));
     const char* p = (const char*)buf + 8 * sizeof(uint64_t);
     ESTH_HIP(hipMemcpy(f->state, p, sizeof(float) * (size_t)f->S * f->N, hipMemcpyHostToDevice));
     p += sizeof(float) * (size_t)f->S * f->N;
     ESTH_HIP(hipMemcpy(f->w, p, sizeof(float) * f->N, hipMemcpyHostToDevice));
     p += sizeof(float) * f->N;
-    ESTH_HIP(hipMemcpy(f->status, p, sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K, hipMemcpyHostToDevice));
     f->steps = hdr[4];
+    ESTH_HIP(hipMemcpy(f->status[f->steps & 1], p, sizeof(uint32_t) * ESTH_MTGP_N * (size_t)f->K, hipMemcpyHostToDevice));
+    f->rng_ready = false;
     return ESTH_OK;
 }
 

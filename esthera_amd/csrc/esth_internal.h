@@ -48,8 +48,8 @@ int esth_params_size(int model, size_t* sz);
 const uint32_t* esth_param_table();
 
 /* launchers shared by filters, batches and stage entry points */
-int esth_launch_mtgp(const uint32_t* d_params, uint32_t* d_status, const esth_mtgp_stream* d_streams,
-                     int nstreams, void* d_out, int float_mode, hipStream_t s);
+int esth_launch_mtgp(const uint32_t* d_params, const uint32_t* d_status_in, uint32_t* d_status_out,
+                     const esth_mtgp_stream* d_streams, int nstreams, void* d_out, int float_mode, hipStream_t s);
 int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame,
                               int log_w, float* w, hipStream_t s);
 int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_out, uint32_t N,
@@ -64,5 +64,7 @@ int esth_launch_exchange(float* st, float* w, int S, uint32_t N, uint32_t m, uin
 int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w,
                          const float* rnd, uint32_t N, uint32_t m, uint32_t G, float thr,
                          int adaptive, int log_w, int* sel, hipStream_t s);
+/* the global top t of the compact best-t buffer (all-to-all exchange) */
+int esth_launch_topt(const float* bt_w, uint32_t n, uint32_t t, uint32_t* top, hipStream_t s);
 
 #endif

@@ -47,14 +47,46 @@ __device__ __forceinline__ double esth_shfl_xor_d(double v, int mask) {
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+/* xor-lane exchange with a compile-time mask: DPP quad_perm (1, 2), DPP
+ * row_ror:8 (8), ds_swizzle SWAP mode (4, 16), ds_bpermute (32). */
+template <int MASK>
+__device__ __forceinline__ int esth_xor_lane(int v) {
+    if constexpr (MASK == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (MASK == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (MASK == 4) return __builtin_amdgcn_ds_swizzle(v, 0x101F);
+    else if constexpr (MASK == 8) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);
+    else if constexpr (MASK == 16) return __builtin_amdgcn_ds_swizzle(v, 0x401F);
+    else return esth_shfl_xor_i(v, MASK);
+}
+template <int MASK>
+__device__ __forceinline__ float esth_xor_lane_f(float v) {
+    return __builtin_bit_cast(float, esth_xor_lane<MASK>(__builtin_bit_cast(int, v)));
+}
+/* butterfly sum over lane groups of size 2^LOG (fixed pairing order) */
+template <int K, int LIM>
+__device__ __forceinline__ float esth_bfly_sum(float v) {
+    if constexpr (K < LIM) return esth_bfly_sum<2 * K, LIM>(v + esth_xor_lane_f<K>(v));
+    else return v;
+}
+template <int K, int LIM>
+__device__ __forceinline__ float esth_bfly_max(float v) {
+    if constexpr (K < LIM) {
+        float o = esth_xor_lane_f<K>(v);
+        return esth_bfly_max<2 * K, LIM>(o > v ? o : v);
+    } else {
+        return v;
+    }
+}
+
 /* ================================================================= MTGP32
  * One work-group of 256 per stream, 768-word status ring in LDS; each round
  * produces 768 words in three 256-wide phases, each one the recursion
  * x[n] = rec(x[n-351], x[n-350], x[n-351+pos]) for 256 consecutive n
  * (mtgp32-opencl.cl:186-242).  Output is the sequential stream of the
  * stream's parameter set, continuing across launches through `status`. */
-__global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ params, uint32_t mask,
-                                              uint32_t* __restrict__ status,
+static __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ params, uint32_t mask,
+                                              const uint32_t* __restrict__ status_in,
+                                              uint32_t* __restrict__ status_out,
                                               const esth_mtgp_stream* __restrict__ streams,
                                               uint32_t* __restrict__ out, int float_mode) {
     __shared__ uint32_t st[ESTH_MTGP_LARGE];
@@ -68,10 +100,11 @@ __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ param
         tbl[tid] = p[4 + tid];
         ttbl[tid] = float_mode ? p[36 + tid] : p[20 + tid];
     }
-    uint32_t* gst = status + (size_t)blockIdx.x * ESTH_MTGP_N;
-    st[ESTH_MTGP_LARGE - ESTH_MTGP_N + tid] = gst[tid];
+    const uint32_t* gin = status_in + (size_t)blockIdx.x * ESTH_MTGP_N;
+    uint32_t* gst = status_out + (size_t)blockIdx.x * ESTH_MTGP_N;
+    st[ESTH_MTGP_LARGE - ESTH_MTGP_N + tid] = gin[tid];
     if (tid < ESTH_MTGP_N - ESTH_MTGP_THREADS)
-        st[ESTH_MTGP_LARGE - ESTH_MTGP_N + ESTH_MTGP_THREADS + tid] = gst[ESTH_MTGP_THREADS + tid];
+        st[ESTH_MTGP_LARGE - ESTH_MTGP_N + ESTH_MTGP_THREADS + tid] = gin[ESTH_MTGP_THREADS + tid];
     __syncthreads();
     uint32_t* o = out + sd.out_off;
     const int size = (int)sd.size;
@@ -104,7 +137,7 @@ __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ param
 }
 
 /* ============================================================ Box-Muller */
-__global__ void k_boxmuller(float* __restrict__ d, int n) {
+static __global__ void k_boxmuller(float* __restrict__ d, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (2 * i < n) {
         float z0, z1;
@@ -139,13 +172,11 @@ struct esth_model_traits<ESTH_MODEL_DH6> {
 };
 
 template <int MODEL>
-__global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state, uint32_t N,
-                                                       const float* __restrict__ R,
-                                                       const float* __restrict__ frame, int log_w,
-                                                       float* __restrict__ w) {
+__device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ state, uint32_t N,
+                                                        const float* __restrict__ R,
+                                                        const float* __restrict__ frame, int log_w,
+                                                        float* __restrict__ w, uint32_t p) {
     using T = esth_model_traits<MODEL>;
-    const uint32_t p = 2u * (blockIdx.x * blockDim.x + threadIdx.x);
-    if (p >= N) return;
     float s0[T::S], s1[T::S], z0[T::V], z1[T::V];
 #pragma unroll
     for (int c = 0; c < T::S; ++c) {
@@ -158,13 +189,41 @@ __global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state
         float2 u = *reinterpret_cast<const float2*>(R + (size_t)k * N + p);
         esth_boxmuller_pair(u.x, u.y, &z0[k], &z1[k]);
     }
-    float lw0 = T::iw(s0, z0, frame);
-    float lw1 = T::iw(s1, z1, frame);
+    float lw0, lw1;
+    if constexpr (MODEL == ESTH_MODEL_UNICYCLE) {
+        /* both particles walk the landmarks together: one scalar load of a
+         * landmark feeds two independent chains (same operations per particle) */
+        float c0, n0, c1, n1;
+        esth_unicycle_propagate(s0, z0, frame, &c0, &n0);
+        esth_unicycle_propagate(s1, z1, frame, &c1, &n1);
+        const int L = (int)frame[8];
+        float r0 = 0.0f, b0 = 0.0f, r1 = 0.0f, b1 = 0.0f;
+#pragma unroll 2
+        for (int j = 0; j < L; ++j) {
+            const float* lm = frame + ESTH_UNI_HDR + 5 * j;
+            esth_unicycle_landmark(lm, s0[0], s0[1], c0, n0, &r0, &b0);
+            esth_unicycle_landmark(lm, s1[0], s1[1], c1, n1, &r1, &b1);
+        }
+        lw0 = esth_unicycle_finish(r0, b0, frame);
+        lw1 = esth_unicycle_finish(r1, b1, frame);
+    } else {
+        lw0 = T::iw(s0, z0, frame);
+        lw1 = T::iw(s1, z1, frame);
+    }
 #pragma unroll
     for (int c = 0; c < T::S; ++c)
         *reinterpret_cast<float2*>(state + (size_t)c * N + p) = make_float2(s0[c], s1[c]);
     float2 wo = log_w ? make_float2(lw0, lw1) : make_float2(esth_expf(lw0), esth_expf(lw1));
     *reinterpret_cast<float2*>(w + p) = wo;
+}
+
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state, uint32_t N,
+                                                       const float* __restrict__ R,
+                                                       const float* __restrict__ frame, int log_w,
+                                                       float* __restrict__ w) {
+    const uint32_t p = 2u * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (p < N) esth_sample_weight_pair<MODEL>(state, N, R, frame, log_w, w, p);
 }
 
 /* ============================================================ block sort
@@ -182,59 +241,182 @@ struct esth_tile {
     static constexpr int LOGL = LOGM - LOGE;
 };
 
-template <int LOGM>
-__device__ __forceinline__ void esth_bitonic(float (&key)[esth_tile<LOGM>::E],
-                                             int (&idx)[esth_tile<LOGM>::E], int le) {
+template <int LOGM, int SIZE, int STRIDE>
+__device__ __forceinline__ void esth_bitonic_stage(float (&key)[esth_tile<LOGM>::E],
+                                                   int (&idx)[esth_tile<LOGM>::E], int le, int laneg) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E;
+    if constexpr (STRIDE < E) {
 #pragma unroll
-    for (int size = 2; size <= M; size <<= 1) {
-#pragma unroll
-        for (int stride = size / 2; stride > 0; stride >>= 1) {
-            if (stride < E) {
-#pragma unroll
-                for (int r = 0; r < E; ++r) {
-                    if ((r & stride) == 0) {
-                        const int a = r, b = r + stride;
-                        const bool d = (size == M) ? false : (((le + a) & size) != 0);
-                        if ((key[a] > key[b]) == d) {
-                            float tk = key[a]; key[a] = key[b]; key[b] = tk;
-                            int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;
-                        }
-                    }
-                }
-            } else {
-                const int lm = stride / E;
-                const bool lower = ((le / E) & lm) == 0;
-#pragma unroll
-                for (int r = 0; r < E; ++r) {
-                    const float other = esth_shfl_xor(key[r], lm);
-                    const int oidx = esth_shfl_xor_i(idx[r], lm);
-                    const bool d = (size == M) ? false : (((le + r) & size) != 0);
-                    const float kl = lower ? key[r] : other;
-                    const float ku = lower ? other : key[r];
-                    if ((kl > ku) == d) { key[r] = other; idx[r] = oidx; }
+        for (int r = 0; r < E; ++r) {
+            if ((r & STRIDE) == 0) {
+                const int a = r, b = r + STRIDE;
+                const bool d = (SIZE == M) ? false : (((le + a) & SIZE) != 0);
+                if ((key[a] > key[b]) == d) {
+                    float tk = key[a]; key[a] = key[b]; key[b] = tk;
+                    int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;
                 }
             }
+        }
+    } else {
+        constexpr int MASK = STRIDE / E;
+        const bool lower = (laneg & MASK) == 0;
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const float other = esth_xor_lane_f<MASK>(key[r]);
+            const int oidx = esth_xor_lane<MASK>(idx[r]);
+            const bool d = (SIZE == M) ? false : (((le + r) & SIZE) != 0);
+            const float kl = lower ? key[r] : other;
+            const float ku = lower ? other : key[r];
+            if ((kl > ku) == d) { key[r] = other; idx[r] = oidx; }
         }
     }
 }
 
+template <int LOGM, int SIZE, int STRIDE>
+__device__ __forceinline__ void esth_bitonic_strides(float (&key)[esth_tile<LOGM>::E],
+                                                     int (&idx)[esth_tile<LOGM>::E], int le, int laneg) {
+    if constexpr (STRIDE > 0) {
+        esth_bitonic_stage<LOGM, SIZE, STRIDE>(key, idx, le, laneg);
+        esth_bitonic_strides<LOGM, SIZE, STRIDE / 2>(key, idx, le, laneg);
+    }
+}
+
+template <int LOGM, int SIZE>
+__device__ __forceinline__ void esth_bitonic_sizes(float (&key)[esth_tile<LOGM>::E],
+                                                   int (&idx)[esth_tile<LOGM>::E], int le, int laneg) {
+    if constexpr (SIZE <= (1 << LOGM)) {
+        esth_bitonic_strides<LOGM, SIZE, SIZE / 2>(key, idx, le, laneg);
+        esth_bitonic_sizes<LOGM, 2 * SIZE>(key, idx, le, laneg);
+    }
+}
+
+template <int LOGM>
+__device__ __forceinline__ void esth_bitonic(float (&key)[esth_tile<LOGM>::E],
+                                             int (&idx)[esth_tile<LOGM>::E], int le) {
+    esth_bitonic_sizes<LOGM, 2>(key, idx, le, le / esth_tile<LOGM>::E);
+}
+
+/* ============================================================== argmax
+ * reduce (reduce.cl:19-55), one work-group of 128 work-items: a strided pass
+ * (the later element wins ties), then a tree (`other` wins ties).  Here the
+ * 128 work-items are the first two waves of a wider group; each walks its
+ * strided elements in order, loading them 16 at a time so the loads overlap.
+ * The whole group then combines the sub-filters' posterior partial sums in
+ * double and copies the best particle (rank 0 of the winning sub-filter,
+ * read before any exchange).
+ * est (doubles): [0] max log-weight, [1] sum w, [2..2+S) sum w x, [2+S] best w,
+ * [3+S] best g, [4+S..4+2S) best particle. */
+template <int S, int NT>
+__device__ __forceinline__ void esth_argmax_block(const float2* __restrict__ winner, uint32_t G, int log_w,
+                                                  const float* __restrict__ part,
+                                                  const float* __restrict__ sorted, uint32_t N, uint32_t m,
+                                                  float2* __restrict__ result, double* __restrict__ est,
+                                                  float2* sc, float* smax, double* sd) {
+    constexpr int NW = NT / 64;
+    const int l = threadIdx.x;
+    const int lane = l & 63, wid = l >> 6;
+    if (l < 128) {
+        float2 acc = make_float2(log_w ? -INFINITY : -1.0f, -1.0f);
+        for (uint32_t g0 = (uint32_t)l; g0 < G; g0 += 128u * 16u) {
+            float2 e[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                uint32_t gi = g0 + 128u * (uint32_t)j;
+                e[j] = gi < G ? winner[gi] : make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (g0 + 128u * (uint32_t)j < G) acc = (acc.x > e[j].x) ? acc : e[j];
+        }
+        sc[l] = acc;
+    }
+    float bmax = -INFINITY;
+    if (log_w) {
+        for (uint32_t gi = (uint32_t)l; gi < G; gi += (uint32_t)NT) {
+            float pm = part[(size_t)gi * (2 + S)];
+            bmax = pm > bmax ? pm : bmax;
+        }
+        bmax = esth_bfly_max<1, 64>(bmax);
+        if (lane == 0) smax[wid] = bmax;
+    }
+    __syncthreads();
+    if (l < 64) {   /* the reference tree over the 128 strided results */
+        for (int off = 64; off > 0; off >>= 1) {
+            if (l < off) {
+                float2 other = sc[l + off];
+                float2 mine = sc[l];
+                sc[l] = (mine.x > other.x) ? mine : other;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
+    }
+    float gmax = 0.0f;
+    if (log_w) {
+        gmax = smax[0];
+        for (int i = 1; i < NW; ++i) gmax = smax[i] > gmax ? smax[i] : gmax;
+    }
+    double a[2 + S];
+#pragma unroll
+    for (int c = 0; c < 2 + S; ++c) a[c] = 0.0;
+    for (uint32_t gi = (uint32_t)l; gi < G; gi += (uint32_t)NT) {
+        const float* pg = part + (size_t)gi * (2 + S);
+        double scale = log_w ? (double)esth_expf(pg[0] - gmax) : 1.0;
+#pragma unroll
+        for (int c = 1; c < 2 + S; ++c) a[c] += (double)pg[c] * scale;
+    }
+#pragma unroll
+    for (int c = 1; c < 2 + S; ++c) {
+        double v = a[c];
+        for (int k = 32; k > 0; k >>= 1) v += esth_shfl_xor_d(v, k);
+        if (lane == 0) sd[wid * (2 + S) + c] = v;
+    }
+    __syncthreads();
+    if (l == 0) {
+        result[0] = sc[0];
+        est[0] = (double)gmax;
+#pragma unroll
+        for (int c = 1; c < 2 + S; ++c) {
+            double v = 0.0;
+            for (int i = 0; i < NW; ++i) v += sd[i * (2 + S) + c];
+            est[c] = v;
+        }
+        est[2 + S] = (double)sc[0].x;
+        est[3 + S] = (double)sc[0].y;
+    }
+    if (sorted && l < S) {
+        int gb = (int)sc[0].y;
+        if (gb >= 0 && (uint32_t)gb < G) est[4 + S + l] = (double)sorted[(size_t)l * N + (size_t)gb * m];
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(1024) void k_argmax(const float2* __restrict__ winner, uint32_t G,
+                                                 int log_w, const float* __restrict__ part,
+                                                 const float* __restrict__ sorted, uint32_t N,
+                                                 uint32_t m, float2* __restrict__ result,
+                                                 double* __restrict__ est) {
+    __shared__ float2 sc[128];
+    __shared__ float smax[16];
+    __shared__ double sd[16 * (2 + S)];
+    esth_argmax_block<S, 1024>(winner, G, log_w, part, sorted, N, m, result, est, sc, smax, sd);
+}
+
 /* Sorted weights in place, gathered state to `sorted`, winner (w[0], g),
- * compact best t, and the posterior partial sums of the sub-filter. */
+ * compact best t, and the posterior partial sums of the sub-filter.  (The
+ * argmax stays a separate launch: folding it into the last work-group needs an
+ * agent-scope release per work-group, which on the 8-XCD part writes back the
+ * L2 and costs more than the launch it saves.) */
 template <int LOGM, int S>
-__global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float* __restrict__ st_in,
-                                              float* __restrict__ st_out, uint32_t N, uint32_t G,
-                                              float2* __restrict__ winner, int* __restrict__ perm,
-                                              uint32_t t, float* __restrict__ bt_w,
-                                              float* __restrict__ bt_s, float* __restrict__ part,
-                                              int log_w) {
+__device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float* __restrict__ st_in,
+                                              float* __restrict__ st_out, uint32_t N, uint32_t G, uint32_t g,
+                                              bool valid, float2* __restrict__ winner, int* __restrict__ perm,
+                                              uint32_t t, float* __restrict__ bt_w, float* __restrict__ bt_s,
+                                              float* __restrict__ part, int log_w) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE;
-    const uint32_t g = wave * (ESTH_WAVE / LPS) + (uint32_t)(lane / LPS);
-    if (g >= G) return;   /* a whole sub-filter leaves together */
     const int le = (lane % LPS) * E;
     const size_t base = (size_t)g * M;
     float key[E];
@@ -242,14 +424,7 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
 #pragma unroll
     for (int r = 0; r < E; ++r) { key[r] = w[base + le + r]; idx[r] = le + r; }
     esth_bitonic<LOGM>(key, idx, le);
-#pragma unroll
-    for (int r = 0; r < E; ++r) w[base + le + r] = key[r];
-    if (perm) {
-#pragma unroll
-        for (int r = 0; r < E; ++r) perm[base + le + r] = idx[r];
-    }
-    /* gather (particle_set of block_sort, pf_generic.cl:281-283) and
-     * posterior partial sums: block max M_g = key of position 0 */
+    /* block max M_g = key of position 0 */
     const int lead = lane - (lane % LPS);
     const float mx = esth_shfl_idx(key[0], lead);
     float wt[E];
@@ -260,6 +435,15 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
         sw = sw + wt[r];
     }
     float sx[S];
+    if (valid) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) w[base + le + r] = key[r];
+        if (perm) {
+#pragma unroll
+            for (int r = 0; r < E; ++r) perm[base + le + r] = idx[r];
+        }
+    }
+    /* gather (particle_set of block_sort, pf_generic.cl:281-283) and partial sums */
 #pragma unroll
     for (int c = 0; c < S; ++c) {
         const float* src = st_in + (size_t)c * N + base;
@@ -267,25 +451,22 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
         float acc = 0.0f;
 #pragma unroll
         for (int r = 0; r < E; ++r) {
-            float v = src[idx[r]];
-            dst[le + r] = v;
+            float v = src[idx[r] & (M - 1)];
+            if (valid) dst[le + r] = v;
             acc = esth_fma(wt[r], v, acc);
-            if (t > 0 && (uint32_t)(le + r) < t) bt_s[(size_t)c * G * t + (size_t)g * t + le + r] = v;
+            if (valid && t > 0 && (uint32_t)(le + r) < t) bt_s[(size_t)c * G * t + (size_t)g * t + le + r] = v;
         }
         sx[c] = acc;
     }
-    if (t > 0) {
+    if (valid && t > 0) {
 #pragma unroll
         for (int r = 0; r < E; ++r)
             if ((uint32_t)(le + r) < t) bt_w[(size_t)g * t + le + r] = key[r];
     }
+    sw = esth_bfly_sum<1, LPS>(sw);
 #pragma unroll
-    for (int k = 1; k < LPS; k <<= 1) {
-        sw = sw + esth_shfl_xor(sw, k);
-#pragma unroll
-        for (int c = 0; c < S; ++c) sx[c] = sx[c] + esth_shfl_xor(sx[c], k);
-    }
-    if (le == 0) {
+    for (int c = 0; c < S; ++c) sx[c] = esth_bfly_sum<1, LPS>(sx[c]);
+    if (valid && le == 0) {
         winner[g] = make_float2(key[0], (float)g);
         float* pg = part + (size_t)g * (2 + S);
         pg[0] = log_w ? mx : 0.0f;
@@ -295,79 +476,38 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
     }
 }
 
-/* ============================================================== argmax
- * reduce (reduce.cl:19-55) as one work-group of 128: strided pass (later
- * element wins ties), then the tree (`other` wins ties).  The same group then
- * combines the sub-filters' posterior partial sums in double and copies the
- * best particle (rank 0 of the winning sub-filter, read before any exchange).
- * est (doubles): [0] max log-weight, [1] sum w, [2..2+S) sum w x, [2+S] best w,
- * [3+S] best g, [4+S..4+2S) best particle. */
-template <int S>
-__global__ __launch_bounds__(128) void k_argmax(const float2* __restrict__ winner, uint32_t G,
-                                                int log_w, const float* __restrict__ part,
-                                                const float* __restrict__ sorted, uint32_t N,
-                                                uint32_t m, float2* __restrict__ result,
-                                                double* __restrict__ est) {
-    __shared__ float2 sc[128];
-    __shared__ double sd[128][2 + S];
-    __shared__ float smax[128];
-    const int l = threadIdx.x;
-    float2 acc = make_float2(log_w ? -INFINITY : -1.0f, -1.0f);
-    float bmax = -INFINITY;
-    for (uint32_t gi = (uint32_t)l; gi < G; gi += 128) {
-        float2 e = winner[gi];
-        acc = (acc.x > e.x) ? acc : e;
-        float pm = part[(size_t)gi * (2 + S)];
-        bmax = pm > bmax ? pm : bmax;
-    }
-    sc[l] = acc;
-    smax[l] = bmax;
-    __syncthreads();
-    for (int off = 64; off > 0; off >>= 1) {
-        if (l < off) {
-            float2 other = sc[l + off];
-            float2 mine = sc[l];
-            sc[l] = (mine.x > other.x) ? mine : other;
-            float a = smax[l], b = smax[l + off];
-            smax[l] = a > b ? a : b;
-        }
-        __syncthreads();
-    }
-    const float gmax = log_w ? smax[0] : 0.0f;
-    double a[2 + S];
-#pragma unroll
-    for (int c = 0; c < 2 + S; ++c) a[c] = 0.0;
-    for (uint32_t gi = (uint32_t)l; gi < G; gi += 128) {
-        const float* pg = part + (size_t)gi * (2 + S);
-        double scale = log_w ? (double)esth_expf(pg[0] - gmax) : 1.0;
-#pragma unroll
-        for (int c = 1; c < 2 + S; ++c) a[c] += (double)pg[c] * scale;
-    }
-#pragma unroll
-    for (int c = 0; c < 2 + S; ++c) sd[l][c] = a[c];
-    __syncthreads();
-    for (int off = 64; off > 0; off >>= 1) {
-        if (l < off) {
-#pragma unroll
-            for (int c = 1; c < 2 + S; ++c) sd[l][c] += sd[l + off][c];
-        }
-        __syncthreads();
-    }
-    if (l == 0) {
-        result[0] = sc[0];
-        est[0] = (double)gmax;
-#pragma unroll
-        for (int c = 1; c < 2 + S; ++c) est[c] = sd[0][c];
-        est[2 + S] = (double)sc[0].x;
-        est[3 + S] = (double)sc[0].y;
-    }
-    if (sorted && l < S) {
-        int gb = (int)sc[0].y;
-        if (gb >= 0 && (uint32_t)gb < G) est[4 + S + l] = (double)sorted[(size_t)l * N + (size_t)gb * m];
-    }
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float* __restrict__ st_in,
+                                              float* __restrict__ st_out, uint32_t N, uint32_t G,
+                                              float2* __restrict__ winner, int* __restrict__ perm,
+                                              uint32_t t, float* __restrict__ bt_w,
+                                              float* __restrict__ bt_s, float* __restrict__ part,
+                                              int log_w) {
+    constexpr int LPS = esth_tile<LOGM>::LPS;
+    const int lane = threadIdx.x & (ESTH_WAVE - 1);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE;
+    uint32_t g = wave * (ESTH_WAVE / LPS) + (uint32_t)(lane / LPS);
+    const bool valid = g < G;   /* a whole sub-filter is valid or not */
+    if (!valid) g = G - 1;
+    esth_sort_sub<LOGM, S>(w, st_in, st_out, N, G, g, valid, winner, perm, t, bt_w, bt_s, part, log_w);
 }
 
-/* ============================================================= exchange
+/* ============================================================= exchange */
+/* compact best-t index feeding received slot (q, k) of sub-filter b */
+__device__ __forceinline__ size_t esth_recv_src(uint32_t b, uint32_t q, uint32_t k, uint32_t G, uint32_t t, int topo,
+                                                int dimx, int dimy, const uint32_t* __restrict__ top) {
+    if (topo == ESTH_TOPO_RING) return (size_t)((b + G - 1) % G) * t + k;
+    if (topo == ESTH_TOPO_ALL2ALL) return top[k];
+    int x = (int)(b % (uint32_t)dimx), y = (int)(b / (uint32_t)dimx);
+    int sx = x, sy = y;
+    if (q == 0) sx = (x + 1) % dimx;
+    else if (q == 1) sx = (x - 1 + dimx) % dimx;
+    else if (q == 2) sy = (y + 1) % dimy;
+    else sy = (y - 1 + dimy) % dimy;
+    return (size_t)(sy * dimx + sx) * t + k;
+}
+
+/*
  * Receiver-side gather from the compact best-t buffer, so every read sees
  * the pre-exchange sorted state whatever the launch order:
  *  ring  : slot m-1-k <- block (b-1) mod G rank k (pf_generic.cl:368-383)
@@ -375,7 +515,7 @@ __global__ __launch_bounds__(128) void k_argmax(const float2* __restrict__ winne
  *          upper (3) neighbour: the receiving side of exchange_2dtorus
  *          (cuda/pf_generic.cu:352-393) with in-block offsets
  *  all2all: slot m-1-k <- global top-t candidate k */
-__global__ void k_exchange(float* __restrict__ st, float* __restrict__ w, int S, uint32_t N,
+static __global__ void k_exchange(float* __restrict__ st, float* __restrict__ w, int S, uint32_t N,
                            uint32_t m, uint32_t G, uint32_t t, int topo, int dimx, int dimy,
                            const float* __restrict__ bt_w, const float* __restrict__ bt_s,
                            const uint32_t* __restrict__ top) {
@@ -386,20 +526,7 @@ __global__ void k_exchange(float* __restrict__ st, float* __restrict__ w, int S,
     const uint32_t k = (uint32_t)(i % t);
     const uint32_t q = (uint32_t)((i / t) % nq);
     const uint32_t b = (uint32_t)(i / ((uint64_t)t * nq));
-    size_t src;
-    if (topo == ESTH_TOPO_RING) {
-        src = (size_t)((b + G - 1) % G) * t + k;
-    } else if (topo == ESTH_TOPO_ALL2ALL) {
-        src = top[k];
-    } else {
-        int x = (int)(b % (uint32_t)dimx), y = (int)(b / (uint32_t)dimx);
-        int sx = x, sy = y;
-        if (q == 0) sx = (x + 1) % dimx;
-        else if (q == 1) sx = (x - 1 + dimx) % dimx;
-        else if (q == 2) sy = (y + 1) % dimy;
-        else sy = (y - 1 + dimy) % dimy;
-        src = (size_t)(sy * dimx + sx) * t + k;
-    }
+    const size_t src = esth_recv_src(b, q, k, G, t, topo, dimx, dimy, top);
     const size_t dst = (size_t)b * m + (m - 1 - k - q * t);
     w[dst] = bt_w[src];
     const size_t Gt = (size_t)G * t;
@@ -408,7 +535,7 @@ __global__ void k_exchange(float* __restrict__ st, float* __restrict__ w, int S,
 
 /* Global top-t of the G*t candidates by (weight desc, index asc): t rounds,
  * each selecting the best candidate strictly after the previous one. */
-__global__ __launch_bounds__(1024) void k_topt(const float* __restrict__ bt_w, uint32_t n, uint32_t t,
+static __global__ __launch_bounds__(1024) void k_topt(const float* __restrict__ bt_w, uint32_t n, uint32_t t,
                                                uint32_t* __restrict__ top) {
     __shared__ float sw[1024];
     __shared__ uint32_t si[1024];
@@ -446,6 +573,17 @@ __global__ __launch_bounds__(1024) void k_topt(const float* __restrict__ bt_w, u
     }
 }
 
+/* cross-lane levels of the scan tree: lx[k] = sibling node at level k */
+template <int K, int LOGL>
+__device__ __forceinline__ void esth_scan_levels(float& node, float* lx) {
+    if constexpr (K < LOGL) {
+        float sib = esth_xor_lane_f<(1 << K)>(node);
+        lx[K] = sib;
+        node = node + sib;
+        esth_scan_levels<K + 1, LOGL>(node, lx);
+    }
+}
+
 /* ============================================ resampling (prefix sum)
  * resampling (pf_generic.cl:385-432): gate, exclusive scan of the sub-filter's
  * weights, selection, binary search, gather.  The scan reproduces the
@@ -455,22 +593,18 @@ __global__ __launch_bounds__(1024) void k_topt(const float* __restrict__ bt_w, u
  * set, L_k being the left sibling's node.  Levels inside a lane come from a
  * register tree, levels across lanes from xor butterflies (a + b == b + a in
  * IEEE arithmetic, so both lanes of a pair hold the same node bits). */
+/* one sub-filter per wave (or 64/m per wave); every wave of the work-group
+ * must call it (one barrier inside); cdf_w: this wave's 64*E floats of LDS */
 template <int LOGM, int S>
-__global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
+__device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                                                   const float* __restrict__ st_tmp,
                                                   const float* __restrict__ w,
-                                                  const float* __restrict__ rnd, uint32_t N, uint32_t G,
-                                                  float thr, int systematic, int log_w,
-                                                  int* __restrict__ sel) {
+                                                  const float* __restrict__ rnd, uint32_t N, uint32_t g,
+                                                  bool valid, float thr, int systematic, int log_w,
+                                                  int* __restrict__ sel, float* cdf_w) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS, LOGE = TL::LOGE, LOGL = TL::LOGL;
-    __shared__ float cdf_s[4][ESTH_WAVE * E];
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
-    const int wv = threadIdx.x / ESTH_WAVE;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE;
-    uint32_t g = wave * (ESTH_WAVE / LPS) + (uint32_t)(lane / LPS);
-    const bool valid = g < G;
-    if (!valid) g = G - 1;   /* keep the lane busy; its writes are dropped */
     const int laneg = lane % LPS;
     const int le = laneg * E;
     const size_t base = (size_t)g * M;
@@ -482,11 +616,7 @@ __global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
         float mx = x[0];
 #pragma unroll
         for (int r = 1; r < E; ++r) mx = x[r] > mx ? x[r] : mx;
-#pragma unroll
-        for (int k = 1; k < LPS; k <<= 1) {
-            float o = esth_shfl_xor(mx, k);
-            mx = o > mx ? o : mx;
-        }
+        mx = esth_bfly_max<1, LPS>(mx);
 #pragma unroll
         for (int r = 0; r < E; ++r) x[r] = esth_expf(x[r] - mx);
     }
@@ -500,18 +630,13 @@ __global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
         for (int j = 0; j < (E >> k); ++j) lvl[k][j] = lvl[k - 1][2 * j] + lvl[k - 1][2 * j + 1];
     float node = lvl[LOGE][0];
     float lx[LOGL > 0 ? LOGL : 1];
-#pragma unroll
-    for (int k = 0; k < LOGL; ++k) {
-        float sib = esth_shfl_xor(node, 1 << k);
-        lx[k] = sib;
-        node = node + sib;
-    }
+    esth_scan_levels<0, LOGL>(node, lx);
     const float total = node;
     float acc = 0.0f;
 #pragma unroll
     for (int k = LOGL - 1; k >= 0; --k)
         if ((laneg >> k) & 1) acc = acc + lx[k];
-    float* cdf = &cdf_s[wv][(lane - laneg) * E];
+    float* cdf = cdf_w + (lane - laneg) * E;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         float a = acc;
@@ -535,7 +660,7 @@ __global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
             if (cdf[cur] > s) mxi = cur - 1;
             else mn = cur;
         } while (mn < mxi);
-        pick[r] = mn;
+        pick[r] = mn & (M - 1);
     }
     if (go) {
 #pragma unroll
@@ -550,6 +675,24 @@ __global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
 #pragma unroll
         for (int r = 0; r < E; ++r) sel[base + le + r] = go ? pick[r] : -1;
     }
+}
+
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
+                                                  const float* __restrict__ st_tmp,
+                                                  const float* __restrict__ w,
+                                                  const float* __restrict__ rnd, uint32_t N, uint32_t G,
+                                                  float thr, int systematic, int log_w,
+                                                  int* __restrict__ sel) {
+    using TL = esth_tile<LOGM>;
+    __shared__ float cdf_s[4][ESTH_WAVE * TL::E];
+    const int lane = threadIdx.x & (ESTH_WAVE - 1);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE;
+    uint32_t g = wave * (ESTH_WAVE / TL::LPS) + (uint32_t)(lane / TL::LPS);
+    const bool valid = g < G;
+    if (!valid) g = G - 1;   /* keep the lane busy; its writes are dropped */
+    esth_resample_sub<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, sel,
+                               cdf_s[threadIdx.x / ESTH_WAVE]);
 }
 
 /* =========================================================== Vose alias
@@ -593,14 +736,13 @@ __device__ __forceinline__ uint32_t esth_block_rank(bool pred, uint32_t* cnt, in
     return before + r;
 }
 
+/* one work-group of M work-items on sub-filter g */
 template <int S>
-__global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
-                                                const float* __restrict__ st_tmp,
-                                                const float* __restrict__ w,
-                                                const float* __restrict__ rnd, uint32_t N, uint32_t M,
-                                                float thr, int adaptive, int log_w,
-                                                int* __restrict__ sel, float* __restrict__ prob_out,
-                                                uint32_t* __restrict__ alias_out) {
+__device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, const float* __restrict__ st_tmp,
+                                               const float* __restrict__ w, const float* __restrict__ rnd,
+                                               uint32_t N, uint32_t M, uint32_t g, float thr, int adaptive,
+                                               int log_w, int* __restrict__ sel, float* __restrict__ prob_out,
+                                               uint32_t* __restrict__ alias_out) {
     __shared__ float prob[ESTH_MAX_NUM_PARTICLES];
     __shared__ uint32_t alias[ESTH_MAX_NUM_PARTICLES];
     __shared__ uint32_t sl[ESTH_MAX_NUM_PARTICLES];
@@ -609,7 +751,6 @@ __global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
     __shared__ float red[16];
     const int l = threadIdx.x;
     const int nw = (int)((M + 63) / 64);
-    const uint32_t g = blockIdx.x;
     const size_t idx = (size_t)g * M + (size_t)l;
     const bool table_only = prob_out != nullptr;
     if (!table_only && !adaptive) {
@@ -704,14 +845,152 @@ __global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
     if (sel) sel[idx] = (int)pick;
 }
 
+template <int S>
+__global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
+                                                const float* __restrict__ st_tmp,
+                                                const float* __restrict__ w,
+                                                const float* __restrict__ rnd, uint32_t N, uint32_t M,
+                                                float thr, int adaptive, int log_w,
+                                                int* __restrict__ sel, float* __restrict__ prob_out,
+                                                uint32_t* __restrict__ alias_out) {
+    esth_alias_sub<S>(st_main, st_tmp, w, rnd, N, M, blockIdx.x, thr, adaptive, log_w, sel, prob_out, alias_out);
+}
+
+
+/* ============================================================== batches
+ * Kernels over many filters: a device table of esth_bfilter and tables of
+ * (filter, sub-filter) pairs, one per sub-filter size class.  Each wave (or
+ * work-group for the alias build) runs exactly the per-sub-filter code of the
+ * single-filter kernels above on its filter's buffers, so a filter's particles
+ * are bit-identical alone and in a batch. */
+__device__ __forceinline__ uint32_t esth_wave_id() {
+    return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE);
+}
+
+template <int MODEL>
+__global__ __launch_bounds__(256) void kb_sample_weight(const esth_bfilter* __restrict__ F,
+                                                        const uint2* __restrict__ sub, uint32_t nsub,
+                                                        const float* __restrict__ R,
+                                                        const float* __restrict__ frames, int log_w) {
+    const uint32_t wv = esth_wave_id();
+    if (wv >= nsub) return;
+    const uint2 e = sub[wv];
+    const esth_bfilter& f = F[e.x];
+    const uint32_t lane = threadIdx.x & (ESTH_WAVE - 1);
+    const uint32_t base = e.y * f.m;
+    for (uint32_t q = lane; q < f.m / 2; q += ESTH_WAVE)
+        esth_sample_weight_pair<MODEL>(f.state, f.N, R + f.r_off, frames + f.f_off, log_w, f.w, base + 2 * q);
+}
+
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void kb_sort(const esth_bfilter* __restrict__ F, const uint2* __restrict__ sub,
+                                               uint32_t nsub, int log_w) {
+    const uint32_t wv = esth_wave_id();
+    if (wv >= nsub) return;
+    const uint2 e = sub[wv];
+    const esth_bfilter& f = F[e.x];
+    esth_sort_sub<LOGM, S>(f.w, f.state, f.tmp, f.N, f.G, e.y, true, (float2*)f.winner, nullptr, f.t, f.bt_w,
+                           f.bt_s, f.part, log_w);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_argmax(const esth_bfilter* __restrict__ F, int log_w) {
+    __shared__ float2 sc[128];
+    __shared__ float smax[4];
+    __shared__ double sd[4 * (2 + S)];
+    const esth_bfilter& f = F[blockIdx.x];
+    esth_argmax_block<S, 256>((const float2*)f.winner, f.G, log_w, f.part, f.tmp, f.N, f.m, (float2*)f.result,
+                              f.est, sc, smax, sd);
+}
+
+/* all-to-all: the global top t of each filter's compact best-t buffer */
+static __global__ __launch_bounds__(1024) void kb_topt(const esth_bfilter* __restrict__ F) {
+    __shared__ float sw[1024];
+    __shared__ uint32_t si[1024];
+    const esth_bfilter& f = F[blockIdx.x];
+    const int l = threadIdx.x;
+    const uint32_t n = f.G * f.t;
+    float pw = INFINITY;
+    uint32_t pi = 0;
+    bool have_prev = false;
+    for (uint32_t k = 0; k < f.t; ++k) {
+        float bw = -INFINITY;
+        uint32_t bi = 0xffffffffu;
+        for (uint32_t c = (uint32_t)l; c < n; c += 1024) {
+            float v = f.bt_w[c];
+            bool after = !have_prev || (v < pw) || (v == pw && c > pi);
+            bool better = (bi == 0xffffffffu) || (v > bw) || (v == bw && c < bi);
+            if (after && better) { bw = v; bi = c; }
+        }
+        sw[l] = bw;
+        si[l] = bi;
+        __syncthreads();
+        for (int off = 512; off > 0; off >>= 1) {
+            if (l < off) {
+                float ow = sw[l + off];
+                uint32_t oi = si[l + off];
+                bool take = (oi != 0xffffffffu) &&
+                            ((si[l] == 0xffffffffu) || ow > sw[l] || (ow == sw[l] && oi < si[l]));
+                if (take) { sw[l] = ow; si[l] = oi; }
+            }
+            __syncthreads();
+        }
+        pw = sw[0];
+        pi = si[0];
+        have_prev = true;
+        if (l == 0) f.top[k] = pi;
+        __syncthreads();
+    }
+}
+
+/* one thread per received slot: nslots = (torus ? 4 : 1) * t slots per sub-filter */
+static __global__ void kb_exchange(const esth_bfilter* __restrict__ F, const uint2* __restrict__ sub,
+                                   uint32_t nsub, int S, uint32_t nslots, int topo) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (uint64_t)nsub * nslots) return;
+    const uint2 e = sub[j / nslots];
+    const uint32_t slot = (uint32_t)(j % nslots);
+    const esth_bfilter& f = F[e.x];
+    const uint32_t t = f.t;
+    const uint32_t q = slot / t, k = slot % t;
+    const size_t src = esth_recv_src(e.y, q, k, f.G, t, topo, f.dimx, f.dimy, f.top);
+    const size_t dst = (size_t)e.y * f.m + (f.m - 1 - k - q * t);
+    f.w[dst] = f.bt_w[src];
+    const size_t Gt = (size_t)f.G * t;
+    for (int c = 0; c < S; ++c) f.tmp[(size_t)c * f.N + dst] = f.bt_s[(size_t)c * Gt + src];
+}
+
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void kb_resample(const esth_bfilter* __restrict__ F,
+                                                   const uint2* __restrict__ sub, uint32_t nsub,
+                                                   const float* __restrict__ R, float thr, int systematic,
+                                                   int log_w) {
+    __shared__ float cdf_s[4][ESTH_WAVE * esth_tile<LOGM>::E];
+    const uint32_t wv = esth_wave_id();
+    const bool valid = wv < nsub;
+    const uint2 e = sub[valid ? wv : nsub - 1];
+    const esth_bfilter& f = F[e.x];
+    esth_resample_sub<LOGM, S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, valid, thr, systematic, log_w,
+                               nullptr, cdf_s[threadIdx.x / ESTH_WAVE]);
+}
+
+template <int S>
+__global__ __launch_bounds__(1024) void kb_alias(const esth_bfilter* __restrict__ F, const uint2* __restrict__ sub,
+                                                 const float* __restrict__ R, float thr, int adaptive, int log_w) {
+    const uint2 e = sub[blockIdx.x];
+    const esth_bfilter& f = F[e.x];
+    esth_alias_sub<S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, f.m, e.y, thr, adaptive, log_w, nullptr,
+                      nullptr, nullptr);
+}
+
 /* ================================================================ copy */
-__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+static __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
 }
 
 /* ================================================================ math */
-__global__ void k_math(int fn, const float* __restrict__ x, const float* __restrict__ x2,
+static __global__ void k_math(int fn, const float* __restrict__ x, const float* __restrict__ x2,
                        float* __restrict__ out, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

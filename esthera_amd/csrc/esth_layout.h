@@ -50,6 +50,27 @@ static inline uint64_t esth_rng_stream_size(int K, uint64_t needed) {
     return total / (uint64_t)K;
 }
 
+/* one filter of a batch, as the batched kernels see it (device pointers;
+ * R and frame are offsets into the per-parity buffers of the launch) */
+struct esth_bfilter {
+    float* state;
+    float* tmp;
+    float* w;
+    float* part;
+    float* bt_w;
+    float* bt_s;
+    void* winner;      /* float2[G] */
+    void* result;      /* float2 */
+    double* est;
+    uint32_t* top;
+    uint64_t r_off;    /* words into R[parity] */
+    uint64_t f_off;    /* floats into frames[parity] */
+    uint32_t N, G, m, t;
+    int32_t dimx, dimy;
+    uint32_t uoff;     /* N * V: first uniform of the step's words */
+    uint32_t pad;
+};
+
 /* one MTGP32 stream of a launch */
 struct esth_mtgp_stream {
     uint32_t set;       /* parameter set */

@@ -185,8 +185,9 @@ ESTH_HD float esth_atan2f(float y, float x) {
     float ax = fabsf(x), ay = fabsf(y);
     float mx = ax > ay ? ax : ay;
     float mn = ax > ay ? ay : ax;
-    if (mx == 0.0f) return 0.0f;
-    float a = esth_atanf_unit(mn / mx);
+    float t = mn / mx;
+    t = (mx == 0.0f) ? 0.0f : t;   /* atan2(0, 0) = 0 without a branch */
+    float a = esth_atanf_unit(t);
     if (ay > ax) a = ESTH_PI_2 - a;
     if (x < 0.0f) a = ESTH_PI - a;
     return (y < 0.0f) ? -a : a;

@@ -97,7 +97,8 @@ ESTH_HD float esth_robotarm5_sampling_iw(float* s, const float* z, const float* 
  * [6] 0.5/sr^2, [7] 0.5/sb^2, [8] L, then at 16+5j: lx, ly, zr, cos zb, sin zb */
 #define ESTH_UNI_HDR 16
 
-ESTH_HD float esth_unicycle_sampling_iw(float* s, const float* z, const float* fr) {
+/* propagation; returns cos and sin of the new heading */
+ESTH_HD void esth_unicycle_propagate(float* s, const float* z, const float* fr, float* ctp, float* stp) {
     float ct, st;
     esth_sincosf(s[2], &st, &ct);
     float x = s[0] + fr[1] * ct;
@@ -107,24 +108,37 @@ ESTH_HD float esth_unicycle_sampling_iw(float* s, const float* z, const float* f
     float th = s[2] + fr[2];
     th = th + fr[5] * z[2];
     s[0] = x; s[1] = y; s[2] = th;
-    esth_sincosf(th, &st, &ct);
+    esth_sincosf(th, stp, ctp);
+}
+
+/* one landmark's squared range and bearing residuals, accumulated */
+ESTH_HD void esth_unicycle_landmark(const float* lm, float x, float y, float ct, float st, float* acc_r,
+                                    float* acc_b) {
+    float dx = lm[0] - x;
+    float dy = lm[1] - y;
+    float r = sqrtf(esth_fma(dx, dx, dy * dy));
+    float er = lm[2] - r;
+    *acc_r = esth_fma(er, er, *acc_r);
+    float C = esth_fma(ct, lm[3], -(st * lm[4]));   /* cos(theta + zb) */
+    float S = esth_fma(st, lm[3], ct * lm[4]);      /* sin(theta + zb) */
+    float u = esth_fma(dx, C, dy * S);
+    float v = esth_fma(dy, C, -(dx * S));
+    float eb = esth_atan2f(v, u);
+    *acc_b = esth_fma(eb, eb, *acc_b);
+}
+
+ESTH_HD float esth_unicycle_finish(float acc_r, float acc_b, const float* fr) {
+    return -esth_fma(acc_r, fr[6], acc_b * fr[7]);
+}
+
+ESTH_HD float esth_unicycle_sampling_iw(float* s, const float* z, const float* fr) {
+    float ct, st;
+    esth_unicycle_propagate(s, z, fr, &ct, &st);
     const int L = (int)fr[8];
     float acc_r = 0.0f, acc_b = 0.0f;
-    for (int j = 0; j < L; ++j) {
-        const float* lm = fr + ESTH_UNI_HDR + 5 * j;
-        float dx = lm[0] - x;
-        float dy = lm[1] - y;
-        float r = sqrtf(esth_fma(dx, dx, dy * dy));
-        float er = lm[2] - r;
-        acc_r = esth_fma(er, er, acc_r);
-        float C = esth_fma(ct, lm[3], -(st * lm[4]));   /* cos(theta + zb) */
-        float S = esth_fma(st, lm[3], ct * lm[4]);      /* sin(theta + zb) */
-        float u = esth_fma(dx, C, dy * S);
-        float v = esth_fma(dy, C, -(dx * S));
-        float eb = esth_atan2f(v, u);
-        acc_b = esth_fma(eb, eb, acc_b);
-    }
-    return -esth_fma(acc_r, fr[6], acc_b * fr[7]);
+    for (int j = 0; j < L; ++j)
+        esth_unicycle_landmark(fr + ESTH_UNI_HDR + 5 * j, s[0], s[1], ct, st, &acc_r, &acc_b);
+    return esth_unicycle_finish(acc_r, acc_b, fr);
 }
 
 /* ------------------------------------------------------------------ dh6

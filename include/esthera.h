@@ -192,10 +192,12 @@ int esth_filter_estimate(esth_filter* f, esth_estimate* out);
 /* Current particles (state_dim planes of m*G floats) and per-particle
  * weights as left by the sort/exchange stages (sorted per sub-filter). */
 int esth_filter_read_particles(esth_filter* f, float* state_soa, float* weights);
-/* Stage timing with hipEvents recorded between the kernels on the filter's
- * stream.  set_timing(f, 1) starts a new measurement (eager steps only, up to
- * 4096 steps); stage_times returns the mean device time per step in
- * microseconds of [rng, sample_weight, sort, argmax, exchange, resample]. */
+/* Stage timing with hipEvents recorded on the filter's stream.
+ * set_timing(f, level) starts a new measurement (eager steps only, up to 4096
+ * steps): level 1 brackets only the sample+weight kernel (two events per
+ * step), level 2 every stage.  stage_times returns the mean device time per
+ * step in microseconds of [rng wait, sample_weight, sort, argmax,
+ * exchange/top-t, resample]; stages not timed are NaN. */
 int esth_filter_set_timing(esth_filter* f, int on);
 int esth_filter_stage_times(esth_filter* f, float us[6]);
 /* Checkpoint / resume: particles, weights, RNG status and step counter. */
@@ -204,6 +206,9 @@ int esth_filter_snapshot(esth_filter* f, void* buf, size_t bytes);
 int esth_filter_restore(esth_filter* f, const void* buf, size_t bytes);
 /* The HIP stream (hipStream_t) the filter's work is enqueued on. */
 int esth_filter_stream(esth_filter* f, void** hip_stream);
+/* Generate the next step's MTGP32 words on a second stream while the current
+ * step runs (default on; results are identical either way). */
+int esth_filter_set_prefetch(esth_filter* f, int enable);
 /* Capture one step into a HIP graph and replay it for later steps with the
  * same control/measurement layout (the frame buffer is updated per call). */
 int esth_filter_use_graph(esth_filter* f, int enable);
@@ -253,9 +258,26 @@ int esth_batch_step(esth_batch* b, const void* controls, size_t control_stride,
 int esth_batch_sync(esth_batch* b);
 int esth_batch_estimate(esth_batch* b, int filter, esth_estimate* out);
 int esth_batch_read_particles(esth_batch* b, int filter, float* state_soa, float* weights);
+/* Replay the step's main-stream work as a captured HIP graph (default on). */
 int esth_batch_use_graph(esth_batch* b, int enable);
 /* Device time of the last step in microseconds. */
 int esth_batch_step_time(esth_batch* b, float* us);
+/* Device times of every step since the previous call (at most `max`, up to
+ * 8192 kept), microseconds; resets the record. */
+int esth_batch_step_times(esth_batch* b, float* us, int max, int* count);
+/* Totals of the batch: particles, MTGP32 streams, sub-filters. */
+int esth_batch_info(esth_batch* b, uint64_t* particles, int* streams, uint32_t* sub_filters);
+
+/* ------------------------------------------------------------ autotuner
+ * Measures, on this device, the cost of a step's kernels per million
+ * particles for sub-filter sizes 64, 128, 256, 512, 1024 (a 2^20-particle
+ * filter with `base`'s model and settings and the given control/measurement).
+ * esth_autotune_pick then chooses a filter's sub-filter size: among the powers
+ * of two in [max(64, min_m), 1024] dividing `particles`, the cheapest (ties:
+ * larger); if none is >= min_m, the largest divisor. */
+int esth_autotune(esth_engine* e, const esth_filter_desc* base, const void* params, size_t params_size,
+                  const void* control, const void* measurement, float dt, float us_per_mparticle[5]);
+int esth_autotune_pick(const float us_per_mparticle[5], uint32_t particles, uint32_t min_m, uint32_t* m_out);
 
 /* ------------------------------------------------------------ serving
  * A request loop with continuous batching: tracks (filters) are opened and

@@ -1,0 +1,97 @@
+"""The batched form: many filters of different sizes, landmark sets and seeds
+stepped by one launch sequence per step.  Every filter's particles must be
+bit-identical to the same filter stepped alone (ESTH_RNG_MTGP_PER_FILTER), in
+eager and graph-replay mode, for each resampler and topology."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import esthera_amd as ea
+
+pytestmark = pytest.mark.gpu
+
+
+def make_filters(nf, seed, sizes, topology=ea.TOPO_RING, t=1, resampler=ea.RESAMPLE_SYSTEMATIC):
+    rng = np.random.default_rng(seed)
+    descs, params, scen = [], [], []
+    for i in range(nf):
+        m, G = sizes[i % len(sizes)]
+        d = ea.filter_desc(ea.MODEL_UNICYCLE, m=m, G=G, t=t, topology=topology, resampler=resampler,
+                           weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=int(rng.integers(1, 1 << 30)))
+        p, ctrl, meas, truth = ea.unicycle_scenario(seed * 1000 + i, 8, 20.0, 8)
+        descs.append(d)
+        params.append(p)
+        scen.append((ctrl, meas))
+    return descs, params, scen
+
+
+def step_inputs(scen, k):
+    nf = len(scen)
+    ctrls = (ea.UnicycleControl * nf)(*[s[0][k] for s in scen])
+    meas = (ea.UnicycleMeasurement * nf)(*[s[1][k] for s in scen])
+    return ctrls, meas, np.full(nf, 0.1, np.float32)
+
+
+CASES = [
+    ("ring-systematic", dict(sizes=[(64, 3), (128, 5), (256, 2), (512, 4), (1024, 1), (64, 1)])),
+    ("torus-multinomial", dict(sizes=[(256, 16), (512, 4), (64, 64)], topology=ea.TOPO_TORUS, t=2,
+                               resampler=ea.RESAMPLE_MULTINOMIAL)),
+    ("ring-alias", dict(sizes=[(64, 4), (256, 3), (1024, 2)], resampler=ea.RESAMPLE_ALIAS)),
+    ("all2all", dict(sizes=[(128, 8), (256, 5)], topology=ea.TOPO_ALL2ALL, t=2)),
+]
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_equals_filters_alone(name, kw, graph, engine):
+    nf = 9
+    descs, params, scen = make_filters(nf, 3, **kw)
+    b = ea.Batch(engine, descs, params)
+    b.use_graph(graph)
+    b.init_uniform(np.zeros((nf, 3), np.float32))
+    alone = []
+    for i in range(nf):
+        f = ea.Filter(engine, descs[i], params[i])
+        f.init_uniform(np.zeros(3, np.float32))
+        alone.append(f)
+    steps = 5
+    for k in range(steps):
+        ctrls, meas, dts = step_inputs(scen, k)
+        b.step(ctrls, meas, dts)
+        for i, f in enumerate(alone):
+            f.step(scen[i][0][k], scen[i][1][k], 0.1)
+    for i, f in enumerate(alone):
+        sb, wb = b.particles(i)
+        sa, wa = f.particles()
+        assert np.array_equal(sb.view(np.uint32), sa.view(np.uint32)), f"filter {i} state differs"
+        assert np.array_equal(wb.view(np.uint32), wa.view(np.uint32)), f"filter {i} weights differ"
+        eb, ea_ = b.estimate(i), f.estimate()
+        np.testing.assert_allclose(np.array(eb.mean[:3]), np.array(ea_.mean[:3]), rtol=1e-9, atol=1e-9)
+        assert eb.best_weight == ea_.best_weight and eb.best_block == ea_.best_block
+        f.close()
+    t = b.step_times()
+    assert len(t) == steps and np.all(t > 0)
+    b.close()
+
+
+def test_batch_info_and_memory(engine):
+    descs, params, scen = make_filters(4, 5, [(256, 4), (512, 2)])
+    used0, _ = engine.memory()
+    b = ea.Batch(engine, descs, params)
+    particles, streams, subs = b.info()
+    assert particles == sum(d.num_particles * d.num_blocks for d in descs)
+    assert subs == sum(d.num_blocks for d in descs)
+    assert engine.memory()[0] > used0
+    b.close()
+    assert engine.memory()[0] == used0
+
+
+def test_autotune_measures_all_sizes(engine):
+    p, ctrl, meas, truth = ea.unicycle_scenario(99, 8, 20.0, 2)
+    d = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=4, t=1, topology=ea.TOPO_RING, resampler=ea.RESAMPLE_SYSTEMATIC,
+                       weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=1)
+    us = ea.autotune(engine, d, p, ctrl[0], meas[0], 0.1)
+    assert us.shape == (5,) and np.all(us > 0) and np.all(np.isfinite(us))
+    m = ea.autotune_pick(us, 65536, 256)
+    assert m in (256, 512, 1024)

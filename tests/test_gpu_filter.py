@@ -156,3 +156,25 @@ def test_c1_scale_properties(engine):
     assert np.all(head[:, :-1] >= head[:, 1:])   # sorted per sub-filter up to the received slots
     est = f.estimate()
     assert np.hypot(est.mean[0] - truth[2, 0], est.mean[1] - truth[2, 1]) < 0.5
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_prefetch_on_off_identical(graph, engine):
+    """Generating step n+1's MTGP words on a second stream during step n
+    changes nothing in the results (eager and graph replay)."""
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=64, t=2, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG,
+                          resampler=ea.RESAMPLE_SYSTEMATIC, seed=5)
+    params, ctrl, meas, init, truth, dt = make_case(ea.MODEL_UNICYCLE, 9)
+    a, b = ea.Filter(engine, desc, params), ea.Filter(engine, desc, params)
+    a.set_prefetch(False)
+    for f in (a, b):
+        f.init_uniform(init)
+    if graph:
+        b.use_graph(True)
+    for k in range(9):
+        a.step(ctrl[k], meas[k], dt)
+        b.step(ctrl[k], meas[k], dt)
+    sa, wa = a.particles()
+    sb, wb = b.particles()
+    assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+    assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
