@@ -83,6 +83,28 @@ class Result(C.Structure):
                 ("submit_us", C.c_double), ("done_us", C.c_double), ("estimate", Estimate)]
 
 
+class ServeWorkload(C.Structure):
+    _fields_ = [("rate", C.c_double), ("duration_s", C.c_double), ("open_fraction", C.c_double),
+                ("unicycle_fraction", C.c_double), ("min_particles", C.c_uint32), ("max_particles", C.c_uint32),
+                ("mean_lifetime", C.c_double), ("initial_tracks", C.c_int), ("landmarks", C.c_int),
+                ("topology", C.c_int), ("num_transfer", C.c_int), ("resampler", C.c_int), ("weights", C.c_int),
+                ("seed", C.c_uint32), ("plain", C.c_int), ("hbm_bytes", C.c_size_t), ("max_tracks", C.c_int),
+                ("max_inflight", C.c_int), ("us_per_mparticle", C.c_void_p), ("ra_controls", C.c_void_p),
+                ("ra_measurements", C.c_void_p), ("ra_dt", C.c_void_p), ("ra_steps", C.c_int)]
+
+
+class ServeReport(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("opens", C.c_uint64), ("updates", C.c_uint64), ("closes", C.c_uint64),
+                ("failed", C.c_uint64), ("backlog", C.c_uint64), ("wall_s", C.c_double),
+                ("updates_per_s", C.c_double), ("particle_updates_per_s", C.c_double),
+                ("mean_particles", C.c_double), ("p50_us", C.c_double), ("p99_us", C.c_double),
+                ("mean_us", C.c_double), ("max_us", C.c_double), ("peak_tracks", C.c_int),
+                ("hbm_peak", C.c_size_t), ("waves", C.c_uint64), ("mean_wave_updates", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 PARAMS_TYPE = {MODEL_ROBOTARM5: None, MODEL_UNICYCLE: UnicycleParams, MODEL_DH6: DH6Params}
 CONTROL_TYPE = {MODEL_ROBOTARM5: RobotArm5Control, MODEL_UNICYCLE: UnicycleControl, MODEL_DH6: DH6Control}
 MEAS_TYPE = {MODEL_ROBOTARM5: RobotArm5Measurement, MODEL_UNICYCLE: UnicycleMeasurement,
@@ -143,7 +165,11 @@ _SIGS = {
     "esth_batch_info": ([_P, _P, _P, _P], C.c_int),
     "esth_autotune": ([_P, _P, _P, C.c_size_t, _P, _P, C.c_float, _P], C.c_int),
     "esth_autotune_pick": ([_P, C.c_uint32, C.c_uint32, _P], C.c_int),
-    "esth_server_create": ([_P, C.c_int, _P], C.c_int),
+    "esth_server_create": ([_P, C.c_int, C.c_size_t, _P], C.c_int),
+    "esth_server_configure": ([_P, C.c_int, C.c_int], C.c_int),
+    "esth_server_wave_stats": ([_P, _P, _P, _P], C.c_int),
+    "esth_server_read_particles": ([_P, C.c_uint64, _P, _P], C.c_int),
+    "esth_serve_simulate": ([_P, _P, _P], C.c_int),
     "esth_server_destroy": ([_P], C.c_int),
     "esth_server_open": ([_P, _P, _P, C.c_size_t, _P, _P, _P], C.c_int),
     "esth_server_update": ([_P, C.c_uint64, _P, _P, C.c_float, _P], C.c_int),
@@ -443,6 +469,102 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching request loop over resident tracks."""
+
+    def __init__(self, engine, max_tracks, hbm_bytes, max_inflight=2, max_wave_tracks=1 << 30):
+        self.engine = engine
+        h = C.c_void_p()
+        check(lib().esth_server_create(engine.h, max_tracks, hbm_bytes, C.byref(h)))
+        self.h = h
+        check(lib().esth_server_configure(self.h, max_inflight, max_wave_tracks))
+        self.S = {}
+
+    def open(self, desc, params, init_state):
+        """Queue a track; returns (track_id, request_id)."""
+        s = np.zeros(16, np.float32)
+        init = np.asarray(init_state, np.float32).ravel()
+        s[: init.size] = init
+        tid, rid = C.c_uint64(), C.c_uint64()
+        psize = C.sizeof(params) if params is not None else 0
+        check(lib().esth_server_open(self.h, ptr(desc), ptr(params), psize, ptr(s), C.byref(tid), C.byref(rid)))
+        self.S[tid.value] = (STATE_DIM[desc.model], desc.num_particles * desc.num_blocks)
+        return tid.value, rid.value
+
+    def update(self, track, control, measurement, dt):
+        rid = C.c_uint64()
+        check(lib().esth_server_update(self.h, track, ptr(control), ptr(measurement), dt, C.byref(rid)))
+        return rid.value
+
+    def close_track(self, track):
+        rid = C.c_uint64()
+        check(lib().esth_server_close(self.h, track, C.byref(rid)))
+        return rid.value
+
+    def pump(self):
+        check(lib().esth_server_pump(self.h))
+
+    def drain(self):
+        check(lib().esth_server_drain(self.h))
+
+    def poll(self, max_results=4096):
+        out = (Result * max_results)()
+        n = C.c_int()
+        check(lib().esth_server_poll(self.h, out, max_results, C.byref(n)))
+        return [out[i] for i in range(n.value)]
+
+    def stats(self):
+        used, open_tracks, queued = C.c_size_t(), C.c_int(), C.c_int()
+        check(lib().esth_server_stats(self.h, C.byref(used), C.byref(open_tracks), C.byref(queued)))
+        return used.value, open_tracks.value, queued.value
+
+    def wave_stats(self):
+        waves, upd, peak = C.c_uint64(), C.c_uint64(), C.c_size_t()
+        check(lib().esth_server_wave_stats(self.h, C.byref(waves), C.byref(upd), C.byref(peak)))
+        return waves.value, upd.value, peak.value
+
+    def particles(self, track):
+        S, N = self.S[track]
+        st = np.empty(S * N, np.float32)
+        w = np.empty(N, np.float32)
+        check(lib().esth_server_read_particles(self.h, track, ptr(st), ptr(w)))
+        return st, w
+
+    def close(self):
+        if self.h:
+            lib().esth_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_simulate(engine, workload, ra_trace=None, us_per_mparticle=None):
+    """Drive a seeded request stream (esth_serve_simulate); ra_trace =
+    (controls, measurements, dts) from load_robotarm_trace for robot-arm tracks."""
+    keep = []
+    if us_per_mparticle is not None:
+        us = np.ascontiguousarray(us_per_mparticle, dtype=np.float32)
+        keep.append(us)
+        workload.us_per_mparticle = us.ctypes.data
+    if ra_trace is not None:
+        meas, ctrl, dts = ra_trace
+        carr = (RobotArm5Control * len(ctrl))(*ctrl)
+        marr = (RobotArm5Measurement * len(meas))(*meas)
+        d = np.ascontiguousarray(dts, dtype=np.float32)
+        keep += [carr, marr, d]
+        workload.ra_controls = C.cast(carr, C.c_void_p)
+        workload.ra_measurements = C.cast(marr, C.c_void_p)
+        workload.ra_dt = d.ctypes.data
+        workload.ra_steps = len(ctrl)
+    rep_ = ServeReport()
+    check(lib().esth_serve_simulate(engine.h, C.byref(workload), C.byref(rep_)))
+    return rep_.as_dict()
 
 
 def autotune(engine, desc, params, control, measurement, dt):

@@ -235,7 +235,7 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
         }
         for (int j = 0; j < b->K[i]; ++j) {
             const size_t sj = (size_t)b->stream0[i] + (size_t)j;
-            hs[sj] = {(uint32_t)j, (uint32_t)b->rsize[i], f.r_off + (uint64_t)j * b->rsize[i]};
+            hs[sj] = {(uint32_t)j, (uint32_t)b->rsize[i], f.r_off + (uint64_t)j * b->rsize[i], (uint64_t)sj};
             esth_mtgp_seed_state(j, descs[i].seed + (uint32_t)j + 1u, st.data() + sj * ESTH_MTGP_N);
         }
         goff += f.G;

@@ -370,7 +370,7 @@ extern "C" int esth_stage_mtgp(uint32_t* d_status, int nstreams, int first_set, 
         return esth_fail(ESTH_ERR_INVALID, "streams %d from set %d", nstreams, first_set);
     if (size <= 0 || size % ESTH_MTGP_LARGE) return esth_fail(ESTH_ERR_INVALID, "size %d not a multiple of 768", size);
     std::vector<esth_mtgp_stream> hs(nstreams);
-    for (int j = 0; j < nstreams; ++j) hs[j] = {(uint32_t)(first_set + j), (uint32_t)size, (uint64_t)j * (uint64_t)size};
+    for (int j = 0; j < nstreams; ++j) hs[j] = {(uint32_t)(first_set + j), (uint32_t)size, (uint64_t)j * (uint64_t)size, (uint64_t)j};
     esth_mtgp_stream* ds = nullptr;
     uint32_t* dp = nullptr;
     ESTH_HIP(hipMalloc((void**)&ds, sizeof(esth_mtgp_stream) * nstreams));

@@ -152,7 +152,7 @@ extern "C" int esth_filter_create(esth_engine* e, const esth_filter_desc* d, con
     std::vector<esth_mtgp_stream> sd(f->K);
     for (int j = 0; j < f->K; ++j) {
         esth_mtgp_seed_state(j, d->seed + (uint32_t)j + 1u, st.data() + (size_t)j * ESTH_MTGP_N);
-        sd[j] = {(uint32_t)j, (uint32_t)f->size, (uint64_t)j * f->size};
+        sd[j] = {(uint32_t)j, (uint32_t)f->size, (uint64_t)j * f->size, (uint64_t)j};
     }
     err = hipMemcpy(f->status[0], st.data(), st.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(f->streams, sd.data(), sd.size() * sizeof(esth_mtgp_stream), hipMemcpyHostToDevice);

@@ -83,10 +83,11 @@ __device__ __forceinline__ float esth_bfly_max(float v) {
  * produces 768 words in three 256-wide phases, each one the recursion
  * x[n] = rec(x[n-351], x[n-350], x[n-351+pos]) for 256 consecutive n
  * (mtgp32-opencl.cl:186-242).  Output is the sequential stream of the
- * stream's parameter set, continuing across launches through `status`. */
+ * stream's parameter set, continuing across launches through `status`
+ * (status_in may equal status_out: each work-group reads its status into LDS
+ * before the first barrier and writes it back after the last). */
 static __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ params, uint32_t mask,
-                                              const uint32_t* __restrict__ status_in,
-                                              uint32_t* __restrict__ status_out,
+                                              const uint32_t* status_in, uint32_t* status_out,
                                               const esth_mtgp_stream* __restrict__ streams,
                                               uint32_t* __restrict__ out, int float_mode) {
     __shared__ uint32_t st[ESTH_MTGP_LARGE];
@@ -100,8 +101,8 @@ static __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict_
         tbl[tid] = p[4 + tid];
         ttbl[tid] = float_mode ? p[36 + tid] : p[20 + tid];
     }
-    const uint32_t* gin = status_in + (size_t)blockIdx.x * ESTH_MTGP_N;
-    uint32_t* gst = status_out + (size_t)blockIdx.x * ESTH_MTGP_N;
+    const uint32_t* gin = status_in + (size_t)sd.status * ESTH_MTGP_N;
+    uint32_t* gst = status_out + (size_t)sd.status * ESTH_MTGP_N;
     st[ESTH_MTGP_LARGE - ESTH_MTGP_N + tid] = gin[tid];
     if (tid < ESTH_MTGP_N - ESTH_MTGP_THREADS)
         st[ESTH_MTGP_LARGE - ESTH_MTGP_N + ESTH_MTGP_THREADS + tid] = gin[ESTH_MTGP_THREADS + tid];
@@ -981,6 +982,57 @@ __global__ __launch_bounds__(1024) void kb_alias(const esth_bfilter* __restrict_
     const esth_bfilter& f = F[e.x];
     esth_alias_sub<S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, f.m, e.y, thr, adaptive, log_w, nullptr,
                       nullptr, nullptr);
+}
+
+/* ============================================================== serving
+ * Start-up of tracks admitted in a wave: the state planes set to the initial
+ * state, the weights to zero and the track's K MTGP32 statuses seeded on the
+ * device as mtgp32_init_state (mtgp32-fast.c:119-137) does on the host, with
+ * stream j of parameter set j seeded seed + j + 1 (mtgp32-opencl.c:47).
+ * grid (blocks per track, tracks). */
+struct esth_sinit {
+    float* state;
+    float* w;
+    uint32_t* status;   /* K consecutive 351-word statuses */
+    uint32_t N, S, K, seed;
+    float init[ESTH_MAX_STATE_DIM];
+};
+
+__device__ __forceinline__ void esth_mtgp_seed_dev(const uint32_t* __restrict__ set_params, uint32_t seed,
+                                                   uint32_t* __restrict__ st) {
+    const uint32_t hidden = set_params[4 + 4] ^ (set_params[4 + 8] << 16);
+    uint32_t tmp = hidden;
+    tmp += tmp >> 16;
+    tmp += tmp >> 8;
+    const uint32_t fill = (tmp & 0xffu) * 0x01010101u;
+    uint32_t prev = seed;
+    st[0] = seed;
+    uint32_t cur = hidden ^ (1812433253u * (prev ^ (prev >> 30)) + 1u);
+    /* a[1] starts as `hidden` (not the fill) before the recursion is applied */
+    st[1] = cur;
+    prev = cur;
+    for (int i = 2; i < ESTH_MTGP_N; ++i) {
+        cur = fill ^ (1812433253u * (prev ^ (prev >> 30)) + (uint32_t)i);
+        st[i] = cur;
+        prev = cur;
+    }
+}
+
+static __global__ __launch_bounds__(256) void ks_init(const esth_sinit* __restrict__ rec,
+                                                      const uint32_t* __restrict__ params) {
+    const esth_sinit& r = rec[blockIdx.y];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t c = 0; c < r.S; ++c) {
+        const float v = r.init[c];
+        float* pl = r.state + (size_t)c * r.N;
+        for (uint32_t i = i0; i < r.N; i += stride) pl[i] = v;
+    }
+    for (uint32_t i = i0; i < r.N; i += stride) r.w[i] = 0.0f;
+    if (blockIdx.x == 0)
+        for (uint32_t j = threadIdx.x; j < r.K; j += blockDim.x)
+            esth_mtgp_seed_dev(params + (size_t)j * ESTH_MTGP_PARAM_WORDS, r.seed + j + 1u,
+                               r.status + (size_t)j * ESTH_MTGP_N);
 }
 
 /* ================================================================ copy */

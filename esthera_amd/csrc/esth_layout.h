@@ -76,6 +76,7 @@ struct esth_mtgp_stream {
     uint32_t set;       /* parameter set */
     uint32_t size;      /* words this launch, multiple of 768 */
     uint64_t out_off;   /* first output word */
+    uint64_t status;    /* index of its 351-word status in status_in / status_out */
 };
 
 /* reference rule for the 2-D torus (cuda/pf_generic.cu:682-683):

@@ -280,33 +280,97 @@ int esth_autotune(esth_engine* e, const esth_filter_desc* base, const void* para
 int esth_autotune_pick(const float us_per_mparticle[5], uint32_t particles, uint32_t min_m, uint32_t* m_out);
 
 /* ------------------------------------------------------------ serving
- * A request loop with continuous batching: tracks (filters) are opened and
- * updated by requests; every admitted request's step runs in the next launch
- * wave together with all others that arrived; device state stays resident;
- * admission is bounded by the engine's HBM budget. */
+ * A request loop with continuous batching.  Tracks (filters) are opened,
+ * updated and closed by requests, queued per track; esth_server_pump() forms
+ * a wave from the oldest queued update of every admitted track and launches it
+ * while up to `max_inflight_waves` earlier waves are still on the device (the
+ * RNG of one wave overlaps the step of the previous one).  Particles, weights
+ * and MTGP32 statuses stay resident in the server's HBM arena from admission
+ * to close; admission (FIFO) waits for a track slot and arena space, which a
+ * close frees once its last wave has completed.  A track's results are
+ * bit-identical to the same filter stepped alone (ESTH_RNG_MTGP_PER_FILTER,
+ * m >= 64), whatever else shares its waves.  Not thread-safe: one thread
+ * drives a server. */
 typedef struct esth_server esth_server;
 typedef struct {
     uint64_t request_id;
     uint64_t track_id;
     int      status;         /* ESTH_OK, or why the request failed */
     int      kind;           /* 0 open, 1 update, 2 close */
-    double   submit_us;      /* host clock at submit */
+    double   submit_us;      /* host clock (steady, microseconds) at submit */
     double   done_us;        /* host clock when the result was collected */
     esth_estimate estimate;  /* for updates */
 } esth_result;
-int esth_server_create(esth_engine* e, int max_tracks, esth_server** out);
+/* hbm_bytes: the arena for resident tracks, taken from the engine's budget. */
+int esth_server_create(esth_engine* e, int max_tracks, size_t hbm_bytes, esth_server** out);
 int esth_server_destroy(esth_server* s);
+/* waves in flight 1..3 (default 2); tracks per wave (default unbounded) */
+int esth_server_configure(esth_server* s, int max_inflight_waves, int max_wave_tracks);
+/* queue a track; its result (kind 0) comes when it has been started on the device */
 int esth_server_open(esth_server* s, const esth_filter_desc* desc, const void* params,
                      size_t params_size, const float* init_state, uint64_t* track_id,
                      uint64_t* request_id);
+/* queue one step of a track; the control and measurement are copied */
 int esth_server_update(esth_server* s, uint64_t track_id, const void* control,
                        const void* measurement, float dt, uint64_t* request_id);
+/* queue the close of a track after its queued updates; further updates fail */
 int esth_server_close(esth_server* s, uint64_t track_id, uint64_t* request_id);
-/* Launch all queued work that fits (non-blocking), collect finished results. */
+/* collect finished waves and launch what is queued and fits (non-blocking) */
 int esth_server_pump(esth_server* s);
+/* take up to max_results collected results, oldest first */
 int esth_server_poll(esth_server* s, esth_result* out, int max_results, int* n_out);
+/* pump until everything that can run has completed */
 int esth_server_drain(esth_server* s);
 int esth_server_stats(esth_server* s, size_t* hbm_used, int* open_tracks, int* queued);
+int esth_server_wave_stats(esth_server* s, uint64_t* waves, uint64_t* wave_updates, size_t* hbm_peak);
+/* particles of an admitted track (waits for the device) */
+int esth_server_read_particles(esth_server* s, uint64_t track_id, float* state_soa, float* weights);
+
+/* A seeded request stream driven against a server (or, with `plain`, against
+ * one esth_filter per track with per-request launches and a blocking estimate
+ * per update).  rate > 0: Poisson arrivals over duration_s, a share
+ * open_fraction opening tracks (unicycle with probability unicycle_fraction,
+ * else the robot arm, particle counts log-uniform in [min_particles,
+ * max_particles] rounded to multiples of 64), the rest updating a uniformly
+ * chosen open track, each track closing after a geometric number of updates
+ * of mean mean_lifetime (0: never).  rate == 0: closed loop, initial_tracks
+ * tracks each with one update always queued.  Latency of an update: from its
+ * arrival (closed loop: its submit) to its collected result.  Unicycle tracks
+ * replay eight seeded scenarios with `landmarks` landmarks; robot-arm tracks
+ * replay the given trace.  Sub-filter size: esth_autotune_pick over
+ * us_per_mparticle when given, else the largest power of two <= 256 dividing
+ * the count. */
+typedef struct {
+    double   rate;
+    double   duration_s;
+    double   open_fraction;
+    double   unicycle_fraction;
+    uint32_t min_particles, max_particles;
+    double   mean_lifetime;
+    int      initial_tracks;
+    int      landmarks;
+    int      topology, num_transfer, resampler, weights;
+    uint32_t seed;
+    int      plain;
+    size_t   hbm_bytes;
+    int      max_tracks;
+    int      max_inflight;
+    const float* us_per_mparticle;
+    const esth_robotarm5_control*     ra_controls;
+    const esth_robotarm5_measurement* ra_measurements;
+    const float* ra_dt;
+    int      ra_steps;
+} esth_serve_workload;
+typedef struct {
+    uint64_t requests, opens, updates, closes, failed, backlog;
+    double   wall_s, updates_per_s, particle_updates_per_s, mean_particles;
+    double   p50_us, p99_us, mean_us, max_us;
+    int      peak_tracks;
+    size_t   hbm_peak;
+    uint64_t waves;
+    double   mean_wave_updates;
+} esth_serve_report;
+int esth_serve_simulate(esth_engine* e, const esth_serve_workload* w, esth_serve_report* out);
 
 /* ------------------------------------------------- device memory helpers */
 int esth_set_device(int device);

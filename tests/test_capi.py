@@ -1,0 +1,133 @@
+"""The C ABI without a GPU: the library loads and exports every function the
+header declares; host-side entry points (models, frames, MTGP seeding and
+parameter table, scenarios, trace reading, autotuner choice) behave."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import esthera_amd as ea
+import oracle as O
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(REPO, "include", "esthera.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*[\w\s\*]+?\b(esth_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_function():
+    lib = C.CDLL(ea.LIB_PATH)
+    names = header_functions()
+    assert len(names) > 60
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_models_and_defaults():
+    L = ea.lib()
+    assert L.esth_model_lookup(b"robotarm5") == 0 and L.esth_model_lookup(b"unicycle") == 1
+    assert L.esth_model_lookup(b"dh6") == 2 and L.esth_model_lookup(b"nope") == -1
+    S, V, ps, cs, ms = C.c_int(), C.c_int(), C.c_size_t(), C.c_size_t(), C.c_size_t()
+    for model, (s, v) in {0: (9, 9), 1: (3, 3), 2: (12, 6)}.items():
+        ea.check(L.esth_model_info(model, C.byref(S), C.byref(V), C.byref(ps), C.byref(cs), C.byref(ms)))
+        assert (S.value, V.value) == (s, v)
+        assert ms.value == C.sizeof(ea.MEAS_TYPE[model])
+    d = ea.FilterDesc()
+    L.esth_filter_desc_default(C.byref(d), 0)
+    assert (d.num_particles, d.num_blocks, d.num_transfer) == (256, 16, 1)   # pf_generic.c:1072-1079
+    assert d.resampling_threshold == 1.0 and d.topology == ea.TOPO_RING
+
+
+def test_frames_match_oracle():
+    p, ctrl, meas, truth = ea.unicycle_scenario(42, 32, 50.0, 2)
+    assert np.array_equal(ea.build_frame(1, p, ctrl[1], meas[1], 0.1).view(np.uint32),
+                          O.build_frame(1, p, ctrl[1], meas[1], 0.1).view(np.uint32))
+    dp, dmeas, dtruth = ea.dh6_scenario(7, 2)
+    assert np.array_equal(ea.build_frame(2, dp, ea.DH6Control(), dmeas[0], 0.01).view(np.uint32),
+                          O.build_frame(2, dp, ea.DH6Control(), dmeas[0], 0.01).view(np.uint32))
+    with pytest.raises(ea.EsthError):
+        bad = ea.UnicycleParams()
+        bad.num_landmarks = 70
+        ea.build_frame(1, bad, ctrl[0], meas[0], 0.1)
+
+
+def test_mtgp_table_and_seeding(golden, mtgp_table):
+    assert np.array_equal(ea.mtgp_params(), mtgp_table)
+    for k, i in enumerate(golden["sets_init"]):
+        assert np.array_equal(ea.mtgp_seed_state(int(i), 1234 + int(i) + 1), golden["init_1234"][k])
+
+
+def test_unicycle_scenario_statistics():
+    p, ctrl, meas, truth = ea.unicycle_scenario(42, 32, 50.0, 500)
+    assert p.num_landmarks == 32
+    lx, ly = np.array(p.lx[:32]), np.array(p.ly[:32])
+    assert np.all(np.abs(lx) <= 50) and np.all(np.abs(ly) <= 50)
+    om = np.array([c.omega for c in ctrl])
+    assert np.allclose(om, 0.2 * np.sin(0.05 * np.arange(500)), atol=1e-6)
+    # truth follows x += v dt cos(theta) + noise: residual std ~ 0.05
+    dx = truth[1:, 0] - truth[:-1, 0] - 0.1 * np.cos(truth[:-1, 2])
+    assert 0.04 < np.std(dx) < 0.06
+    # range residuals ~ 0.1, bearing residuals ~ 0.02
+    r = np.array([m.range[:32] for m in meas])
+    rt = np.hypot(lx[None, :] - truth[:, 0:1], ly[None, :] - truth[:, 1:2])
+    assert 0.09 < np.std(r - rt) < 0.11
+    b = np.array([m.bearing[:32] for m in meas])
+    bt = np.arctan2(ly[None, :] - truth[:, 1:2], lx[None, :] - truth[:, 0:1]) - truth[:, 2:3]
+    eb = np.angle(np.exp(1j * (b - bt)))
+    assert 0.018 < np.std(eb) < 0.022
+    p4, _, _, _ = ea.unicycle_scenario(1234, 4, 0.0, 1)
+    assert sorted(zip(p4.lx[:4], p4.ly[:4])) == sorted([(10, 10), (-10, 10), (-10, -10), (10, -10)])
+
+
+def test_dh6_scenario():
+    p, meas, truth = ea.dh6_scenario(7, 200)
+    a, d = np.array(p.a), np.array(p.d)
+    assert np.all((a >= 0.2) & (a <= 0.6)) and np.all((d >= -0.1) & (d <= 0.1))
+    enc = np.array([m.q[:6] for m in meas]) - truth[1:, :6]
+    assert 0.004 < np.std(enc) < 0.006
+
+
+def test_trace_loader_reads_reference_trace():
+    path = os.path.join(REPO, "tests", "golden", "robotarm_input5.txt")
+    meas, ctrl, dt, truth = ea.load_robotarm_trace(path)
+    assert len(meas) == 100
+    first = [float(x) for x in open(path).readline().split()]
+    assert np.isclose(meas[0].x, first[0]) and np.isclose(meas[0].angles[4], first[6])
+    assert np.isclose(ctrl[0].angles[0], first[7]) and np.isclose(dt[0], first[12])
+    assert np.isclose(truth[0, 5], first[13]) and np.isclose(truth[0, 0], first[15])
+
+
+def test_trace_loader_stops_at_garbled_row(tmp_path):
+    p = tmp_path / "t.txt"
+    row = " ".join(["0.5"] * 20)
+    p.write_text(row + "\n" + row + "\n" + "1 2 3 x\n")
+    meas, ctrl, dt, truth = ea.load_robotarm_trace(str(p))
+    assert len(meas) == 2
+    with pytest.raises(ea.EsthError):
+        ea.load_robotarm_trace(str(tmp_path / "missing.txt"))
+
+
+def test_estimate_error_matches_reference_formula():
+    est = np.arange(9, dtype=np.float32)
+    act = est + np.float32(0.5)
+    e = ea.lib().esth_robotarm5_estimate_error(ea.ptr(est), ea.ptr(act))
+    assert np.isclose(e, 7 * 0.25 * 0.1, rtol=1e-6)   # 5 angles + x + y, each (0.5)^2 * 0.1
+
+
+def test_autotune_pick_rules():
+    us = np.array([5.0, 4.0, 3.0, 3.5, 3.0], np.float32)   # m = 64 .. 1024
+    assert ea.autotune_pick(us, 65536, 256) == 1024          # tie 256 vs 1024: larger
+    assert ea.autotune_pick(us, 64 * 3, 256) == 64           # only 64 divides 192
+    assert ea.autotune_pick(us, 512 * 3, 256) == 256         # 256 cheaper than 512
+    assert ea.autotune_pick(us, 128 * 5, 64) == 128
+    with pytest.raises(ea.EsthError):
+        ea.autotune_pick(us, 100, 64)
+
+
+def test_batch_autotune_decl_parses():
+    assert "esth_batch_create" in header_functions() and "esth_autotune" in header_functions()
