@@ -93,6 +93,141 @@ def cpu_baseline(budget_s=15.0, max_steps=200):
                       f"restatement, OpenMP {threads} threads, {dt:.1f} s"}
 
 
+def run_batched(ea, eng, steps=200, warmup=3, nf=10000, plain_every=20):
+    """10,000 unicycle filters with 8 landmarks each (64 seeded landmark sets),
+    particle counts log-uniform in [256, 65536] rounded to multiples of 64
+    (seed 99), ring exchange t=1, systematic resampling, sub-filter size per
+    filter from the autotuner; the whole batch stepped by one graph launch per
+    step.  Beside it, every `plain_every`-th filter stepped as its own filter
+    (per-filter launches), and the two compared filter by filter."""
+    import numpy as np
+    rng = np.random.default_rng(99)
+    counts = np.maximum((np.round(np.exp(rng.uniform(np.log(256), np.log(65536), nf)) / 64) * 64).astype(np.int64), 256)
+    NS = 64
+    scen = [ea.unicycle_scenario(1000 + j, 8, 20.0, 8) for j in range(NS)]
+    p0, c0, m0, _ = scen[0]
+    d0 = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=4, t=1, topology=ea.TOPO_RING, resampler=ea.RESAMPLE_SYSTEMATIC,
+                        weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=1)
+    us = ea.autotune(eng, d0, p0, c0[0], m0[0], 0.1)
+    descs, params = [], []
+    for i, n in enumerate(counts):
+        m = ea.autotune_pick(us, int(n), 64)
+        Gi = int(n) // m
+        descs.append(ea.filter_desc(ea.MODEL_UNICYCLE, m=m, G=Gi, t=1,
+                                    topology=ea.TOPO_RING if Gi > 1 else ea.TOPO_NONE,
+                                    resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG,
+                                    rng=ea.RNG_MTGP_PER_FILTER, seed=99 * 1000003 + i))
+        params.append(scen[i % NS][0])
+    b = ea.Batch(eng, descs, params)
+    b.init_uniform(np.zeros((nf, 3), np.float32))
+    IN = [((ea.UnicycleControl * nf)(*[scen[i % NS][1][k] for i in range(nf)]),
+           (ea.UnicycleMeasurement * nf)(*[scen[i % NS][2][k] for i in range(nf)])) for k in range(8)]
+    dts = np.full(nf, 0.1, np.float32)
+    for k in range(warmup):
+        b.step(IN[k % 8][0], IN[k % 8][1], dts)
+    b.sync()
+    b.step_times()
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        b.step(IN[k % 8][0], IN[k % 8][1], dts)
+    b.sync()
+    secs = time.perf_counter() - t0
+    st = b.step_times() / 1e3
+    sub = list(range(0, nf, plain_every))
+    fs = []
+    for i in sub:
+        f = ea.Filter(eng, descs[i], params[i])
+        f.init_uniform(np.zeros(3, np.float32))
+        fs.append(f)
+    for k in range(warmup):
+        for j, f in enumerate(fs):
+            f.step(scen[sub[j] % NS][1][k % 8], scen[sub[j] % NS][2][k % 8], 0.1)
+    ea.device_sync()
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        for j, f in enumerate(fs):
+            f.step(scen[sub[j] % NS][1][k % 8], scen[sub[j] % NS][2][k % 8], 0.1)
+    ea.device_sync()
+    psecs = time.perf_counter() - t0
+    worst, same = 0.0, 0
+    for j, f in enumerate(fs):
+        a, e = b.estimate(sub[j]), f.estimate()
+        ma, me = np.array(a.mean[:3]), np.array(e.mean[:3])
+        worst = max(worst, float(np.max(np.abs(ma - me) / np.maximum(np.abs(me), 1.0))))
+        sa, _ = b.particles(sub[j])
+        se, _ = f.particles()
+        same += int(np.array_equal(sa.view(np.uint32), se.view(np.uint32)))
+        f.close()
+    total = int(counts.sum())
+    ms = [d.num_particles for d in descs]
+    b.close()
+    plain_particles = int(sum(int(counts[i]) for i in sub))
+    return {
+        "workload": f"{nf} unicycle filters, 8 landmarks each, particles log-uniform in [256, 65536] (x64, "
+                    f"seed 99), ring t=1, systematic resampling, per-filter MTGP32 streams, one HIP graph per step",
+        "filters": nf, "particles": total, "steps": steps,
+        "sub_filter_sizes": {str(m): ms.count(m) for m in sorted(set(ms))},
+        "autotune_us_per_mparticle": {str(64 << k): float(us[k]) for k in range(5)},
+        "ms_per_step": secs * 1e3 / steps,
+        "particles_per_s": total * steps / secs,
+        "filter_updates_per_s": nf * steps / secs,
+        "step_latency_ms": {"p50": float(np.percentile(st, 50)), "p99": float(np.percentile(st, 99)),
+                            "max": float(np.max(st))},
+        "plain": {"filters": len(sub), "particles": plain_particles, "ms_per_round": psecs * 1e3 / steps,
+                  "filter_updates_per_s": len(sub) * steps / psecs,
+                  "particles_per_s": plain_particles * steps / psecs},
+        "speedup_filter_updates": (nf * steps / secs) / (len(sub) * steps / psecs),
+        "parity_vs_plain": {"filters_compared": len(sub), "bit_identical_particles": same,
+                            "max_rel_mean_diff": worst},
+    }
+
+
+def run_served(ea, eng, rates=(1000.0, 5000.0, 20000.0, 50000.0), duration=2.0, tracks=512):
+    """Poisson request streams (5 % opens: 70 % unicycle with 8 landmarks / 30 %
+    robot arm replaying input5, particles log-uniform in [1024, 262144]; 95 %
+    updates of a uniformly chosen open track; geometric lifetime of mean 500
+    updates; HBM arena 200 GB or what the device holds) at each rate, then the
+    closed-loop capacity of the server against plain per-request launches on
+    the same `tracks` tracks."""
+    import numpy as np
+    meas, ctrl, dts, _ = ea.load_robotarm_trace(os.path.join(REPO, "tests", "golden", "robotarm_input5.txt"))
+    p, c, m, _ = ea.unicycle_scenario(1000, 8, 20.0, 2)
+    d0 = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=4, t=1, topology=ea.TOPO_RING, resampler=ea.RESAMPLE_SYSTEMATIC,
+                        weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=1)
+    us = ea.autotune(eng, d0, p, c[0], m[0], 0.1)
+    used, budget = eng.memory()
+    arena = int(min(200 * 10 ** 9, 0.7 * (budget - used)))
+
+    def workload(rate, initial, lifetime, plain, dur):
+        return ea.ServeWorkload(rate=rate, duration_s=dur, open_fraction=0.05, unicycle_fraction=0.7,
+                                min_particles=1024, max_particles=262144, mean_lifetime=lifetime,
+                                initial_tracks=initial, landmarks=8, topology=ea.TOPO_TORUS, num_transfer=1,
+                                resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, seed=4, plain=plain,
+                                hbm_bytes=arena, max_tracks=1 << 20, max_inflight=2)
+
+    keys = ("updates_per_s", "p50_us", "p99_us", "max_us", "mean_wave_updates", "peak_tracks", "hbm_peak",
+            "backlog", "failed", "mean_particles")
+    sweep = []
+    for rate in rates:
+        r = ea.serve_simulate(eng, workload(rate, 256, 500.0, 0, duration), (meas, ctrl, dts), us)
+        sweep.append(dict({"rate": rate}, **{k: r[k] for k in keys}))
+    cap = ea.serve_simulate(eng, workload(0.0, tracks, 0.0, 0, duration), (meas, ctrl, dts), us)
+    plain = ea.serve_simulate(eng, workload(0.0, tracks, 0.0, 1, duration), (meas, ctrl, dts), us)
+    return {
+        "workload": "Poisson requests: 5% open (70% unicycle 8 landmarks / 30% robot arm on input5; particles "
+                    "log-uniform [1024, 262144]), 95% update, geometric lifetime mean 500 updates",
+        "hbm_arena_bytes": arena,
+        "duration_s": duration,
+        "sweep": sweep,
+        "capacity": {"tracks": tracks,
+                     "served": {k: cap[k] for k in ("updates_per_s", "particle_updates_per_s", "p50_us", "p99_us",
+                                                    "mean_wave_updates")},
+                     "plain": {k: plain[k] for k in ("updates_per_s", "particle_updates_per_s", "p50_us",
+                                                     "p99_us")},
+                     "speedup_updates": cap["updates_per_s"] / max(plain["updates_per_s"], 1e-9)},
+    }
+
+
 def traffic_from_profiles():
     p = os.path.join(REPO, "profiles", "sample_weight_c1_traffic.json")
     if not os.path.exists(p):
@@ -109,6 +244,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-alias", action="store_true", help="skip the Vose alias variant")
     ap.add_argument("--graph-only", action="store_true", help="only the HIP-graph replay run (profiling)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 10k-filter batched workload")
+    ap.add_argument("--no-served", action="store_true", help="skip the served request-stream workload")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -170,6 +307,10 @@ def main():
         out["alias"] = {"value": N * max(1, args.steps // 2) / asecs, "unit": "particles/s",
                         "ms_per_step": asecs * 1e3 / max(1, args.steps // 2),
                         "stages_us": {k: v for k, v in zip(STAGES, astage) if v == v}}
+    if not args.no_batched:
+        out["batched"] = run_batched(ea, eng)
+    if not args.no_served:
+        out["served"] = run_served(ea, eng)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     eng.close()

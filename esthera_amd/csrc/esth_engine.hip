@@ -505,7 +505,8 @@ extern "C" int esth_stage_copy(void* dst, const void* src, size_t bytes) {
 
 extern "C" int esth_stage_math(int fn, const float* x, const float* x2, float* out, int n) {
     if (n <= 0) return ESTH_OK;
-    if (fn == 4 && !x2) return esth_fail(ESTH_ERR_INVALID, "atan2 needs x2");
+    if (fn < 0 || fn > 7) return esth_fail(ESTH_ERR_INVALID, "math function %d", fn);
+    if ((fn == 4 || fn == 5 || fn == 6) && !x2) return esth_fail(ESTH_ERR_INVALID, "function %d needs x2", fn);
     hipLaunchKernelGGL(k_math, dim3(grid_for((uint64_t)n, 256)), dim3(256), 0, 0, fn, x, x2, out, n);
     ESTH_HIP(hipGetLastError());
     ESTH_HIP(hipDeviceSynchronize());

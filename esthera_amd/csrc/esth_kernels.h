@@ -799,11 +799,16 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
     uint32_t slot = small ? rk : (M - 1u - ((uint32_t)l - rk));
     sl[slot] = (uint32_t)l;
     uint32_t small_end = nsmall, small_begin = 0;
+    /* Pairing rounds.  The pairs of a round never outnumber those of the
+     * round before (a round of n = min(small, large) pairs leaves at most n of
+     * the smaller side), so once a round fits in one wave, wave 0 runs every
+     * remaining round alone with wave-level ordering instead of work-group
+     * barriers: the same rounds in the same order, hence the same table. */
     for (;;) {
         __syncthreads();
         uint32_t a = small_end - small_begin, b = M - small_end;
         uint32_t nactive = a < b ? a : b;
-        if (nactive == 0) break;
+        if (nactive <= (uint32_t)ESTH_WAVE) break;
         const bool act = (uint32_t)l < nactive;
         uint32_t si = 0, li = 0;
         float pS = 0.0f, pL = 0.0f;
@@ -829,6 +834,43 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
         }
         small_end += nbs;
         small_begin += nactive;
+    }
+    if (l < ESTH_WAVE) {
+        const uint32_t lane = (uint32_t)l;
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        for (;;) {
+            const uint32_t a = small_end - small_begin, b = M - small_end;
+            const uint32_t nactive = a < b ? a : b;
+            if (nactive == 0) break;
+            const bool act = lane < nactive;
+            uint32_t si = 0, li = 0;
+            float pS = 0.0f, pL = 0.0f;
+            if (act) {
+                si = sl[small_begin + lane];
+                li = sl[small_end + lane];
+                pS = prob[si];
+                alias[si] = li;
+                pL = prob[li];
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            bool bs = false;
+            if (act) {
+                pL = (pL + pS) - 1.0f;
+                bs = pL <= 1.0f;
+            }
+            const uint64_t bal = __ballot(act && bs);
+            const uint32_t rs = (uint32_t)__popcll(bal & below), nbs = (uint32_t)__popcll(bal);
+            if (act) {
+                const uint32_t s2 = bs ? (small_end + rs) : (small_end - 1u + nactive - (lane - rs));
+                sl[s2] = li;
+                prob[li] = pL;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            small_end += nbs;
+            small_begin += nactive;
+        }
     }
     __syncthreads();
     if (table_only) {
@@ -1053,6 +1095,9 @@ static __global__ void k_math(int fn, const float* __restrict__ x, const float* 
     case 2: esth_sincosf(v, &s, &c); r = s; break;
     case 3: esth_sincosf(v, &s, &c); r = c; break;
     case 4: r = esth_atan2f(v, x2[i]); break;
+    case 5: esth_polar_abs(v, x2[i], &r, &s); break;
+    case 6: esth_polar_abs(v, x2[i], &s, &r); break;
+    case 7: r = esth_rcpf(v); break;
     default: r = 0.0f;
     }
     out[i] = r;

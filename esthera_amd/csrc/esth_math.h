@@ -193,6 +193,55 @@ ESTH_HD float esth_atan2f(float y, float x) {
     return (y < 0.0f) ? -a : a;
 }
 
+/* sqrt(1 + s) on s in [0, 1]: degree-7 fit (gen_atan_poly.py), max relative
+ * error 1.2e-7 */
+#define ESTH_HYP_C0 1.0f                     /* 0x3f800000 */
+#define ESTH_HYP_C1 0.4999963343143463f      /* 0x3effff85 */
+#define ESTH_HYP_C2 -0.12491840124130249f    /* 0xbdffd538 */
+#define ESTH_HYP_C3 0.061791423708200455f    /* 0x3d7d1901 */
+#define ESTH_HYP_C4 -0.03592449799180031f    /* 0xbd132591 */
+#define ESTH_HYP_C5 0.019269991666078568f    /* 0x3c9ddc1a */
+#define ESTH_HYP_C6 -0.007379245478659868f   /* 0xbbf1cd99 */
+#define ESTH_HYP_C7 0.001377946580760181f    /* 0x3ab49c37 */
+
+/* 1/x for normal x > 0 without a divide: an integer first guess (relative
+ * error below 0.051 over every binade) and three Newton steps y += y (1 - x y)
+ * in fused multiply-adds, each squaring the error; within 1 ulp. */
+ESTH_HD float esth_rcpf(float x) {
+    float y = esth_u2f(0x7ef311c3u - esth_f2u(x));
+    float e = esth_fma(-x, y, 1.0f);
+    y = esth_fma(y, e, y);
+    e = esth_fma(-x, y, 1.0f);
+    y = esth_fma(y, e, y);
+    e = esth_fma(-x, y, 1.0f);
+    return esth_fma(y, e, y);
+}
+
+/* Length r and |angle| of the vector (u, v): r = max * sqrt(1 + t^2) and
+ * |atan2(v, u)| in [0, pi] from one ratio t = min/max in [0, 1] (one
+ * reciprocal, one square shared by both polynomials).  r is within about
+ * 3 ulp of hypot(u, v), the angle within 1.2e-7 rad.  Only multiplies, fused
+ * multiply-adds, min/max, compares and an integer subtract: no divide and no
+ * square root, whose hardware sequences are the costly part of a likelihood. */
+ESTH_HD void esth_polar_abs(float u, float v, float* r, float* ang) {
+    const float au = fabsf(u), av = fabsf(v);
+    const float mx = fmaxf(au, av), mn = fminf(au, av);
+    const float t = mn * esth_rcpf(fmaxf(mx, 1e-30f));
+    const float s = t * t;
+    float a = esth_atanf_unit(t);
+    a = (av > au) ? ESTH_PI_2 - a : a;
+    *ang = (u < 0.0f) ? ESTH_PI - a : a;
+    float q = ESTH_HYP_C7;
+    q = esth_fma(q, s, ESTH_HYP_C6);
+    q = esth_fma(q, s, ESTH_HYP_C5);
+    q = esth_fma(q, s, ESTH_HYP_C4);
+    q = esth_fma(q, s, ESTH_HYP_C3);
+    q = esth_fma(q, s, ESTH_HYP_C2);
+    q = esth_fma(q, s, ESTH_HYP_C1);
+    q = esth_fma(q, s, ESTH_HYP_C0);
+    *r = mx * q;
+}
+
 /* wrap an angle to (-pi, pi] using a 2-part 2*pi */
 ESTH_HD float esth_wrap_pi(float a) {
     float k = esth_rint(a * (1.0f / ESTH_2PI));

@@ -111,19 +111,22 @@ ESTH_HD void esth_unicycle_propagate(float* s, const float* z, const float* fr, 
     esth_sincosf(th, stp, ctp);
 }
 
-/* one landmark's squared range and bearing residuals, accumulated */
+/* one landmark's squared range and bearing residuals, accumulated.  The
+ * landmark offset is rotated by -(theta + zb): its length is the predicted
+ * range and its angle the bearing residual wrapped to (-pi, pi]; only the
+ * residual's square enters, so its magnitude suffices (esth_polar_abs). */
 ESTH_HD void esth_unicycle_landmark(const float* lm, float x, float y, float ct, float st, float* acc_r,
                                     float* acc_b) {
     float dx = lm[0] - x;
     float dy = lm[1] - y;
-    float r = sqrtf(esth_fma(dx, dx, dy * dy));
-    float er = lm[2] - r;
-    *acc_r = esth_fma(er, er, *acc_r);
     float C = esth_fma(ct, lm[3], -(st * lm[4]));   /* cos(theta + zb) */
     float S = esth_fma(st, lm[3], ct * lm[4]);      /* sin(theta + zb) */
     float u = esth_fma(dx, C, dy * S);
     float v = esth_fma(dy, C, -(dx * S));
-    float eb = esth_atan2f(v, u);
+    float r, eb;
+    esth_polar_abs(u, v, &r, &eb);
+    float er = lm[2] - r;
+    *acc_r = esth_fma(er, er, *acc_r);
     *acc_b = esth_fma(eb, eb, *acc_b);
 }
 

@@ -429,7 +429,9 @@ int esth_stage_alias_table(const float* d_w, uint32_t m, uint32_t G, int weights
  * used to calibrate HBM byte counters. */
 int esth_stage_copy(void* d_dst, const void* d_src, size_t bytes);
 /* Deterministic math functions evaluated on the device (n inputs): fn 0 exp,
- * 1 log, 2 sin, 3 cos, 4 atan2(y=x[i], x=x2[i]). */
+ * 1 log, 2 sin, 3 cos, 4 atan2(y=x[i], x=x2[i]), 5 length and 6 |angle| of the
+ * vector (x[i], x2[i]) (the unicycle likelihood's division-free forms),
+ * 7 reciprocal. */
 int esth_stage_math(int fn, const float* d_x, const float* d_x2, float* d_out, int n);
 
 #ifdef __cplusplus

@@ -609,6 +609,8 @@ float or_expf(float x) { return esth_expf(x); }
 float or_logf(float x) { return esth_logf(x); }
 void or_sincosf(float x, float* s, float* c) { esth_sincosf(x, s, c); }
 float or_atan2f(float y, float x) { return esth_atan2f(y, x); }
+void or_polar_abs(float u, float v, float* r, float* a) { esth_polar_abs(u, v, r, a); }
+float or_rcpf(float x) { return esth_rcpf(x); }
 
 /* =========================================================== whole step
  * particle_filter's loop body, pf_generic.c:438-808. */

@@ -46,6 +46,8 @@ float or_expf(float x);
 float or_logf(float x);
 void  or_sincosf(float x, float* s, float* c);
 float or_atan2f(float y, float x);
+void  or_polar_abs(float u, float v, float* r, float* a);
+float or_rcpf(float x);
 
 /* whole filter (FP32 restatement, bit-exact target) with an FP64 posterior */
 typedef struct or_filter or_filter;

@@ -54,6 +54,9 @@ def lib():
         L.or_atan2f.argtypes = [C.c_float, C.c_float]
         L.or_atan2f.restype = C.c_float
         L.or_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.or_polar_abs.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.or_rcpf.argtypes = [C.c_float]
+        L.or_rcpf.restype = C.c_float
         L.or_filter_create.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, _u32p]
         L.or_filter_create.restype = C.c_void_p
         L.or_filter_destroy.argtypes = [C.c_void_p]

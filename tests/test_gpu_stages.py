@@ -90,7 +90,7 @@ def test_boxmuller_bit_exact(engine):
     assert np.max(np.abs(got - ref)) < 2e-6
 
 
-@pytest.mark.parametrize("fn", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("fn", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_math_bit_exact(fn, engine):
     rng = np.random.default_rng(fn)
     n = 1 << 16
@@ -100,6 +100,8 @@ def test_math_bit_exact(fn, engine):
         x = np.exp(rng.uniform(-90, 88, n)).astype(np.float32)
     elif fn in (2, 3):
         x = rng.uniform(-1000, 1000, n).astype(np.float32)
+    elif fn == 7:
+        x = np.exp(rng.uniform(-80, 80, n)).astype(np.float32)
     else:
         x = rng.normal(size=n).astype(np.float32)
     x2 = rng.normal(size=n).astype(np.float32)
@@ -113,6 +115,16 @@ def test_math_bit_exact(fn, engine):
         ref = np.array([L.or_logf(v) for v in x], np.float32)
     elif fn == 4:
         ref = np.array([L.or_atan2f(a, b) for a, b in zip(x, x2)], np.float32)
+    elif fn in (5, 6):
+        import ctypes as C
+        r, a = C.c_float(), C.c_float()
+        ref = []
+        for uu, vv in zip(x, x2):
+            L.or_polar_abs(uu, vv, C.byref(r), C.byref(a))
+            ref.append(r.value if fn == 5 else a.value)
+        ref = np.array(ref, np.float32)
+    elif fn == 7:
+        ref = np.array([L.or_rcpf(v) for v in x], np.float32)
     else:
         import ctypes as C
         s, c = C.c_float(), C.c_float()
@@ -310,3 +322,26 @@ def test_alias_table_bit_exact_and_invariant(m, G, engine):
         mass = pr.copy()
         np.add.at(mass, alias[g * m:(g + 1) * m], 1.0 - pr)
         np.testing.assert_allclose(mass, m * p, atol=2e-5 * m)
+
+
+@pytest.mark.parametrize("shape", ["peaked", "two-heavy", "flat"])
+@pytest.mark.parametrize("m", [128, 1024])
+def test_alias_table_pairing_regimes(shape, m, engine):
+    """Weights whose pairing runs many short rounds (one or two heavy
+    particles), and equal weights (no round at all), in log mode."""
+    rng = np.random.default_rng(m + len(shape))
+    G = 6
+    if shape == "peaked":
+        lw = (30.0 * rng.random(m * G)).astype(np.float32)
+    elif shape == "two-heavy":
+        lw = rng.normal(size=m * G).astype(np.float32)
+        lw[::m] += 9.0
+        lw[m // 2::m] += 8.0
+    else:
+        lw = np.full(m * G, -3.25, np.float32)
+    dw, dp, da = DB.from_array(lw), DB(m * G * 4), DB(m * G * 4)
+    ea.check(ea.lib().esth_stage_alias_table(dw.ptr, m, G, ea.WEIGHTS_LOG, dp.ptr, da.ptr))
+    prob, alias = dp.to_array(np.float32, m * G), da.to_array(np.uint32, m * G)
+    op, oa = O.alias_table(lw, m, G, ea.WEIGHTS_LOG)
+    assert_bits(prob, op)
+    assert_bits(alias, oa)

@@ -108,6 +108,34 @@ def test_atan2f_error_bound():
         assert abs(L.or_atan2f(a, b) - np.arctan2(a, b)) < 3e-7
 
 
+def test_rcpf_within_one_ulp():
+    L = O.lib()
+    rng = np.random.default_rng(2)
+    x = np.concatenate([np.exp(rng.uniform(-60, 60, 20000)), [1.0, 2.0, 3.0, 1e-30, 0.999999]]).astype(np.float32)
+    got = np.array([L.or_rcpf(v) for v in x], np.float32)
+    ref = (1.0 / x.astype(np.float64)).astype(np.float32)
+    assert np.max(np.abs(got.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))) <= 1
+    assert L.or_rcpf(1.0) == 1.0 and L.or_rcpf(2.0) == 0.5
+
+
+def test_polar_abs_error_bound():
+    import ctypes as C
+    L = O.lib()
+    rng = np.random.default_rng(3)
+    u = (rng.normal(size=20000) * np.exp(rng.uniform(-3, 5, 20000))).astype(np.float32)
+    v = (rng.normal(size=20000) * np.exp(rng.uniform(-3, 5, 20000))).astype(np.float32)
+    pts = list(zip(u, v)) + [(0.0, 0.0), (1.0, 0.0), (-1.0, 0.0), (0.0, 2.0), (0.0, -2.0), (3.0, 3.0),
+                             (-3.0, -3.0), (1e-20, -1e-20)]
+    r, a = C.c_float(), C.c_float()
+    for uu, vv in pts:
+        L.or_polar_abs(float(uu), float(vv), C.byref(r), C.byref(a))
+        h = np.hypot(np.float64(uu), np.float64(vv))
+        assert abs(r.value - h) <= 3.6e-7 * h + 1e-37, (uu, vv, r.value, h)
+        assert abs(a.value - abs(np.arctan2(np.float64(vv), np.float64(uu)))) < 3e-7, (uu, vv)
+    L.or_polar_abs(0.0, 0.0, C.byref(r), C.byref(a))
+    assert r.value == 0.0 and a.value == 0.0
+
+
 def test_boxmuller_closed_form():
     rng = np.random.default_rng(1)
     u = (1.0 + rng.random(10000, dtype=np.float32)).astype(np.float32)
