@@ -18,15 +18,15 @@ LIB_PATH = os.path.join(HERE, "libesthera.so")
 
 # --------------------------------------------------------------- constants
 ESTH_OK = 0
-MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6 = 0, 1, 2
+MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6, MODEL_UNICYCLE10 = 0, 1, 2, 3
 TOPO_NONE, TOPO_RING, TOPO_TORUS, TOPO_ALL2ALL = 0, 1, 2, 3
 RESAMPLE_MULTINOMIAL, RESAMPLE_SYSTEMATIC, RESAMPLE_ALIAS = 0, 1, 2
 WEIGHTS_LINEAR, WEIGHTS_LOG = 0, 1
 RNG_MTGP_COMPAT, RNG_MTGP_PER_FILTER = 0, 1
 FRAME_FLOATS = 336
 MAX_LANDMARKS = 64
-STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12}
-NOISE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 6}
+STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12, MODEL_UNICYCLE10: 3}
+NOISE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 6, MODEL_UNICYCLE10: 3}
 
 
 # ------------------------------------------------------------- structures
@@ -78,6 +78,15 @@ class DH6Measurement(C.Structure):
     _fields_ = [("p", C.c_float * 3), ("q", C.c_float * 6)]
 
 
+class Unicycle10Params(C.Structure):
+    _fields_ = [("sx", C.c_float * 10), ("sy", C.c_float * 10), ("sz", C.c_float * 10), ("noise_v", C.c_float),
+                ("noise_omega", C.c_float), ("noise_gamma", C.c_float)]
+
+
+class Unicycle10Measurement(C.Structure):
+    _fields_ = [("range", C.c_float * 10)]
+
+
 class Result(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("track_id", C.c_uint64), ("status", C.c_int), ("kind", C.c_int),
                 ("submit_us", C.c_double), ("done_us", C.c_double), ("estimate", Estimate)]
@@ -105,10 +114,12 @@ class ServeReport(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
-PARAMS_TYPE = {MODEL_ROBOTARM5: None, MODEL_UNICYCLE: UnicycleParams, MODEL_DH6: DH6Params}
-CONTROL_TYPE = {MODEL_ROBOTARM5: RobotArm5Control, MODEL_UNICYCLE: UnicycleControl, MODEL_DH6: DH6Control}
+PARAMS_TYPE = {MODEL_ROBOTARM5: None, MODEL_UNICYCLE: UnicycleParams, MODEL_DH6: DH6Params,
+               MODEL_UNICYCLE10: Unicycle10Params}
+CONTROL_TYPE = {MODEL_ROBOTARM5: RobotArm5Control, MODEL_UNICYCLE: UnicycleControl, MODEL_DH6: DH6Control,
+                MODEL_UNICYCLE10: UnicycleControl}
 MEAS_TYPE = {MODEL_ROBOTARM5: RobotArm5Measurement, MODEL_UNICYCLE: UnicycleMeasurement,
-             MODEL_DH6: DH6Measurement}
+             MODEL_DH6: DH6Measurement, MODEL_UNICYCLE10: Unicycle10Measurement}
 
 
 class EsthError(RuntimeError):
@@ -150,6 +161,10 @@ _SIGS = {
                               _P, _P], C.c_int),
     "esth_trace_load_robotarm": ([C.c_char_p, C.c_int, _P, _P, _P, _P, _P], C.c_int),
     "esth_robotarm5_estimate_error": ([_P, _P], C.c_float),
+    "esth_trace_load_unicycle10": ([C.c_char_p, C.c_int, _P, _P, _P, _P, _P], C.c_int),
+    "esth_unicycle10_filter": ([C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p, C.c_uint32, C.c_int,
+                                _P, _P], C.c_int),
+    "esth_unicycle10_params_default": ([_P], None),
     "esth_unicycle_scenario": ([C.c_uint32, C.c_int, C.c_float, C.c_int, C.c_float, _P, _P, _P, _P], C.c_int),
     "esth_dh6_scenario": ([C.c_uint32, C.c_int, C.c_float, _P, _P, _P], C.c_int),
     "esth_batch_create": ([_P, C.c_int, _P, _P, C.c_size_t, _P], C.c_int),
@@ -617,6 +632,34 @@ def load_robotarm_trace(path, max_rows=100000):
     check(lib().esth_trace_load_robotarm(path.encode(), max_rows, meas, ctrl, ptr(dt), ptr(truth), C.byref(rows)))
     n = rows.value
     return meas[:n], ctrl[:n], dt[:n], truth[: n * 9].reshape(n, 9)
+
+
+def load_unicycle10_trace(path, max_rows=100000):
+    """The reference unicycle's 16-column input: (measurements, controls, dt, truth rows x 3)."""
+    meas = (Unicycle10Measurement * max_rows)()
+    ctrl = (UnicycleControl * max_rows)()
+    dt = np.zeros(max_rows, np.float32)
+    truth = np.zeros(max_rows * 3, np.float32)
+    rows = C.c_int()
+    check(lib().esth_trace_load_unicycle10(path.encode(), max_rows, meas, ctrl, ptr(dt), ptr(truth), C.byref(rows)))
+    n = rows.value
+    return meas[:n], ctrl[:n], dt[:n], truth[: n * 3].reshape(n, 3)
+
+
+def unicycle10_params_default():
+    p = Unicycle10Params()
+    lib().esth_unicycle10_params_default(C.byref(p))
+    return p
+
+
+def unicycle10_filter(m, G, t, threshold, trace_path, seed=1234, resampler=RESAMPLE_MULTINOMIAL):
+    """The reference unicycle program's loop on the engine: (mean best-particle
+    distance to the truth, steps)."""
+    err = C.c_double()
+    steps = C.c_int()
+    check(lib().esth_unicycle10_filter(m, G, t, threshold, trace_path.encode(), seed, resampler, C.byref(err),
+                                       C.byref(steps)))
+    return err.value, steps.value
 
 
 def build_frame(model, params, control, measurement, dt):

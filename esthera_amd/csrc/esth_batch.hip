@@ -14,14 +14,76 @@
  * bit-identical to the same filter stepped alone with ESTH_RNG_MTGP_PER_FILTER.
  */
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <thread>
 
 #include "esth_internal.h"
 #include "esth_kernels.h"
 
 #define ESTH_BATCH_TIMING_STEPS 8192
+
+/* A few host threads that build the step's per-filter frames in parallel
+ * (10^4 filters take milliseconds on one core, as long as the device step).
+ * Every worker takes part in every generation, so no worker can still be in
+ * one generation's job when the next is posted. */
+struct esth_pool {
+    std::vector<std::thread> th;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::function<void(int)> job;
+    int njobs = 0, finished = 0, nth = 0;
+    std::atomic<int> next{0};
+    uint64_t gen = 0;
+    bool stop = false;
+    void work() {
+        for (int j = next.fetch_add(1); j < njobs; j = next.fetch_add(1)) job(j);
+    }
+    void start(int n) {
+        nth = n;
+        for (int i = 0; i < n; ++i)
+            th.emplace_back([this] {
+                uint64_t seen = 0;
+                for (;;) {
+                    {
+                        std::unique_lock<std::mutex> lk(mu);
+                        cv.wait(lk, [&] { return stop || gen != seen; });
+                        if (stop) return;
+                        seen = gen;
+                    }
+                    work();
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (++finished == nth) done_cv.notify_all();
+                }
+            });
+    }
+    void run(int n, std::function<void(int)> f) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            job = std::move(f);
+            njobs = n;
+            next = 0;
+            finished = 0;
+            ++gen;
+        }
+        cv.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu);
+        done_cv.wait(lk, [&] { return finished == nth; });
+    }
+    ~esth_pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto& t : th) t.join();
+    }
+};
 
 struct esth_bclass {
     int logm = 0;
@@ -71,6 +133,7 @@ struct esth_batch {
     /* step latency: an event pair per step */
     std::vector<hipEvent_t> tev;
     int tsteps = 0;
+    std::unique_ptr<esth_pool> pool;
 };
 
 static int balloc(esth_batch* b, size_t bytes, void** p) {
@@ -99,14 +162,7 @@ static void batch_free(esth_batch* b) {
     delete b;
 }
 
-static size_t frame_floats(int model, const void* params) {
-    if (model == ESTH_MODEL_UNICYCLE) {
-        const esth_unicycle_params* p = (const esth_unicycle_params*)params;
-        return (size_t)ESTH_UNI_HDR + 5u * (size_t)p->num_landmarks;
-    }
-    if (model == ESTH_MODEL_DH6) return (size_t)ESTH_DH6_HDR + 24u;
-    return 16u;
-}
+static size_t frame_floats(int model, const void* params) { return (size_t)esth_model_frame_floats(model, params); }
 
 extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* descs, const void* params,
                                  size_t params_stride, esth_batch** out) {
@@ -324,6 +380,9 @@ static int batch_kernels_s(esth_batch* b, int p) {
     case ESTH_MODEL_UNICYCLE:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
         break;
+    case ESTH_MODEL_UNICYCLE10:
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
+        break;
     default:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
         break;
@@ -385,19 +444,30 @@ extern "C" int esth_batch_step(esth_batch* b, const void* controls, size_t contr
     const int p = (int)(b->steps & 1);
     if (b->frame_used[p]) ESTH_HIP(hipEventSynchronize(b->frame_ev[p]));
     float* hf = b->h_frames[p];
-    float fr[ESTH_FRAME_FLOATS];
-    for (int i = 0; i < b->n; ++i) {
-        const void* c = controls ? (const unsigned char*)controls + (size_t)i * control_stride : nullptr;
-        const void* m = (const unsigned char*)measurements + (size_t)i * meas_stride;
-        const void* pp = b->params[i].empty() ? nullptr : b->params[i].data();
-        if (b->d.model == ESTH_MODEL_DH6 && !c) {
+    std::atomic<int> bad{-1};
+    auto build = [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            const void* c = controls ? (const unsigned char*)controls + (size_t)i * control_stride : nullptr;
+            const void* m = (const unsigned char*)measurements + (size_t)i * meas_stride;
+            const void* pp = b->params[i].empty() ? nullptr : b->params[i].data();
             static const esth_dh6_control none = {0.0f};
-            c = &none;
+            if (b->d.model == ESTH_MODEL_DH6 && !c) c = &none;
+            if (esth_build_frame_n(b->d.model, pp, c, m, dts[i], hf + (size_t)i * b->frame_stride, (int)b->frame_stride))
+                bad = i;
         }
-        if (esth_build_frame_impl(b->d.model, pp, c, m, dts[i], fr))
-            return esth_fail(ESTH_ERR_INVALID, "batch: bad control/measurement for filter %d", i);
-        memcpy(hf + (size_t)i * b->frame_stride, fr, sizeof(float) * b->frame_stride);
+    };
+    const int chunk = 256;
+    if (b->n >= 2 * chunk) {
+        if (!b->pool) {
+            b->pool.reset(new esth_pool());
+            const unsigned hc = std::thread::hardware_concurrency();
+            b->pool->start((int)std::min<unsigned>(7u, hc > 1 ? hc - 1 : 1u));
+        }
+        b->pool->run((b->n + chunk - 1) / chunk, [&](int j) { build(j * chunk, std::min(b->n, (j + 1) * chunk)); });
+    } else {
+        build(0, b->n);
     }
+    if (bad >= 0) return esth_fail(ESTH_ERR_INVALID, "batch: bad control/measurement for filter %d", bad.load());
     if (b->use_graph && !b->gexec[p]) ESTH_TRY(capture_batch(b, p));
     hipEvent_t* ev = nullptr;
     if (b->tsteps < ESTH_BATCH_TIMING_STEPS) {

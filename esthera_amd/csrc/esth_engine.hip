@@ -60,6 +60,7 @@ extern "C" int esth_model_lookup(const char* name) {
     if (!strcmp(name, "robotarm5") || !strcmp(name, "robotarm")) return ESTH_MODEL_ROBOTARM5;
     if (!strcmp(name, "unicycle")) return ESTH_MODEL_UNICYCLE;
     if (!strcmp(name, "dh6")) return ESTH_MODEL_DH6;
+    if (!strcmp(name, "unicycle10")) return ESTH_MODEL_UNICYCLE10;
     return -1;
 }
 
@@ -68,6 +69,7 @@ int esth_params_size(int model, size_t* sz) {
     case ESTH_MODEL_ROBOTARM5: *sz = 0; return ESTH_OK;
     case ESTH_MODEL_UNICYCLE: *sz = sizeof(esth_unicycle_params); return ESTH_OK;
     case ESTH_MODEL_DH6: *sz = sizeof(esth_dh6_params); return ESTH_OK;
+    case ESTH_MODEL_UNICYCLE10: *sz = sizeof(esth_unicycle10_params); return ESTH_OK;
     default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
     }
 }
@@ -81,12 +83,28 @@ extern "C" int esth_model_info(int model, int* S, int* V, size_t* ps, size_t* cs
     esth_params_size(model, &p);
     if (ps) *ps = p;
     if (cs) *cs = model == ESTH_MODEL_ROBOTARM5 ? sizeof(esth_robotarm5_control)
-                  : model == ESTH_MODEL_UNICYCLE ? sizeof(esth_unicycle_control)
-                                                 : sizeof(esth_dh6_control);
-    if (ms) *ms = model == ESTH_MODEL_ROBOTARM5 ? sizeof(esth_robotarm5_measurement)
+                  : model == ESTH_MODEL_DH6      ? sizeof(esth_dh6_control)
+                                                 : sizeof(esth_unicycle_control);
+    if (ms) *ms = model == ESTH_MODEL_ROBOTARM5  ? sizeof(esth_robotarm5_measurement)
                   : model == ESTH_MODEL_UNICYCLE ? sizeof(esth_unicycle_measurement)
-                                                 : sizeof(esth_dh6_measurement);
+                  : model == ESTH_MODEL_DH6      ? sizeof(esth_dh6_measurement)
+                                                 : sizeof(esth_unicycle10_measurement);
     return ESTH_OK;
+}
+
+/* particle_filter.cu:60-74 */
+extern "C" void esth_unicycle10_params_default(esth_unicycle10_params* p) {
+    if (!p) return;
+    static const float sx[10] = {12, 12, 13, 14, 16, 16, 18, 20, 20, 20};
+    static const float sy[10] = {12, 37, 25, 18, 12, 37, 30, 12, 25, 37};
+    for (int j = 0; j < 10; ++j) {
+        p->sx[j] = sx[j];
+        p->sy[j] = sy[j];
+        p->sz[j] = 3.0f;
+    }
+    p->noise_v = 0.5f;
+    p->noise_omega = 0.1f;
+    p->noise_gamma = 0.01f;
 }
 
 extern "C" int esth_model_build_frame(int model, const void* params, const void* control,
@@ -234,6 +252,9 @@ int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* 
     case ESTH_MODEL_DH6:
         hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_DH6>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
         break;
+    case ESTH_MODEL_UNICYCLE10:
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
+        break;
     default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
     }
     ESTH_HIP(hipGetLastError());
@@ -348,6 +369,37 @@ int esth_launch_resample(int resampler, int S, float* st_main, const float* st_t
     case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_DH6_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
     default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
     }
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+template <int LOGM, int S>
+static void launch_resample_argmax_t(float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
+                                     uint32_t G, float thr, int systematic, int log_w, const float2* winner,
+                                     const float* part, const float* best_src, size_t plane_stride, uint32_t g_stride,
+                                     float2* result, double* est, hipStream_t s) {
+    using TL = esth_tile<LOGM>;
+    const uint64_t waves = (G + (ESTH_WAVE / TL::LPS) - 1) / (ESTH_WAVE / TL::LPS);
+    hipLaunchKernelGGL((k_resample_argmax<LOGM, S>), dim3(1 + grid_for(waves * ESTH_WAVE, 256)), dim3(256), 0, s,
+                       st_main, st_tmp, w, rnd, N, G, thr, systematic, log_w, winner, part, best_src, plane_stride,
+                       g_stride, result, est);
+}
+
+int esth_launch_resample_argmax(int resampler, int S, float* st_main, const float* st_tmp, const float* w,
+                                const float* rnd, uint32_t N, uint32_t m, uint32_t G, float thr, int log_w,
+                                const float2* winner, const float* part, const float* best_src, size_t plane_stride,
+                                uint32_t g_stride, float2* result, double* est, hipStream_t s) {
+    if (resampler == ESTH_RESAMPLE_ALIAS) return esth_fail(ESTH_ERR_UNSUPPORTED, "fused argmax: scan resamplers only");
+    const int lm = log2u(m);
+    const int sys = resampler == ESTH_RESAMPLE_SYSTEMATIC;
+#define ESTH_RA_ARGS st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, winner, part, best_src, plane_stride, g_stride, result, est, s
+    switch (S) {
+    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_UNI_S, ESTH_RA_ARGS); break;
+    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_RA5_S, ESTH_RA_ARGS); break;
+    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_DH6_S, ESTH_RA_ARGS); break;
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+    }
+#undef ESTH_RA_ARGS
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }

@@ -242,16 +242,28 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
     ESTH_TRY(esth_launch_sort(f->S, f->m, f->w, f->state, f->tmp, N, f->G, f->winner, nullptr, f->t, f->bt_w,
                               f->bt_s, f->part, log_w, s));
     if (all) ESTH_HIP(hipEventRecord(ev[3], s));
-    /* reduce (pf_generic.c:604-615), result kept on the device */
-    ESTH_TRY(esth_launch_argmax(f->S, f->winner, f->G, log_w, f->part, f->tmp, N, f->m, f->result, f->est, s));
+    /* reduce (pf_generic.c:604-615), result kept on the device.  With a scan
+     * resampler it rides in the resampling launch (below); its best particle
+     * is rank 0 of the winning sub-filter, read from the sort's compact best-t
+     * copy, which the exchange does not touch. */
+    const bool fused = d.resampler != ESTH_RESAMPLE_ALIAS;
+    if (!fused) ESTH_TRY(esth_launch_argmax(f->S, f->winner, f->G, log_w, f->part, f->tmp, N, f->m, f->result, f->est, s));
     if (all) ESTH_HIP(hipEventRecord(ev[4], s));
     /* exchange (pf_generic.c:702-715) */
     ESTH_TRY(esth_launch_exchange(f->tmp, f->w, f->S, N, f->m, f->G, f->t, d.topology, f->dimx, f->dimy, f->bt_w,
                                   f->bt_s, f->top, s));
     if (all) ESTH_HIP(hipEventRecord(ev[5], s));
     /* resampling (pf_generic.c:732-757), uniforms at rand_offset = N*V */
-    ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m, f->G,
-                                  d.resampling_threshold, d.adaptive, log_w, nullptr, s));
+    if (fused) {
+        const bool bt = f->t > 0 && d.topology != ESTH_TOPO_NONE;
+        ESTH_TRY(esth_launch_resample_argmax(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m,
+                                             f->G, d.resampling_threshold, log_w, f->winner, f->part,
+                                             bt ? f->bt_s : f->tmp, bt ? (size_t)f->G * f->t : (size_t)N,
+                                             bt ? f->t : f->m, f->result, f->est, s));
+    } else {
+        ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m, f->G,
+                                      d.resampling_threshold, d.adaptive, log_w, nullptr, s));
+    }
     if (all) ESTH_HIP(hipEventRecord(ev[6], s));
     return ESTH_OK;
 }
@@ -492,6 +504,8 @@ extern "C" float esth_robotarm5_estimate_error(const float* est, const float* ac
     return error;
 }
 
+static int count_lines(const char* path, int* lines);
+
 /* particle_filter() for the robot arm over a trace (pf_generic.c:245-887) */
 extern "C" int esth_particle_filter(uint32_t m, uint32_t G, uint32_t t, float thr, const char* trace, uint32_t seed,
                                     int resampler, double* mean_error, int* steps) {
@@ -500,14 +514,8 @@ extern "C" int esth_particle_filter(uint32_t m, uint32_t G, uint32_t t, float th
     std::vector<esth_robotarm5_control> ctrl;
     std::vector<float> dt, truth;
     int rows = 0;
-    {
-        FILE* fh = fopen(trace, "r");
-        if (!fh) return esth_fail(ESTH_ERR_IO, "could not open %s", trace);
-        int lines = 0, ch;
-        while ((ch = fgetc(fh)) != EOF) lines += ch == '\n';
-        fclose(fh);
-        rows = std::min(lines + 1, max_rows);
-    }
+    ESTH_TRY(count_lines(trace, &rows));
+    rows = std::min(rows, max_rows);
     meas.resize(rows); ctrl.resize(rows); dt.resize(rows); truth.resize((size_t)rows * 9);
     ESTH_TRY(esth_trace_load_robotarm(trace, rows, meas.data(), ctrl.data(), dt.data(), truth.data(), &rows));
     esth_engine* e = nullptr;
@@ -533,6 +541,83 @@ extern "C" int esth_particle_filter(uint32_t m, uint32_t G, uint32_t t, float th
             for (int c = 0; c < 9; ++c) best[c] = est.best[c];
             best[7] = 0.0f; best[8] = 0.0f;
             err_sum += esth_robotarm5_estimate_error(best, &truth[(size_t)i * 9]);
+        }
+    }
+    esth_filter_destroy(f);
+    esth_engine_destroy(e);
+    if (rc != ESTH_OK) return rc;
+    if (mean_error) *mean_error = rows ? err_sum / rows : 0.0;
+    if (steps) *steps = rows;
+    return ESTH_OK;
+}
+
+/* the reference unicycle's input rows (particle_filter.cu:756-772): 16 floats */
+extern "C" int esth_trace_load_unicycle10(const char* path, int max_rows, esth_unicycle10_measurement* meas,
+                                          esth_unicycle_control* ctrl, float* dt, float* truth, int* rows) {
+    if (!path || !rows) return esth_fail(ESTH_ERR_INVALID, "null");
+    FILE* fh = fopen(path, "r");
+    if (!fh) return esth_fail(ESTH_ERR_IO, "could not open %s", path);
+    int n = 0;
+    while (n < max_rows) {
+        float v[16];
+        int k = 0;
+        for (; k < 16; ++k)
+            if (fscanf(fh, "%f", &v[k]) != 1) break;
+        if (k < 16) break;   /* the reference loop runs while fscanf reads all 16 */
+        if (meas) for (int i = 0; i < 10; ++i) meas[n].range[i] = v[i];
+        if (ctrl) { ctrl[n].v = v[10]; ctrl[n].omega = v[11]; }
+        if (dt) dt[n] = v[12];
+        if (truth) { truth[(size_t)n * 3] = v[13]; truth[(size_t)n * 3 + 1] = v[14]; truth[(size_t)n * 3 + 2] = v[15]; }
+        ++n;
+    }
+    fclose(fh);
+    *rows = n;
+    return ESTH_OK;
+}
+
+static int count_lines(const char* path, int* lines) {
+    FILE* fh = fopen(path, "r");
+    if (!fh) return esth_fail(ESTH_ERR_IO, "could not open %s", path);
+    int n = 0, ch;
+    while ((ch = fgetc(fh)) != EOF) n += ch == '\n';
+    fclose(fh);
+    *lines = n + 1;
+    return ESTH_OK;
+}
+
+/* particle_filter.cu:729-1157 with the engine's step: one filter over the
+ * trace, error = |best particle - truth| in the plane, averaged over steps */
+extern "C" int esth_unicycle10_filter(uint32_t m, uint32_t G, uint32_t t, float thr, const char* trace, uint32_t seed,
+                                      int resampler, double* mean_error, int* steps) {
+    if (!trace) return esth_fail(ESTH_ERR_INVALID, "null trace");
+    int rows = 0;
+    ESTH_TRY(count_lines(trace, &rows));
+    rows = std::min(rows, 1 << 20);
+    std::vector<esth_unicycle10_measurement> meas(rows);
+    std::vector<esth_unicycle_control> ctrl(rows);
+    std::vector<float> dt(rows), truth((size_t)rows * 3);
+    ESTH_TRY(esth_trace_load_unicycle10(trace, rows, meas.data(), ctrl.data(), dt.data(), truth.data(), &rows));
+    esth_unicycle10_params prm;
+    esth_unicycle10_params_default(&prm);
+    esth_engine* e = nullptr;
+    ESTH_TRY(esth_engine_create(0, 0, &e));
+    esth_filter_desc d;
+    esth_filter_desc_default(&d, ESTH_MODEL_UNICYCLE10);
+    d.num_particles = m; d.num_blocks = G; d.num_transfer = t; d.resampling_threshold = thr;
+    d.resampler = resampler; d.seed = seed; d.topology = t ? ESTH_TOPO_RING : ESTH_TOPO_NONE;
+    esth_filter* f = nullptr;
+    int rc = esth_filter_create(e, &d, &prm, sizeof(prm), &f);
+    if (rc != ESTH_OK) { esth_engine_destroy(e); return rc; }
+    const float init[3] = {20.0f, 40.0f, (float)(-M_PI / 2)};
+    rc = esth_filter_init_uniform(f, init);
+    double err_sum = 0.0;
+    for (int i = 0; rc == ESTH_OK && i < rows; ++i) {
+        rc = esth_filter_step(f, &ctrl[i], &meas[i], dt[i]);
+        esth_estimate est;
+        if (rc == ESTH_OK) rc = esth_filter_estimate(f, &est);
+        if (rc == ESTH_OK) {
+            const double ex = (double)est.best[0] - truth[(size_t)i * 3], ey = (double)est.best[1] - truth[(size_t)i * 3 + 1];
+            err_sum += std::sqrt(ex * ex + ey * ey);
         }
     }
     esth_filter_destroy(f);

@@ -171,6 +171,11 @@ struct esth_model_traits<ESTH_MODEL_DH6> {
     static constexpr int S = ESTH_DH6_S, V = ESTH_DH6_V;
     __device__ static float iw(float* s, const float* z, const float* fr) { return esth_dh6_sampling_iw(s, z, fr); }
 };
+template <>
+struct esth_model_traits<ESTH_MODEL_UNICYCLE10> {
+    static constexpr int S = ESTH_U10_S, V = ESTH_U10_V;
+    __device__ static float iw(float* s, const float* z, const float* fr) { return esth_unicycle10_sampling_iw(s, z, fr); }
+};
 
 template <int MODEL>
 __device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ state, uint32_t N,
@@ -311,9 +316,9 @@ __device__ __forceinline__ void esth_bitonic(float (&key)[esth_tile<LOGM>::E],
 template <int S, int NT>
 __device__ __forceinline__ void esth_argmax_block(const float2* __restrict__ winner, uint32_t G, int log_w,
                                                   const float* __restrict__ part,
-                                                  const float* __restrict__ sorted, uint32_t N, uint32_t m,
-                                                  float2* __restrict__ result, double* __restrict__ est,
-                                                  float2* sc, float* smax, double* sd) {
+                                                  const float* __restrict__ best_src, size_t plane_stride,
+                                                  uint32_t g_stride, float2* __restrict__ result,
+                                                  double* __restrict__ est, float2* sc, float* smax, double* sd) {
     constexpr int NW = NT / 64;
     const int l = threadIdx.x;
     const int lane = l & 63, wid = l >> 6;
@@ -386,9 +391,9 @@ __device__ __forceinline__ void esth_argmax_block(const float2* __restrict__ win
         est[2 + S] = (double)sc[0].x;
         est[3 + S] = (double)sc[0].y;
     }
-    if (sorted && l < S) {
+    if (best_src && l < S) {   /* rank 0 of the winning sub-filter, component l */
         int gb = (int)sc[0].y;
-        if (gb >= 0 && (uint32_t)gb < G) est[4 + S + l] = (double)sorted[(size_t)l * N + (size_t)gb * m];
+        if (gb >= 0 && (uint32_t)gb < G) est[4 + S + l] = (double)best_src[(size_t)l * plane_stride + (size_t)gb * g_stride];
     }
 }
 
@@ -693,6 +698,41 @@ __global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
     const bool valid = g < G;
     if (!valid) g = G - 1;   /* keep the lane busy; its writes are dropped */
     esth_resample_sub<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, sel,
+                               cdf_s[threadIdx.x / ESTH_WAVE]);
+}
+
+/* The step's argmax + posterior (work-group 0) beside the scan resampling
+ * (work-groups 1..): the two read disjoint data (the argmax: winners,
+ * partials and the compact best-t copy of rank 0; the resampling: weights,
+ * uniforms and the exchanged sorted state) and neither writes what the other
+ * reads, so one launch does both and the argmax's single work-group runs in
+ * the shadow of the resampling instead of on its own before the exchange. */
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_main,
+                                                         const float* __restrict__ st_tmp,
+                                                         const float* __restrict__ w,
+                                                         const float* __restrict__ rnd, uint32_t N, uint32_t G,
+                                                         float thr, int systematic, int log_w,
+                                                         const float2* __restrict__ winner,
+                                                         const float* __restrict__ part,
+                                                         const float* __restrict__ best_src, size_t plane_stride,
+                                                         uint32_t g_stride, float2* __restrict__ result,
+                                                         double* __restrict__ est) {
+    using TL = esth_tile<LOGM>;
+    __shared__ float cdf_s[4][ESTH_WAVE * TL::E];
+    __shared__ float2 sc[128];
+    __shared__ float smax[4];
+    __shared__ double sd[4 * (2 + S)];
+    if (blockIdx.x == 0) {
+        esth_argmax_block<S, 256>(winner, G, log_w, part, best_src, plane_stride, g_stride, result, est, sc, smax, sd);
+        return;
+    }
+    const int lane = threadIdx.x & (ESTH_WAVE - 1);
+    const uint32_t wave = ((blockIdx.x - 1) * blockDim.x + threadIdx.x) / ESTH_WAVE;
+    uint32_t g = wave * (ESTH_WAVE / TL::LPS) + (uint32_t)(lane / TL::LPS);
+    const bool valid = g < G;
+    if (!valid) g = G - 1;
+    esth_resample_sub<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, nullptr,
                                cdf_s[threadIdx.x / ESTH_WAVE]);
 }
 

@@ -187,19 +187,79 @@ ESTH_HD float esth_dh6_sampling_iw(float* s, const float* z, const float* fr) {
     return -esth_fma(acc_p, fr[3], acc_q * fr[4]);
 }
 
+/* ------------------------------------------------------------ unicycle10
+ * The reference unicycle (src/models/unicycle/particle_filter.cu): sampling()
+ * (:313-333) moves the particle on the arc of the noisy speed and turn rate,
+ *   v = v_c + nv z0, w = w_c + nw z1, gamma = ng z2,
+ *   x += v/w (sin(th + w dt) - sin th), y -= v/w (cos(th + w dt) - cos th),
+ *   th += w dt + gamma dt,
+ * and importance_weight() (:339-359) compares the distances to 10 sensors at
+ * height sz with the measured ranges: weight = norm exp(-sum (d - range)^2),
+ * norm = 1e5 / (2 pi)^5, here returned as its logarithm.  The arc is written
+ * in its half-angle form, h = w dt / 2:
+ *   x += v dt (sin h / h) cos(th + h),  y += v dt (sin h / h) sin(th + h),
+ * the same displacement without the cancellation of the two sines (whose
+ * difference loses all digits as w -> 0, and is 0/0 at w = 0); sin h / h is
+ * 1 - h^2/6 below |h| = 1e-3. */
+#define ESTH_U10_S 3
+#define ESTH_U10_V 3
+/* frame: [0] dt, [1] v, [2] w, [3] nv, [4] nw, [5] ng, [6] log norm,
+ * [16..26) ranges, then at 26+3j: sensor x, y, z */
+#define ESTH_U10_HDR 16
+#define ESTH_U10_FLOATS (ESTH_U10_HDR + 10 + 30)
+#define ESTH_U10_LOG_NORM 2.3235402f   /* log(1e5) - 5 log(2 pi) */
+
+ESTH_HD float esth_unicycle10_sampling_iw(float* s, const float* z, const float* fr) {
+    const float dt = fr[0];
+    const float v = fr[1] + fr[3] * z[0];
+    const float w = fr[2] + fr[4] * z[1];
+    const float g = fr[5] * z[2];
+    const float th = s[2];
+    const float wdt = w * dt;
+    const float h = 0.5f * wdt;
+    float sh, ch, sm, cm;
+    esth_sincosf(h, &sh, &ch);
+    esth_sincosf(th + h, &sm, &cm);
+    const float sinc = fabsf(h) < 1e-3f ? 1.0f - (h * h) * (1.0f / 6.0f) : sh / h;
+    const float chord = (v * dt) * sinc;
+    const float x = s[0] + chord * cm;
+    const float y = s[1] + chord * sm;
+    s[0] = x;
+    s[1] = y;
+    s[2] = (th + wdt) + g * dt;
+    float value = 0.0f;
+    for (int j = 0; j < 10; ++j) {
+        const float* sp = fr + ESTH_U10_HDR + 10 + 3 * j;
+        const float d1 = x - sp[0];
+        const float d2 = y - sp[1];
+        const float d3 = sp[2];
+        const float e = sqrtf(d1 * d1 + d2 * d2 + d3 * d3) - fr[ESTH_U10_HDR + j];
+        value += e * e;
+    }
+    return fr[6] - value;
+}
+
 /* ---------------------------------------------------------- frame build */
 ESTH_HD int esth_model_dims(int model, int* S, int* V) {
     switch (model) {
     case ESTH_MODEL_ROBOTARM5: *S = ESTH_RA5_S; *V = ESTH_RA5_V; return 0;
     case ESTH_MODEL_UNICYCLE:  *S = ESTH_UNI_S; *V = ESTH_UNI_V; return 0;
     case ESTH_MODEL_DH6:       *S = ESTH_DH6_S; *V = ESTH_DH6_V; return 0;
+    case ESTH_MODEL_UNICYCLE10: *S = ESTH_U10_S; *V = ESTH_U10_V; return 0;
     default: return -1;
     }
 }
 
-ESTH_HD int esth_build_frame_impl(int model, const void* params, const void* control,
-                                  const void* measurement, float dt, float* fr) {
-    for (int i = 0; i < ESTH_FRAME_FLOATS; ++i) fr[i] = 0.0f;
+/* the frame's first `nfloats` floats (at least the model's own length) */
+ESTH_HD int esth_build_frame_n(int model, const void* params, const void* control, const void* measurement,
+                               float dt, float* fr, int nfloats) {
+    for (int i = 0; i < nfloats; ++i) fr[i] = 0.0f;
+    if (model == ESTH_MODEL_UNICYCLE && params &&
+        ESTH_UNI_HDR + 5 * ((const esth_unicycle_params*)params)->num_landmarks > nfloats)
+        return -1;
+    if (model == ESTH_MODEL_DH6 && ESTH_DH6_HDR + 24 > nfloats) return -1;
+    if (model == ESTH_MODEL_UNICYCLE10 && ESTH_U10_FLOATS > nfloats) return -1;
+    if (nfloats < 16) return -1;
     fr[0] = dt;
     if (model == ESTH_MODEL_ROBOTARM5) {
         const esth_robotarm5_control* c = (const esth_robotarm5_control*)control;
@@ -237,6 +297,26 @@ ESTH_HD int esth_build_frame_impl(int model, const void* params, const void* con
         }
         return 0;
     }
+    if (model == ESTH_MODEL_UNICYCLE10) {
+        const esth_unicycle10_params* p = (const esth_unicycle10_params*)params;
+        const esth_unicycle_control* c = (const esth_unicycle_control*)control;
+        const esth_unicycle10_measurement* m = (const esth_unicycle10_measurement*)measurement;
+        if (!p || !c || !m) return -1;
+        fr[1] = c->v;
+        fr[2] = c->omega;
+        fr[3] = p->noise_v;
+        fr[4] = p->noise_omega;
+        fr[5] = p->noise_gamma;
+        fr[6] = ESTH_U10_LOG_NORM;
+        for (int j = 0; j < 10; ++j) {
+            fr[ESTH_U10_HDR + j] = m->range[j];
+            float* sp = fr + ESTH_U10_HDR + 10 + 3 * j;
+            sp[0] = p->sx[j];
+            sp[1] = p->sy[j];
+            sp[2] = p->sz[j];
+        }
+        return 0;
+    }
     if (model == ESTH_MODEL_DH6) {
         const esth_dh6_params* p = (const esth_dh6_params*)params;
         const esth_dh6_measurement* m = (const esth_dh6_measurement*)measurement;
@@ -260,6 +340,20 @@ ESTH_HD int esth_build_frame_impl(int model, const void* params, const void* con
         return 0;
     }
     return -1;
+}
+
+/* floats of a model's frame that its sampling_iw reads */
+ESTH_HD int esth_model_frame_floats(int model, const void* params) {
+    if (model == ESTH_MODEL_UNICYCLE)
+        return params ? ESTH_UNI_HDR + 5 * ((const esth_unicycle_params*)params)->num_landmarks : ESTH_FRAME_FLOATS;
+    if (model == ESTH_MODEL_DH6) return ESTH_DH6_HDR + 24;
+    if (model == ESTH_MODEL_UNICYCLE10) return ESTH_U10_FLOATS;
+    return 16;
+}
+
+ESTH_HD int esth_build_frame_impl(int model, const void* params, const void* control,
+                                  const void* measurement, float dt, float* fr) {
+    return esth_build_frame_n(model, params, control, measurement, dt, fr, ESTH_FRAME_FLOATS);
 }
 
 /* Box-Muller of one uniform pair in [1,2) (pf_generic.cl:465-483):

@@ -323,11 +323,7 @@ static uint64_t track_layout(esth_strack* t, uint64_t off, bool assign, unsigned
     return p;
 }
 
-static size_t srv_frame_floats(int model, const void* params) {
-    if (model == ESTH_MODEL_UNICYCLE) return (size_t)ESTH_UNI_HDR + 5u * (size_t)((const esth_unicycle_params*)params)->num_landmarks;
-    if (model == ESTH_MODEL_DH6) return (size_t)ESTH_DH6_HDR + 24u;
-    return 16u;
-}
+static size_t srv_frame_floats(int model, const void* params) { return (size_t)esth_model_frame_floats(model, params); }
 
 static void push_result(esth_server* s, uint64_t req, uint64_t track, int status, int kind, double submit,
                         const esth_estimate* est) {
@@ -595,6 +591,9 @@ static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wa
         break;
     case ESTH_MODEL_UNICYCLE:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, st, Fg, sa, nsub, R, frames, log_w);
+        break;
+    case ESTH_MODEL_UNICYCLE10:
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, st, Fg, sa, nsub, R, frames, log_w);
         break;
     default:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, st, Fg, sa, nsub, R, frames, log_w);

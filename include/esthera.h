@@ -67,11 +67,16 @@ int esth_version(void);
  *              3 normals per particle.
  *  dh6       : 6-joint Denavit-Hartenberg arm, constant-velocity dynamics;
  *              state q0..q5, qd0..qd5 (12); 6 normals per particle.
+ *  unicycle10: the reference's unicycle (src/models/unicycle/particle_filter.cu:
+ *              60-74,313-362): velocity motion model with noisy speed, turn
+ *              rate and heading drift, 10 fixed range sensors above the
+ *              plane; state x, y, theta (3); 3 normals per particle.
  */
-#define ESTH_MODEL_ROBOTARM5 0
-#define ESTH_MODEL_UNICYCLE  1
-#define ESTH_MODEL_DH6       2
-#define ESTH_NUM_MODELS      3
+#define ESTH_MODEL_ROBOTARM5  0
+#define ESTH_MODEL_UNICYCLE   1
+#define ESTH_MODEL_DH6        2
+#define ESTH_MODEL_UNICYCLE10 3
+#define ESTH_NUM_MODELS       4
 
 /* robotarm5 (pf_model.h:34-56) */
 typedef struct { float angles[5]; } esth_robotarm5_control;
@@ -101,7 +106,17 @@ typedef struct {
 typedef struct { float unused; } esth_dh6_control;
 typedef struct { float p[3]; float q[6]; } esth_dh6_measurement;
 
-/* Model id for "robotarm5" | "unicycle" | "dh6"; -1 if unknown. */
+/* unicycle10: control is esth_unicycle_control (velocity, angular velocity) */
+#define ESTH_UNICYCLE10_SENSORS 10
+typedef struct {
+    float sx[ESTH_UNICYCLE10_SENSORS], sy[ESTH_UNICYCLE10_SENSORS], sz[ESTH_UNICYCLE10_SENSORS];
+    float noise_v, noise_omega, noise_gamma;
+} esth_unicycle10_params;
+typedef struct { float range[ESTH_UNICYCLE10_SENSORS]; } esth_unicycle10_measurement;
+/* the reference's sensor positions and noise constants (particle_filter.cu:60-74) */
+void esth_unicycle10_params_default(esth_unicycle10_params* p);
+
+/* Model id for "robotarm5" | "unicycle" | "dh6" | "unicycle10"; -1 if unknown. */
 int esth_model_lookup(const char* name);
 /* Sizes for a model: state floats per particle, normals per particle, and
  * the byte sizes of its params / control / measurement structs. */
@@ -228,6 +243,17 @@ int esth_trace_load_robotarm(const char* path, int max_rows, esth_robotarm5_meas
                              esth_robotarm5_control* ctrl, float* dt, float* truth, int* rows);
 /* estimate_error (pf_model_host.h:56-69) on robotarm5 states (9 floats). */
 float esth_robotarm5_estimate_error(const float* estimate, const float* actual);
+
+/* Trace rows of the reference unicycle format (16 columns,
+ * particle_filter.cu:756-772): 10 sensor ranges, velocity, angular velocity,
+ * dt, truth x, y, theta.  truth receives 3 floats per row. */
+int esth_trace_load_unicycle10(const char* path, int max_rows, esth_unicycle10_measurement* meas,
+                               esth_unicycle_control* ctrl, float* dt, float* truth, int* rows);
+/* The reference unicycle program's loop (particle_filter.cu:729-1157) on the
+ * engine: particles start at (20, 40, -pi/2) (:655-663); writes the mean over
+ * steps of the distance between the best particle and the truth (:1044-1057). */
+int esth_unicycle10_filter(uint32_t m, uint32_t G, uint32_t t, float thr, const char* trace_path,
+                           uint32_t seed, int resampler, double* mean_error, int* steps);
 
 /* Seeded synthetic scenarios.
  * unicycle: dt, v=1, omega_t = 0.2 sin(0.05 t); landmarks at (+-10,+-10) when

@@ -127,6 +127,7 @@ int or_sample_weight(int model, float* state, uint32_t N, const float* normals, 
         float lw;
         if (model == ESTH_MODEL_ROBOTARM5) lw = esth_robotarm5_sampling_iw(s, z, frame);
         else if (model == ESTH_MODEL_UNICYCLE) lw = esth_unicycle_sampling_iw(s, z, frame);
+        else if (model == ESTH_MODEL_UNICYCLE10) lw = esth_unicycle10_sampling_iw(s, z, frame);
         else lw = esth_dh6_sampling_iw(s, z, frame);
         for (int c = 0; c < S; ++c) state[(size_t)c * N + i] = s[c];
         w[i] = (weights == ESTH_WEIGHTS_LOG) ? lw : esth_expf(lw);
@@ -195,6 +196,31 @@ static double uni_f64(double* s, const double* z, const esth_unicycle_params* p,
     return -0.5 * acc;
 }
 
+/* unicycle10: particle_filter.cu:313-359 in double, the arc as the reference
+ * writes it (straight line only where w dt is below double resolution) */
+static double u10_f64(double* s, const double* z, const esth_unicycle10_params* p,
+                      const esth_unicycle_control* c, const esth_unicycle10_measurement* m, double dt) {
+    double v = (double)c->v + (double)p->noise_v * z[0];
+    double w = (double)c->omega + (double)p->noise_omega * z[1];
+    double g = (double)p->noise_gamma * z[2];
+    double th = s[2];
+    if (fabs(w * dt) >= 1e-9) {
+        s[0] += v / w * (sin(th + w * dt) - sin(th));
+        s[1] -= v / w * (cos(th + w * dt) - cos(th));
+    } else {
+        s[0] += v * dt * cos(th);
+        s[1] += v * dt * sin(th);
+    }
+    s[2] = th + w * dt + g * dt;
+    double value = 0.0;
+    for (int j = 0; j < 10; ++j) {
+        double d1 = s[0] - p->sx[j], d2 = s[1] - p->sy[j], d3 = p->sz[j];
+        double e = sqrt(d1 * d1 + d2 * d2 + d3 * d3) - (double)m->range[j];
+        value += e * e;
+    }
+    return log(1e5) - 5.0 * log(2.0 * M_PI) - value;
+}
+
 static double dh6_f64(double* s, const double* z, const esth_dh6_params* p,
                       const esth_dh6_measurement* m, double dt) {
     for (int i = 0; i < 6; ++i) {
@@ -244,6 +270,9 @@ int or_sample_weight_f64(int model, const float* state_in, uint32_t N, const flo
             lw = uni_f64(s, z, (const esth_unicycle_params*)params,
                          (const esth_unicycle_control*)ctrl,
                          (const esth_unicycle_measurement*)meas, dt);
+        else if (model == ESTH_MODEL_UNICYCLE10)
+            lw = u10_f64(s, z, (const esth_unicycle10_params*)params, (const esth_unicycle_control*)ctrl,
+                         (const esth_unicycle10_measurement*)meas, dt);
         else
             lw = dh6_f64(s, z, (const esth_dh6_params*)params, (const esth_dh6_measurement*)meas, dt);
         for (int c = 0; c < S; ++c) state_out[(size_t)c * N + i] = s[c];

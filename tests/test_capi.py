@@ -33,7 +33,8 @@ def test_models_and_defaults():
     assert L.esth_model_lookup(b"robotarm5") == 0 and L.esth_model_lookup(b"unicycle") == 1
     assert L.esth_model_lookup(b"dh6") == 2 and L.esth_model_lookup(b"nope") == -1
     S, V, ps, cs, ms = C.c_int(), C.c_int(), C.c_size_t(), C.c_size_t(), C.c_size_t()
-    for model, (s, v) in {0: (9, 9), 1: (3, 3), 2: (12, 6)}.items():
+    assert L.esth_model_lookup(b"unicycle10") == 3
+    for model, (s, v) in {0: (9, 9), 1: (3, 3), 2: (12, 6), 3: (3, 3)}.items():
         ea.check(L.esth_model_info(model, C.byref(S), C.byref(V), C.byref(ps), C.byref(cs), C.byref(ms)))
         assert (S.value, V.value) == (s, v)
         assert ms.value == C.sizeof(ea.MEAS_TYPE[model])
@@ -54,6 +55,25 @@ def test_frames_match_oracle():
         bad = ea.UnicycleParams()
         bad.num_landmarks = 70
         ea.build_frame(1, bad, ctrl[0], meas[0], 0.1)
+
+
+def test_unicycle10_trace_params_and_frame():
+    path = os.path.join(REPO, "tests", "golden", "unicycle_input.txt")
+    meas, ctrl, dt, truth = ea.load_unicycle10_trace(path)
+    rows = [ln.split() for ln in open(path) if len(ln.split()) == 16]
+    assert len(meas) == len(rows) == 131
+    first = [float(x) for x in rows[0]]
+    assert np.allclose(list(meas[0].range), first[:10])
+    assert np.isclose(ctrl[0].v, first[10]) and np.isclose(ctrl[0].omega, first[11]) and np.isclose(dt[0], first[12])
+    assert np.allclose(truth[0], first[13:16])
+    p = ea.unicycle10_params_default()
+    assert list(p.sx) == [12, 12, 13, 14, 16, 16, 18, 20, 20, 20]       # particle_filter.cu:69-74
+    assert list(p.sy) == [12, 37, 25, 18, 12, 37, 30, 12, 25, 37]
+    assert list(p.sz) == [3.0] * 10
+    assert np.isclose(p.noise_v, 0.5) and np.isclose(p.noise_omega, 0.1) and np.isclose(p.noise_gamma, 0.01)
+    fr = ea.build_frame(3, p, ctrl[5], meas[5], float(dt[5]))
+    assert np.array_equal(fr.view(np.uint32), O.build_frame(3, p, ctrl[5], meas[5], float(dt[5])).view(np.uint32))
+    assert np.isclose(fr[6], np.log(1e5) - 5 * np.log(2 * np.pi))
 
 
 def test_mtgp_table_and_seeding(golden, mtgp_table):

@@ -95,3 +95,34 @@ def test_autotune_measures_all_sizes(engine):
     assert us.shape == (5,) and np.all(us > 0) and np.all(np.isfinite(us))
     m = ea.autotune_pick(us, 65536, 256)
     assert m in (256, 512, 1024)
+
+
+def test_many_filters_parallel_frames(engine):
+    """Enough filters that the step's frames are built by several host threads."""
+    nf = 700
+    rng = np.random.default_rng(8)
+    scen = [ea.unicycle_scenario(500 + j, 6, 15.0, 4) for j in range(7)]
+    descs, params = [], []
+    for i in range(nf):
+        G = int(rng.integers(2, 5))
+        descs.append(ea.filter_desc(ea.MODEL_UNICYCLE, m=64, G=G, t=1, topology=ea.TOPO_RING,
+                                    resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG,
+                                    rng=ea.RNG_MTGP_PER_FILTER, seed=7000 + i))
+        params.append(scen[i % 7][0])
+    b = ea.Batch(engine, descs, params)
+    b.init_uniform(np.zeros((nf, 3), np.float32))
+    for k in range(3):
+        ctrls = (ea.UnicycleControl * nf)(*[scen[i % 7][1][k] for i in range(nf)])
+        meas = (ea.UnicycleMeasurement * nf)(*[scen[i % 7][2][k] for i in range(nf)])
+        b.step(ctrls, meas, np.full(nf, 0.1, np.float32))
+    for i in (0, 255, 256, 511, 699):
+        f = ea.Filter(engine, descs[i], params[i])
+        f.init_uniform(np.zeros(3, np.float32))
+        for k in range(3):
+            f.step(scen[i % 7][1][k], scen[i % 7][2][k], 0.1)
+        sa, wa = f.particles()
+        sb, wb = b.particles(i)
+        assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32)), f"filter {i}"
+        assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32)), f"filter {i}"
+        f.close()
+    b.close()

@@ -23,6 +23,10 @@ def make_case(model, steps):
         p, meas, truth = ea.dh6_scenario(7, steps)
         ctrl = [ea.DH6Control() for _ in range(steps)]
         return p, ctrl, meas, truth[0].copy(), truth[1:], 0.01
+    if model == ea.MODEL_UNICYCLE10:
+        meas, ctrl, dts, truth = ea.load_unicycle10_trace(os.path.join(HERE, "golden", "unicycle_input.txt"))
+        init = np.array([20.0, 40.0, -np.pi / 2], np.float32)
+        return ea.unicycle10_params_default(), ctrl, meas, init, truth, dts
     meas, ctrl, dts, truth = ea.load_robotarm_trace(os.path.join(HERE, "golden", "robotarm_input5.txt"))
     a0 = np.float32(0.75 * np.pi)
     init = np.array([a0] * 5 + [7.0, 0.0, 0.0, 0.0], np.float32)
@@ -69,6 +73,9 @@ CASES = [
     ("robotarm-alias-adaptive", dict(model=ea.MODEL_ROBOTARM5, m=256, G=8, t=1, resampler=ea.RESAMPLE_ALIAS,
                                      adaptive=1, threshold=0.01)),
     ("dh6-torus-log", dict(model=ea.MODEL_DH6, m=512, G=8, t=1, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG)),
+    ("unicycle10-reference-linear", dict(model=ea.MODEL_UNICYCLE10, m=128, G=16, t=1)),
+    ("unicycle10-alias-log", dict(model=ea.MODEL_UNICYCLE10, m=256, G=8, t=2, resampler=ea.RESAMPLE_ALIAS,
+                                  weights=ea.WEIGHTS_LOG)),
     ("unicycle-small-m", dict(model=ea.MODEL_UNICYCLE, m=16, G=12, t=2, weights=ea.WEIGHTS_LOG,
                               rng=ea.RNG_MTGP_PER_FILTER)),
 ]
@@ -122,6 +129,14 @@ def test_reference_driver_robotarm_input5(engine):
     err, steps = ea.particle_filter(64, 16, 1, 1.0, os.path.join(HERE, "golden", "robotarm_input5.txt"))
     assert steps == 100
     assert np.isfinite(err) and err > 0
+
+
+def test_reference_unicycle_driver(engine):
+    """The reference unicycle program's loop over its own input file: the best
+    particle follows the recorded truth."""
+    err, steps = ea.unicycle10_filter(256, 16, 1, 1.0, os.path.join(HERE, "golden", "unicycle_input.txt"))
+    assert steps == 131
+    assert np.isfinite(err) and err < 1.5, err
 
 
 def test_unicycle_tracks_truth(engine):

@@ -136,6 +136,33 @@ def test_polar_abs_error_bound():
     assert r.value == 0.0 and a.value == 0.0
 
 
+def test_unicycle10_model_against_fp64_twin():
+    """The FP32 hook (particle_filter.cu:313-359 restated) against the
+    double-precision twin written from the same definition with libm, on the
+    reference's own input rows; and the straight-line limit at w = 0."""
+    import os
+    import esthera_amd as ea
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "unicycle_input.txt")
+    meas, ctrl, dts, truth = ea.load_unicycle10_trace(path)
+    p = ea.unicycle10_params_default()
+    rng = np.random.default_rng(4)
+    N = 2048
+    for k in (1, 40, 100):
+        st = np.concatenate([truth[k - 1, c] + rng.normal(0, 0.3, N) for c in range(3)]).astype(np.float32)
+        normals = rng.normal(size=3 * N).astype(np.float32)
+        fr = O.build_frame(3, p, ctrl[k], meas[k], float(dts[k]))
+        s32, w32 = O.sample_weight(3, st, N, normals, fr, 1)
+        s64, lw64 = O.sample_weight_f64(3, st, N, normals, p, ctrl[k], meas[k], float(dts[k]), 3)
+        np.testing.assert_allclose(s32, s64, rtol=2e-6, atol=4e-6)
+        np.testing.assert_allclose(w32, lw64, rtol=1e-4, atol=1e-3)
+    # w = 0 exactly: the particle moves straight along theta
+    c0 = ea.UnicycleControl(2.0, 0.0)
+    st = np.array([1.0, 2.0, 0.5], np.float32)
+    fr = O.build_frame(3, p, c0, meas[0], 0.5)
+    s32, _ = O.sample_weight(3, st, 1, np.zeros(3, np.float32), fr, 1)
+    np.testing.assert_allclose(s32, [1.0 + np.cos(0.5), 2.0 + np.sin(0.5), 0.5], rtol=1e-6)
+
+
 def test_boxmuller_closed_form():
     rng = np.random.default_rng(1)
     u = (1.0 + rng.random(10000, dtype=np.float32)).astype(np.float32)
