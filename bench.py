@@ -241,7 +241,7 @@ def run_served(ea, eng, rates=(1000.0, 5000.0, 20000.0, 50000.0), duration=2.0, 
 
 
 def traffic_from_profiles():
-    p = os.path.join(REPO, "profiles", "sample_weight_c1_traffic.json")
+    p = os.path.join(REPO, "profiles", "sample_weight_unicycle2m_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as fh:
@@ -301,7 +301,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded unicycle trajectory and range+bearing measurements)",
-        "config": {"workload": "unicycle c1: 4096 sub-filters x 512 = 2097152 particles, 32 landmarks, "
+        "config": {"workload": "unicycle, 4096 sub-filters x 512 = 2097152 particles, 32 landmarks, "
                                "64x64 torus t=4, systematic resampling, MTGP32 seed 42",
                    "particles": N, "sub_filters": G, "particles_per_sub_filter": M, "landmarks": L,
                    "parallelism": "single GPU"},

@@ -10,7 +10,7 @@ Contents (all uint32):
   f32_<seed>       [200, 16]  first 16 close1_open2 float bit patterns, set i, seed+i+1
   u32_1234         [4, 4096]  first 4096 tempered uint32 words, sets 0..3, seed 1234+i+1
   f32long_1234     [4, 4096]  first 4096 float words, sets 0..3, seed 1234+i+1
-  c0_step1         [96, 768]  the first-step buffer of the 16x256 robot-arm /
+  buf16x256_step1         [96, 768]  the first-step buffer of the 16x256 robot-arm /
                               unicycle small configuration, seed 1234 (each
                               stream's first 768 float words)
 """
@@ -36,7 +36,7 @@ def main():
         data[f"f32_{s}"] = np.stack([run("f32", (s + i + 1) & 0xFFFFFFFF, i, 16) for i in range(200)])
     data["u32_1234"] = np.stack([run("u32", 1234 + i + 1, i, 4096) for i in range(4)])
     data["f32long_1234"] = np.stack([run("f32", 1234 + i + 1, i, 4096) for i in range(4)])
-    data["c0_step1"] = np.stack([run("f32", 1234 + i + 1, i, 768) for i in range(96)])
+    data["buf16x256_step1"] = np.stack([run("f32", 1234 + i + 1, i, 768) for i in range(96)])
     data["sets_init"] = np.array(SETS_INIT, dtype=np.uint32)
     data["seeds"] = np.array(SEEDS, dtype=np.uint32)
     np.savez_compressed(os.path.join(HERE, "mtgp_ref_words.npz"), **data)

@@ -154,7 +154,7 @@ def test_unicycle_tracks_truth(engine):
     assert rmse < 0.2, rmse
 
 
-def test_c1_scale_properties(engine):
+def test_two_million_particle_properties(engine):
     """The 2M-particle configuration (64x64 torus of 512): one step, checked
     through properties that do not need the oracle at this size."""
     desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=512, G=4096, t=4, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG,

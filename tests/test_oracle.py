@@ -44,10 +44,10 @@ def test_uint32_words_and_continuation(golden, mtgp_table):
         assert np.array_equal(np.concatenate(parts), golden["f32long_1234"][i])
 
 
-def test_reference_buffer_layout_c0(golden, mtgp_table):
+def test_reference_buffer_layout_16x256(golden, mtgp_table):
     status = np.concatenate([O.mtgp_seed(mtgp_table, i, 1234 + i + 1) for i in range(96)])
     buf = O.mtgp_fill(mtgp_table, 96, status, 768)
-    assert np.array_equal(buf.reshape(96, 768), golden["c0_step1"])
+    assert np.array_equal(buf.reshape(96, 768), golden["buf16x256_step1"])
 
 
 def test_float_words_in_unit_interval(golden):
