@@ -150,6 +150,7 @@ _SIGS = {
     "esth_filter_estimate": ([_P, _P], C.c_int),
     "esth_filter_read_particles": ([_P, _P, _P], C.c_int),
     "esth_filter_set_timing": ([_P, C.c_int], C.c_int),
+    "esth_filter_set_timing_every": ([_P, C.c_int], C.c_int),
     "esth_filter_stage_times": ([_P, _P], C.c_int),
     "esth_filter_snapshot_size": ([_P, _P], C.c_int),
     "esth_filter_snapshot": ([_P, _P, C.c_size_t], C.c_int),
@@ -376,9 +377,11 @@ class Filter:
         check(lib().esth_filter_read_particles(self.h, ptr(st), ptr(w)))
         return st, w
 
-    def set_timing(self, level=1):
-        """0 off, 1 time the sample+weight kernel, 2 time every stage."""
+    def set_timing(self, level=1, every=1):
+        """0 off, 1 time the sample+weight kernel, 2 time every stage; one
+        step in `every` is timed."""
         check(lib().esth_filter_set_timing(self.h, int(level)))
+        check(lib().esth_filter_set_timing_every(self.h, int(every)))
 
     def stage_times(self):
         us = np.zeros(6, np.float32)

@@ -45,11 +45,15 @@ struct esth_filter {
     std::vector<std::pair<void*, size_t>> allocs;
     /* host */
     float* h_frames = nullptr;   /* pinned ring */
-    hipEvent_t frame_ev[ESTH_FRAME_RING];
+    hipEvent_t frame_ev[ESTH_FRAME_RING];   /* graph replay: per parity slot */
     bool frame_used[ESTH_FRAME_RING] = {};
+    hipEvent_t ring_ev;                     /* eager: after the last copy of a pass over the ring */
+    bool ring_pending = false;
     int slot = 0;
     hipEvent_t ev[7];
     int timing = 0;   /* 0 off, 1 sample+weight only, 2 every stage */
+    int timing_every = 1;   /* time one step in this many */
+    uint64_t tcount = 0;
     /* per-step stage events while timing is on: 7 per step, up to ESTH_TIMING_STEPS steps */
     std::vector<hipEvent_t> tev;
     int tsteps = 0;
@@ -76,6 +80,7 @@ static void filter_free(esth_filter* f) {
     f->allocs.clear();
     if (f->h_frames) hipHostFree(f->h_frames);
     for (int i = 0; i < ESTH_FRAME_RING; ++i) hipEventDestroy(f->frame_ev[i]);
+    hipEventDestroy(f->ring_ev);
     for (int i = 0; i < 7; ++i) hipEventDestroy(f->ev[i]);
     for (int i = 0; i < 2; ++i) { hipEventDestroy(f->ev_rng_done[i]); hipEventDestroy(f->ev_consumed[i]); }
     hipEventDestroy(f->ev_fork);
@@ -114,6 +119,7 @@ extern "C" int esth_filter_create(esth_engine* e, const esth_filter_desc* d, con
     f->dimy = d->torus_dimy;
     if (d->topology == ESTH_TOPO_TORUS && (f->dimx <= 0 || f->dimy <= 0)) esth_torus_dims(f->G, &f->dimx, &f->dimy);
     for (int i = 0; i < ESTH_FRAME_RING; ++i) hipEventCreateWithFlags(&f->frame_ev[i], hipEventDisableTiming);
+    hipEventCreateWithFlags(&f->ring_ev, hipEventDisableTiming);
     for (int i = 0; i < 7; ++i) hipEventCreate(&f->ev[i]);
     for (int i = 0; i < 2; ++i) {
         hipEventCreateWithFlags(&f->ev_rng_done[i], hipEventDisableTiming);
@@ -193,13 +199,19 @@ static int launch_rng(esth_filter* f, uint64_t k, hipStream_t s) {
 /* RNG side of step n = f->steps, before its kernels: if the step's words are
  * not ready they are generated first on the main stream; with prefetch on,
  * step n+1's words are generated on the RNG stream while step n runs (their
- * buffer was last read by step n-1: the ev_consumed event orders that). */
-static int rng_before_step(esth_filter* f) {
+ * buffer was last read by step n-1: the ev_consumed event orders that).
+ * With `frame` (eager steps) the step's frame is copied here too: on the RNG
+ * stream ahead of the event the main stream waits for anyway, so the copy and
+ * the words cost the main stream one wait; *copy_stream tells where it went. */
+static int rng_before_step(esth_filter* f, float* dfr, const float* hf, bool frame, hipStream_t* copy_stream) {
     hipStream_t s = f->stream;
     const uint64_t n = f->steps;
     const int p = (int)(n & 1);
+    const size_t fbytes = sizeof(float) * ESTH_FRAME_FLOATS;
     bool forked = false;
+    *copy_stream = s;
     if (!f->rng_ready) {
+        if (frame) ESTH_HIP(hipMemcpyAsync(dfr, hf, fbytes, hipMemcpyHostToDevice, s));
         ESTH_TRY(launch_rng(f, n, s));
         if (f->prefetch) {   /* step n+1's words continue from this status */
             ESTH_HIP(hipEventRecord(f->ev_fork, s));
@@ -207,12 +219,18 @@ static int rng_before_step(esth_filter* f) {
             forked = true;
         }
     } else {
+        if (frame) {
+            ESTH_HIP(hipMemcpyAsync(dfr, hf, fbytes, hipMemcpyHostToDevice, f->rng_stream));
+            ESTH_HIP(hipEventRecord(f->ev_rng_done[p], f->rng_stream));
+            *copy_stream = f->rng_stream;
+        }
         ESTH_HIP(hipStreamWaitEvent(s, f->ev_rng_done[p], 0));
     }
     if (f->prefetch) {
         if (!forked && n > 0) ESTH_HIP(hipStreamWaitEvent(f->rng_stream, f->ev_consumed[p ^ 1], 0));
         ESTH_TRY(launch_rng(f, n + 1, f->rng_stream));
-        ESTH_HIP(hipEventRecord(f->ev_rng_done[p ^ 1], f->rng_stream));
+        /* an eager step n+1 records it after its frame copy instead */
+        if (!frame) ESTH_HIP(hipEventRecord(f->ev_rng_done[p ^ 1], f->rng_stream));
     }
     return ESTH_OK;
 }
@@ -292,13 +310,18 @@ extern "C" int esth_filter_step(esth_filter* f, const void* control, const void*
     ESTH_HIP(hipSetDevice(f->e->device));
     /* frame slot: graphs use the step parity, eager steps a ring */
     int slot = f->use_graph ? (int)(f->steps & 1) : f->slot;
-    if (f->frame_used[slot]) ESTH_HIP(hipEventSynchronize(f->frame_ev[slot]));
+    if (f->use_graph) {
+        if (f->frame_used[slot]) ESTH_HIP(hipEventSynchronize(f->frame_ev[slot]));
+    } else if (slot == 0 && f->ring_pending) {   /* the previous pass's copies are done */
+        ESTH_HIP(hipEventSynchronize(f->ring_ev));
+        f->ring_pending = false;
+    }
     float* hf = f->h_frames + (size_t)slot * ESTH_FRAME_FLOATS;
     if (esth_build_frame_impl(f->d.model, f->params, control, measurement, dt, hf))
         return esth_fail(ESTH_ERR_INVALID, "bad control/measurement for model %d", f->d.model);
     if (f->use_graph && !f->gexec[slot]) ESTH_TRY(capture_graph(f, slot, &f->gexec[slot]));
     hipEvent_t* ev = nullptr;
-    if (!f->use_graph && f->timing && f->tsteps < ESTH_TIMING_STEPS) {
+    if (!f->use_graph && f->timing && f->tsteps < ESTH_TIMING_STEPS && (f->tcount++ % (uint64_t)f->timing_every) == 0) {
         if ((size_t)(f->tsteps + 1) * 7 > f->tev.size()) {
             for (int i = 0; i < 7; ++i) {
                 hipEvent_t e;
@@ -310,10 +333,9 @@ extern "C" int esth_filter_step(esth_filter* f, const void* control, const void*
         f->tsteps++;
     }
     float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
-    if (!f->use_graph)   /* the frame copy goes ahead of the wait for the step's words */
-        ESTH_HIP(hipMemcpyAsync(dfr, hf, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice, f->stream));
     if (ev && f->timing >= 2) ESTH_HIP(hipEventRecord(ev[0], f->stream));
-    ESTH_TRY(rng_before_step(f));
+    hipStream_t copy_stream = f->stream;
+    ESTH_TRY(rng_before_step(f, dfr, hf, !f->use_graph, &copy_stream));
     if (f->use_graph) {
         ESTH_HIP(hipGraphLaunch(f->gexec[slot], f->stream));
     } else {
@@ -321,8 +343,13 @@ extern "C" int esth_filter_step(esth_filter* f, const void* control, const void*
         f->slot = (f->slot + 1) % ESTH_FRAME_RING;
     }
     ESTH_TRY(rng_after_step(f));
-    ESTH_HIP(hipEventRecord(f->frame_ev[slot], f->stream));
-    f->frame_used[slot] = true;
+    if (f->use_graph) {
+        ESTH_HIP(hipEventRecord(f->frame_ev[slot], f->stream));
+        f->frame_used[slot] = true;
+    } else if (slot == ESTH_FRAME_RING - 1) {
+        ESTH_HIP(hipEventRecord(f->ring_ev, copy_stream));
+        f->ring_pending = true;
+    }
     f->steps++;
     return ESTH_OK;
 }
@@ -364,6 +391,14 @@ extern "C" int esth_filter_set_timing(esth_filter* f, int on) {
     ESTH_HIP(hipStreamSynchronize(f->stream));
     f->timing = on < 0 ? 0 : (on > 2 ? 2 : on);
     f->tsteps = 0;
+    f->tcount = 0;
+    return ESTH_OK;
+}
+
+extern "C" int esth_filter_set_timing_every(esth_filter* f, int every) {
+    if (!f || every < 1) return esth_fail(ESTH_ERR_INVALID, "timing stride must be >= 1");
+    f->timing_every = every;
+    f->tcount = 0;
     return ESTH_OK;
 }
 

@@ -247,21 +247,83 @@ struct esth_tile {
     static constexpr int LOGL = LOGM - LOGE;
 };
 
+/* a lane's E consecutive floats at p (p aligned to 4 * min(E, 4) bytes) */
+template <int E>
+__device__ __forceinline__ void esth_store_run(float* __restrict__ p, const float (&v)[E]) {
+    if constexpr (E % 4 == 0) {
+#pragma unroll
+        for (int r = 0; r < E; r += 4)
+            *reinterpret_cast<float4*>(p + r) = make_float4(v[r], v[r + 1], v[r + 2], v[r + 3]);
+    } else if constexpr (E == 2) {
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; ++r) p[r] = v[r];
+    }
+}
+
+/* a lane's E consecutive floats from p (p aligned like esth_store_run's) */
+template <int E>
+__device__ __forceinline__ void esth_load_run(const float* __restrict__ p, float (&v)[E]) {
+    if constexpr (E % 4 == 0) {
+#pragma unroll
+        for (int r = 0; r < E; r += 4) {
+            const float4 q = *reinterpret_cast<const float4*>(p + r);
+            v[r] = q.x; v[r + 1] = q.y; v[r + 2] = q.z; v[r + 3] = q.w;
+        }
+    } else if constexpr (E == 2) {
+        const float2 q = *reinterpret_cast<const float2*>(p);
+        v[0] = q.x; v[1] = q.y;
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = p[r];
+    }
+}
+
+/* orders this wave's LDS accesses before and after it (the wave's lanes
+ * exchange data through its own LDS region) */
+__device__ __forceinline__ void esth_wave_lds_fence() {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+/* v[r] = plane[pos[r]] over one sub-filter's M floats, through LDS: the
+ * sub-filter's plane is read coalesced (each lane its E consecutive floats,
+ * vector loads) into `buf`, the sub-filter's M-float region of this wave's
+ * LDS, and the permuted reads are served there instead of as scattered
+ * global loads, each of which would touch a different cache line. */
+template <int M, int E>
+__device__ __forceinline__ void esth_gather_lds(const float* __restrict__ plane, int le, float* buf,
+                                                const int (&pos)[E], float (&v)[E]) {
+    float in[E];
+    esth_load_run<E>(plane + le, in);
+    esth_wave_lds_fence();   /* earlier reads of buf are done */
+    esth_store_run<E>(buf + le, in);
+    esth_wave_lds_fence();
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = buf[pos[r] & (M - 1)];
+}
+
 template <int LOGM, int SIZE, int STRIDE>
 __device__ __forceinline__ void esth_bitonic_stage(float (&key)[esth_tile<LOGM>::E],
                                                    int (&idx)[esth_tile<LOGM>::E], int le, int laneg) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E;
+    /* compare-exchanges as selects: a swap under a divergent branch costs an
+     * exec-mask save/restore and register moves per pair */
     if constexpr (STRIDE < E) {
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             if ((r & STRIDE) == 0) {
                 const int a = r, b = r + STRIDE;
                 const bool d = (SIZE == M) ? false : (((le + a) & SIZE) != 0);
-                if ((key[a] > key[b]) == d) {
-                    float tk = key[a]; key[a] = key[b]; key[b] = tk;
-                    int ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;
-                }
+                const float ka = key[a], kb = key[b];
+                const int ia = idx[a], ib = idx[b];
+                const bool sw = (ka > kb) == d;
+                key[a] = sw ? kb : ka;
+                key[b] = sw ? ka : kb;
+                idx[a] = sw ? ib : ia;
+                idx[b] = sw ? ia : ib;
             }
         }
     } else {
@@ -274,7 +336,9 @@ __device__ __forceinline__ void esth_bitonic_stage(float (&key)[esth_tile<LOGM>:
             const bool d = (SIZE == M) ? false : (((le + r) & SIZE) != 0);
             const float kl = lower ? key[r] : other;
             const float ku = lower ? other : key[r];
-            if ((kl > ku) == d) { key[r] = other; idx[r] = oidx; }
+            const bool sw = (kl > ku) == d;
+            key[r] = sw ? other : key[r];
+            idx[r] = sw ? oidx : idx[r];
         }
     }
 }
@@ -338,10 +402,16 @@ __device__ __forceinline__ void esth_argmax_block(const float2* __restrict__ win
         sc[l] = acc;
     }
     float bmax = -INFINITY;
-    if (log_w) {
-        for (uint32_t gi = (uint32_t)l; gi < G; gi += (uint32_t)NT) {
-            float pm = part[(size_t)gi * (2 + S)];
-            bmax = pm > bmax ? pm : bmax;
+    if (log_w) {   /* 8 loads in flight per work-item */
+        for (uint32_t g0 = (uint32_t)l; g0 < G; g0 += 8u * (uint32_t)NT) {
+            float pm[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t gi = g0 + (uint32_t)(j * NT);
+                pm[j] = gi < G ? part[(size_t)gi * (2 + S)] : -INFINITY;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bmax = pm[j] > bmax ? pm[j] : bmax;
         }
         bmax = esth_bfly_max<1, 64>(bmax);
         if (lane == 0) smax[wid] = bmax;
@@ -366,11 +436,22 @@ __device__ __forceinline__ void esth_argmax_block(const float2* __restrict__ win
     double a[2 + S];
 #pragma unroll
     for (int c = 0; c < 2 + S; ++c) a[c] = 0.0;
-    for (uint32_t gi = (uint32_t)l; gi < G; gi += (uint32_t)NT) {
-        const float* pg = part + (size_t)gi * (2 + S);
-        double scale = log_w ? (double)esth_expf(pg[0] - gmax) : 1.0;
+    /* the same per-work-item order of sub-filters, 4 of them loaded at once */
+    for (uint32_t g0 = (uint32_t)l; g0 < G; g0 += 4u * (uint32_t)NT) {
+        float pg[4][2 + S];
 #pragma unroll
-        for (int c = 1; c < 2 + S; ++c) a[c] += (double)pg[c] * scale;
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t gi = g0 + (uint32_t)(j * NT);
+#pragma unroll
+            for (int c = 0; c < 2 + S; ++c) pg[j][c] = gi < G ? part[(size_t)gi * (2 + S) + c] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (g0 + (uint32_t)(j * NT) >= G) break;
+            const double scale = log_w ? (double)esth_expf(pg[j][0] - gmax) : 1.0;
+#pragma unroll
+            for (int c = 1; c < 2 + S; ++c) a[c] += (double)pg[j][c] * scale;
+        }
     }
 #pragma unroll
     for (int c = 1; c < 2 + S; ++c) {
@@ -419,11 +500,12 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
                                               float* __restrict__ st_out, uint32_t N, uint32_t G, uint32_t g,
                                               bool valid, float2* __restrict__ winner, int* __restrict__ perm,
                                               uint32_t t, float* __restrict__ bt_w, float* __restrict__ bt_s,
-                                              float* __restrict__ part, int log_w) {
+                                              float* __restrict__ part, int log_w, float* stage_w) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
     const int le = (lane % LPS) * E;
+    float* stage = stage_w + (lane - lane % LPS) * E;   /* this sub-filter's M floats */
     const size_t base = (size_t)g * M;
     float key[E];
     int idx[E];
@@ -449,18 +531,22 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
             for (int r = 0; r < E; ++r) perm[base + le + r] = idx[r];
         }
     }
-    /* gather (particle_set of block_sort, pf_generic.cl:281-283) and partial sums */
+    /* gather (particle_set of block_sort, pf_generic.cl:281-283) and partial
+     * sums; each lane's E consecutive outputs leave as vector stores */
 #pragma unroll
     for (int c = 0; c < S; ++c) {
         const float* src = st_in + (size_t)c * N + base;
         float* dst = st_out + (size_t)c * N + base;
         float acc = 0.0f;
+        float v[E];
+        esth_gather_lds<M, E>(src, le, stage, idx, v);
 #pragma unroll
-        for (int r = 0; r < E; ++r) {
-            float v = src[idx[r] & (M - 1)];
-            if (valid) dst[le + r] = v;
-            acc = esth_fma(wt[r], v, acc);
-            if (valid && t > 0 && (uint32_t)(le + r) < t) bt_s[(size_t)c * G * t + (size_t)g * t + le + r] = v;
+        for (int r = 0; r < E; ++r) acc = esth_fma(wt[r], v[r], acc);
+        if (valid) esth_store_run<E>(dst + le, v);
+        if (valid && t > 0) {
+#pragma unroll
+            for (int r = 0; r < E; ++r)
+                if ((uint32_t)(le + r) < t) bt_s[(size_t)c * G * t + (size_t)g * t + le + r] = v[r];
         }
         sx[c] = acc;
     }
@@ -495,7 +581,9 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
     uint32_t g = wave * (ESTH_WAVE / LPS) + (uint32_t)(lane / LPS);
     const bool valid = g < G;   /* a whole sub-filter is valid or not */
     if (!valid) g = G - 1;
-    esth_sort_sub<LOGM, S>(w, st_in, st_out, N, G, g, valid, winner, perm, t, bt_w, bt_s, part, log_w);
+    __shared__ float stage[4][ESTH_WAVE * esth_tile<LOGM>::E];
+    esth_sort_sub<LOGM, S>(w, st_in, st_out, N, G, g, valid, winner, perm, t, bt_w, bt_s, part, log_w,
+                           stage[threadIdx.x / ESTH_WAVE]);
 }
 
 /* ============================================================= exchange */
@@ -660,22 +748,26 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
         float s;
         if (systematic) s = ((float)(le + r) + u0) * step;
         else s = (rnd[base + le + r] - 1.0f) * total;
-        int mn = 0, mxi = M - 1;          /* binary_search, pf_generic.cl:182-201 */
-        do {
-            int cur = mn + (mxi - mn + 1) / 2;
-            if (cdf[cur] > s) mxi = cur - 1;
-            else mn = cur;
-        } while (mn < mxi);
+        /* binary_search, pf_generic.cl:182-201: on a power-of-two range each
+         * round halves it exactly, so its do-while runs LOGM rounds; unrolled,
+         * the E searches of a lane interleave */
+        int mn = 0, mxi = M - 1;
+#pragma unroll
+        for (int it = 0; it < LOGM; ++it) {
+            const int cur = mn + (mxi - mn + 1) / 2;
+            const bool left = cdf[cur] > s;
+            mxi = left ? cur - 1 : mxi;
+            mn = left ? mn : cur;
+        }
         pick[r] = mn & (M - 1);
     }
-    if (go) {
+    /* gather the selected particles through the cdf's LDS region, plane by
+     * plane (every lane takes part; only resampled sub-filters store) */
 #pragma unroll
-        for (int c = 0; c < S; ++c) {
-            const float* src = st_tmp + (size_t)c * N + base;
-            float* dst = st_main + (size_t)c * N + base;
-#pragma unroll
-            for (int r = 0; r < E; ++r) dst[le + r] = src[pick[r]];
-        }
+    for (int c = 0; c < S; ++c) {
+        float v[E];
+        esth_gather_lds<M, E>(st_tmp + (size_t)c * N + base, le, cdf, pick, v);
+        if (go) esth_store_run<E>(st_main + (size_t)c * N + base + le, v);
     }
     if (sel && valid) {
 #pragma unroll
@@ -972,8 +1064,9 @@ __global__ __launch_bounds__(256) void kb_sort(const esth_bfilter* __restrict__ 
     if (wv >= nsub) return;
     const uint2 e = sub[wv];
     const esth_bfilter& f = F[e.x];
+    __shared__ float stage[4][ESTH_WAVE * esth_tile<LOGM>::E];
     esth_sort_sub<LOGM, S>(f.w, f.state, f.tmp, f.N, f.G, e.y, true, (float2*)f.winner, nullptr, f.t, f.bt_w,
-                           f.bt_s, f.part, log_w);
+                           f.bt_s, f.part, log_w, stage[threadIdx.x / ESTH_WAVE]);
 }
 
 template <int S>

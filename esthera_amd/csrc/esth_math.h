@@ -75,13 +75,15 @@ ESTH_HD float esth_pow2i(int n) { return esth_u2f((uint32_t)(n + 127) << 23); }
 #define ESTH_PIO2_3  -1.7151245100058819e-15f      /* 0xa6f72ced */
 #define ESTH_2_OVER_PI 0.63661977236758134308f
 
-/* e^x.  exp(x) = 2^n * e^r with n = rint(x/ln2), r = x - n ln2 (two-part ln2). */
+/* e^x.  exp(x) = 2^n * e^r with n = rint(x/ln2), r = x - n ln2 (two-part ln2).
+ * Branch-free: the polynomial runs on x clamped to [-87.33, 88.72] and the
+ * out-of-range and NaN results are selected at the end; 2^n is applied as
+ * 2^(n>>1) * 2^(n - (n>>1)), both factors normal, the first product exact, so
+ * the one rounding is that of p * 2^n. */
 ESTH_HD float esth_expf(float x) {
-    if (x < -87.33f) return 0.0f;
-    if (x > 88.72f) return esth_u2f(0x7f800000u);
-    if (x != x) return x;
-    float n = esth_rint(x * ESTH_INV_LN2);
-    float r = esth_fma(-n, ESTH_LN2_HI, x);
+    const float xc = fminf(fmaxf(x, -87.33f), 88.72f);
+    float n = esth_rint(xc * ESTH_INV_LN2);
+    float r = esth_fma(-n, ESTH_LN2_HI, xc);
     r = esth_fma(-n, ESTH_LN2_LO, r);
     /* Taylor degree 7: 1 + r + r^2/2! + ... + r^7/7! */
     float p = 1.98412698412698412698e-04f;            /* 1/5040 */
@@ -92,11 +94,12 @@ ESTH_HD float esth_expf(float x) {
     p = esth_fma(p, r, 0.5f);
     p = esth_fma(p, r, 1.0f);
     p = esth_fma(p, r, 1.0f);
-    int ni = (int)n;
-    /* split the scaling so that neither factor leaves the normal range */
-    if (ni > 127) { p *= esth_pow2i(127); ni -= 127; }
-    if (ni < -126) { p *= esth_pow2i(-126); ni += 126; }
-    return p * esth_pow2i(ni);
+    const int ni = (int)n;
+    const int n1 = ni >> 1;
+    float y = (p * esth_pow2i(n1)) * esth_pow2i(ni - n1);
+    y = (x < -87.33f) ? 0.0f : y;
+    y = (x > 88.72f) ? esth_u2f(0x7f800000u) : y;
+    return (x != x) ? x : y;
 }
 
 /* natural log for x > 0 (x <= 0 returns -inf / NaN). */

@@ -214,6 +214,10 @@ int esth_filter_read_particles(esth_filter* f, float* state_soa, float* weights)
  * step in microseconds of [rng wait, sample_weight, sort, argmax,
  * exchange/top-t, resample]; stages not timed are NaN. */
 int esth_filter_set_timing(esth_filter* f, int on);
+/* With timing on, time one step in `every` (default 1): the events around a
+ * kernel cost the stream a few microseconds each, so sparse timing keeps the
+ * measured stream close to the untimed one. */
+int esth_filter_set_timing_every(esth_filter* f, int every);
 int esth_filter_stage_times(esth_filter* f, float us[6]);
 /* Checkpoint / resume: particles, weights, RNG status and step counter. */
 int esth_filter_snapshot_size(esth_filter* f, size_t* bytes);

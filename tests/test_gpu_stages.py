@@ -28,14 +28,14 @@ def assert_bits(a, b):
 
 
 # ----------------------------------------------------------------- MTGP32
-def test_mtgp_c0_buffer_matches_reference_library(golden, engine):
+def test_mtgp_16x256_buffer_matches_reference_library(golden, engine):
     """96 streams x 768 words, seed 1234: the reference's first-step buffer."""
     st = np.concatenate([ea.mtgp_seed_state(i, 1234 + i + 1) for i in range(96)])
     dst = DB.from_array(st)
     out = DB(96 * 768 * 4)
     ea.check(ea.lib().esth_stage_mtgp(dst.ptr, 96, 0, 768, out.ptr, 1))
     got = out.to_array(np.uint32, 96 * 768).reshape(96, 768)
-    assert_bits(got.ravel(), golden["c0_step1"].ravel())
+    assert_bits(got.ravel(), golden["buf16x256_step1"].ravel())
 
 
 def test_mtgp_u32_and_continuation(golden, engine):
