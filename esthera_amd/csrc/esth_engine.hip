@@ -377,22 +377,31 @@ template <int LOGM, int S>
 static void launch_resample_argmax_t(float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
                                      uint32_t G, float thr, int systematic, int log_w, const float2* winner,
                                      const float* part, const float* best_src, size_t plane_stride, uint32_t g_stride,
-                                     float2* result, double* est, hipStream_t s) {
+                                     float2* result, double* est, const esth_xch& xch, hipStream_t s) {
     using TL = esth_tile<LOGM>;
     const uint64_t waves = (G + (ESTH_WAVE / TL::LPS) - 1) / (ESTH_WAVE / TL::LPS);
     hipLaunchKernelGGL((k_resample_argmax<LOGM, S>), dim3(1 + grid_for(waves * ESTH_WAVE, 256)), dim3(256), 0, s,
                        st_main, st_tmp, w, rnd, N, G, thr, systematic, log_w, winner, part, best_src, plane_stride,
-                       g_stride, result, est);
+                       g_stride, result, est, xch);
 }
 
-int esth_launch_resample_argmax(int resampler, int S, float* st_main, const float* st_tmp, const float* w,
+int esth_launch_resample_argmax(int resampler, int S, float* st_main, const float* st_tmp, float* w,
                                 const float* rnd, uint32_t N, uint32_t m, uint32_t G, float thr, int log_w,
                                 const float2* winner, const float* part, const float* best_src, size_t plane_stride,
-                                uint32_t g_stride, float2* result, double* est, hipStream_t s) {
+                                uint32_t g_stride, float2* result, double* est, uint32_t t, int topo, int dimx,
+                                int dimy, const float* bt_w, const float* bt_s, uint32_t* top, hipStream_t s) {
     if (resampler == ESTH_RESAMPLE_ALIAS) return esth_fail(ESTH_ERR_UNSUPPORTED, "fused argmax: scan resamplers only");
     const int lm = log2u(m);
     const int sys = resampler == ESTH_RESAMPLE_SYSTEMATIC;
-#define ESTH_RA_ARGS st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, winner, part, best_src, plane_stride, g_stride, result, est, s
+    esth_xch xch{};
+    if (t > 0 && topo != ESTH_TOPO_NONE) {
+        if (topo == ESTH_TOPO_ALL2ALL) {   /* the global top t the receivers read */
+            hipLaunchKernelGGL(k_topt, dim3(1), dim3(1024), 0, s, bt_w, G * t, t, top);
+            ESTH_HIP(hipGetLastError());
+        }
+        xch = {bt_w, bt_s, top, w, G, t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, dimx, dimy};
+    }
+#define ESTH_RA_ARGS st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, winner, part, best_src, plane_stride, g_stride, result, est, xch, s
     switch (S) {
     case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_UNI_S, ESTH_RA_ARGS); break;
     case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_RA5_S, ESTH_RA_ARGS); break;

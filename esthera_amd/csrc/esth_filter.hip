@@ -267,9 +267,11 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
     const bool fused = d.resampler != ESTH_RESAMPLE_ALIAS;
     if (!fused) ESTH_TRY(esth_launch_argmax(f->S, f->winner, f->G, log_w, f->part, f->tmp, N, f->m, f->result, f->est, s));
     if (all) ESTH_HIP(hipEventRecord(ev[4], s));
-    /* exchange (pf_generic.c:702-715) */
-    ESTH_TRY(esth_launch_exchange(f->tmp, f->w, f->S, N, f->m, f->G, f->t, d.topology, f->dimx, f->dimy, f->bt_w,
-                                  f->bt_s, f->top, s));
+    /* exchange (pf_generic.c:702-715): with a scan resampler its receiving
+     * side is applied by the resampling launch as it reads each sub-filter */
+    if (!fused)
+        ESTH_TRY(esth_launch_exchange(f->tmp, f->w, f->S, N, f->m, f->G, f->t, d.topology, f->dimx, f->dimy, f->bt_w,
+                                      f->bt_s, f->top, s));
     if (all) ESTH_HIP(hipEventRecord(ev[5], s));
     /* resampling (pf_generic.c:732-757), uniforms at rand_offset = N*V */
     if (fused) {
@@ -277,7 +279,8 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
         ESTH_TRY(esth_launch_resample_argmax(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m,
                                              f->G, d.resampling_threshold, log_w, f->winner, f->part,
                                              bt ? f->bt_s : f->tmp, bt ? (size_t)f->G * f->t : (size_t)N,
-                                             bt ? f->t : f->m, f->result, f->est, s));
+                                             bt ? f->t : f->m, f->result, f->est, f->t, d.topology, f->dimx,
+                                             f->dimy, f->bt_w, f->bt_s, f->top, s));
     } else {
         ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m, f->G,
                                       d.resampling_threshold, d.adaptive, log_w, nullptr, s));

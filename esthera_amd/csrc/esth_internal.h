@@ -64,12 +64,14 @@ int esth_launch_exchange(float* st, float* w, int S, uint32_t N, uint32_t m, uin
 int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w,
                          const float* rnd, uint32_t N, uint32_t m, uint32_t G, float thr,
                          int adaptive, int log_w, int* sel, hipStream_t s);
-/* scan resampling with the step's argmax + posterior in the same launch; the
- * best particle is read from best_src[c * plane_stride + g * g_stride] */
-int esth_launch_resample_argmax(int resampler, int S, float* st_main, const float* st_tmp, const float* w,
+/* scan resampling with the step's argmax + posterior and the exchange (its
+ * receiving side, plus the all-to-all top t) in the same launch; the best
+ * particle is read from best_src[c * plane_stride + g * g_stride] */
+int esth_launch_resample_argmax(int resampler, int S, float* st_main, const float* st_tmp, float* w,
                                 const float* rnd, uint32_t N, uint32_t m, uint32_t G, float thr, int log_w,
                                 const float2* winner, const float* part, const float* best_src, size_t plane_stride,
-                                uint32_t g_stride, float2* result, double* est, hipStream_t s);
+                                uint32_t g_stride, float2* result, double* est, uint32_t t, int topo, int dimx,
+                                int dimy, const float* bt_w, const float* bt_s, uint32_t* top, hipStream_t s);
 /* the global top t of the compact best-t buffer (all-to-all exchange) */
 int esth_launch_topt(const float* bt_w, uint32_t n, uint32_t t, uint32_t* top, hipStream_t s);
 

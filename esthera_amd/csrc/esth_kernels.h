@@ -177,25 +177,18 @@ struct esth_model_traits<ESTH_MODEL_UNICYCLE10> {
     __device__ static float iw(float* s, const float* z, const float* fr) { return esth_unicycle10_sampling_iw(s, z, fr); }
 };
 
+/* particles p (s0) and p+1 (s1) given their uniform pairs (u0[k], u1[k]) =
+ * (R[k N + p], R[k N + p + 1]): Box-Muller, propagation and log-weight */
 template <int MODEL>
-__device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ state, uint32_t N,
-                                                        const float* __restrict__ R,
-                                                        const float* __restrict__ frame, int log_w,
-                                                        float* __restrict__ w, uint32_t p) {
+__device__ __forceinline__ void esth_sample_pair_regs(float (&s0)[esth_model_traits<MODEL>::S],
+                                                      float (&s1)[esth_model_traits<MODEL>::S],
+                                                      const float (&u0)[esth_model_traits<MODEL>::V],
+                                                      const float (&u1)[esth_model_traits<MODEL>::V],
+                                                      const float* __restrict__ frame, float& lw0, float& lw1) {
     using T = esth_model_traits<MODEL>;
-    float s0[T::S], s1[T::S], z0[T::V], z1[T::V];
+    float z0[T::V], z1[T::V];
 #pragma unroll
-    for (int c = 0; c < T::S; ++c) {
-        float2 v = *reinterpret_cast<const float2*>(state + (size_t)c * N + p);
-        s0[c] = v.x;
-        s1[c] = v.y;
-    }
-#pragma unroll
-    for (int k = 0; k < T::V; ++k) {
-        float2 u = *reinterpret_cast<const float2*>(R + (size_t)k * N + p);
-        esth_boxmuller_pair(u.x, u.y, &z0[k], &z1[k]);
-    }
-    float lw0, lw1;
+    for (int k = 0; k < T::V; ++k) esth_boxmuller_pair(u0[k], u1[k], &z0[k], &z1[k]);
     if constexpr (MODEL == ESTH_MODEL_UNICYCLE) {
         /* both particles walk the landmarks together: one scalar load of a
          * landmark feeds two independent chains (same operations per particle) */
@@ -216,6 +209,29 @@ __device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ stat
         lw0 = T::iw(s0, z0, frame);
         lw1 = T::iw(s1, z1, frame);
     }
+}
+
+template <int MODEL>
+__device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ state, uint32_t N,
+                                                        const float* __restrict__ R,
+                                                        const float* __restrict__ frame, int log_w,
+                                                        float* __restrict__ w, uint32_t p) {
+    using T = esth_model_traits<MODEL>;
+    float s0[T::S], s1[T::S], u0[T::V], u1[T::V];
+#pragma unroll
+    for (int c = 0; c < T::S; ++c) {
+        float2 v = *reinterpret_cast<const float2*>(state + (size_t)c * N + p);
+        s0[c] = v.x;
+        s1[c] = v.y;
+    }
+#pragma unroll
+    for (int k = 0; k < T::V; ++k) {
+        float2 u = *reinterpret_cast<const float2*>(R + (size_t)k * N + p);
+        u0[k] = u.x;
+        u1[k] = u.y;
+    }
+    float lw0, lw1;
+    esth_sample_pair_regs<MODEL>(s0, s1, u0, u1, frame, lw0, lw1);
 #pragma unroll
     for (int c = 0; c < T::S; ++c)
         *reinterpret_cast<float2*>(state + (size_t)c * N + p) = make_float2(s0[c], s1[c]);
@@ -293,15 +309,21 @@ __device__ __forceinline__ void esth_wave_lds_fence() {
  * LDS, and the permuted reads are served there instead of as scattered
  * global loads, each of which would touch a different cache line. */
 template <int M, int E>
-__device__ __forceinline__ void esth_gather_lds(const float* __restrict__ plane, int le, float* buf,
-                                                const int (&pos)[E], float (&v)[E]) {
-    float in[E];
-    esth_load_run<E>(plane + le, in);
+__device__ __forceinline__ void esth_permute_lds(const float (&in)[E], int le, float* buf, const int (&pos)[E],
+                                                 float (&v)[E]) {
     esth_wave_lds_fence();   /* earlier reads of buf are done */
     esth_store_run<E>(buf + le, in);
     esth_wave_lds_fence();
 #pragma unroll
     for (int r = 0; r < E; ++r) v[r] = buf[pos[r] & (M - 1)];
+}
+
+template <int M, int E>
+__device__ __forceinline__ void esth_gather_lds(const float* __restrict__ plane, int le, float* buf,
+                                                const int (&pos)[E], float (&v)[E]) {
+    float in[E];
+    esth_load_run<E>(plane + le, in);
+    esth_permute_lds<M, E>(in, le, buf, pos, v);
 }
 
 template <int LOGM, int SIZE, int STRIDE>
@@ -495,6 +517,9 @@ __global__ __launch_bounds__(1024) void k_argmax(const float2* __restrict__ winn
  * argmax stays a separate launch: folding it into the last work-group needs an
  * agent-scope release per work-group, which on the 8-XCD part writes back the
  * L2 and costs more than the launch it saves.) */
+/* One sub-filter (one wave, or 64/m sub-filters per wave): sort its weights
+ * (block_sort, pf_generic.cl:252-291) and write the sorted weights, the
+ * gathered state, the compact best t, the winner and the posterior partials. */
 template <int LOGM, int S>
 __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float* __restrict__ st_in,
                                               float* __restrict__ st_out, uint32_t N, uint32_t G, uint32_t g,
@@ -509,8 +534,9 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
     const size_t base = (size_t)g * M;
     float key[E];
     int idx[E];
+    esth_load_run<E>(w + base + le, key);
 #pragma unroll
-    for (int r = 0; r < E; ++r) { key[r] = w[base + le + r]; idx[r] = le + r; }
+    for (int r = 0; r < E; ++r) idx[r] = le + r;
     esth_bitonic<LOGM>(key, idx, le);
     /* block max M_g = key of position 0 */
     const int lead = lane - (lane % LPS);
@@ -687,15 +713,31 @@ __device__ __forceinline__ void esth_scan_levels(float& node, float* lx) {
  * set, L_k being the left sibling's node.  Levels inside a lane come from a
  * register tree, levels across lanes from xor butterflies (a + b == b + a in
  * IEEE arithmetic, so both lanes of a pair hold the same node bits). */
+/* The receiving side of the exchange, applied while the resampling reads its
+ * sub-filter: slot m-1-k-q*t takes rank k of sender q's compact best t
+ * (esth_recv_src), in the weights the scan sees, in the states the gather
+ * draws from, and in the stored sorted weights (w_out).  The exchanged
+ * states are only ever read by the resampling, so they are not stored. */
+struct esth_xch {
+    const float* bt_w;
+    const float* bt_s;
+    const uint32_t* top;
+    float* w_out;
+    uint32_t G, t, nq;
+    int topo, dimx, dimy;
+};
+
 /* one sub-filter per wave (or 64/m per wave); every wave of the work-group
- * must call it (one barrier inside); cdf_w: this wave's 64*E floats of LDS */
+ * must call it (one barrier inside); cdf_w: this wave's 64*E floats of LDS;
+ * xch (optional): the exchange to apply on the way in */
 template <int LOGM, int S>
 __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                                                   const float* __restrict__ st_tmp,
-                                                  const float* __restrict__ w,
+                                                  const float* w,   /* xch->w_out may write it */
                                                   const float* __restrict__ rnd, uint32_t N, uint32_t g,
                                                   bool valid, float thr, int systematic, int log_w,
-                                                  int* __restrict__ sel, float* cdf_w) {
+                                                  int* __restrict__ sel, float* cdf_w,
+                                                  const esth_xch* xch = nullptr) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS, LOGE = TL::LOGE, LOGL = TL::LOGL;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
@@ -704,8 +746,23 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
     const size_t base = (size_t)g * M;
     const bool go = valid && !((rnd[N + g] - 1.0f) > thr);   /* gate, pf_generic.cl:403 */
     float x[E];
+    esth_load_run<E>(w + base + le, x);
+    int rsrc[E];   /* compact best-t index of a received slot, or -1 */
 #pragma unroll
-    for (int r = 0; r < E; ++r) x[r] = w[base + le + r];
+    for (int r = 0; r < E; ++r) rsrc[r] = -1;
+    if (xch && xch->t > 0) {
+        const uint32_t t = xch->t, nrecv = xch->nq * t;
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t pos = (uint32_t)(le + r);
+            if (pos >= (uint32_t)M - nrecv) {
+                const uint32_t j = (uint32_t)M - 1u - pos, q = j / t, k = j - q * t;
+                rsrc[r] = (int)esth_recv_src(g, q, k, xch->G, t, xch->topo, xch->dimx, xch->dimy, xch->top);
+                x[r] = xch->bt_w[rsrc[r]];
+                if (valid) xch->w_out[base + pos] = x[r];
+            }
+        }
+    }
     if (log_w) {
         float mx = x[0];
 #pragma unroll
@@ -765,8 +822,15 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
      * plane (every lane takes part; only resampled sub-filters store) */
 #pragma unroll
     for (int c = 0; c < S; ++c) {
-        float v[E];
-        esth_gather_lds<M, E>(st_tmp + (size_t)c * N + base, le, cdf, pick, v);
+        float in[E], v[E];
+        esth_load_run<E>(st_tmp + (size_t)c * N + base + le, in);
+        if (xch && xch->t > 0) {
+            const size_t Gt = (size_t)xch->G * xch->t;
+#pragma unroll
+            for (int r = 0; r < E; ++r)
+                if (rsrc[r] >= 0) in[r] = xch->bt_s[(size_t)c * Gt + (size_t)rsrc[r]];
+        }
+        esth_permute_lds<M, E>(in, le, cdf, pick, v);
         if (go) esth_store_run<E>(st_main + (size_t)c * N + base + le, v);
     }
     if (sel && valid) {
@@ -802,14 +866,14 @@ __global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
 template <int LOGM, int S>
 __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_main,
                                                          const float* __restrict__ st_tmp,
-                                                         const float* __restrict__ w,
+                                                         const float* w,
                                                          const float* __restrict__ rnd, uint32_t N, uint32_t G,
                                                          float thr, int systematic, int log_w,
                                                          const float2* __restrict__ winner,
                                                          const float* __restrict__ part,
                                                          const float* __restrict__ best_src, size_t plane_stride,
                                                          uint32_t g_stride, float2* __restrict__ result,
-                                                         double* __restrict__ est) {
+                                                         double* __restrict__ est, esth_xch xch) {
     using TL = esth_tile<LOGM>;
     __shared__ float cdf_s[4][ESTH_WAVE * TL::E];
     __shared__ float2 sc[128];
@@ -825,7 +889,7 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
     const bool valid = g < G;
     if (!valid) g = G - 1;
     esth_resample_sub<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, nullptr,
-                               cdf_s[threadIdx.x / ESTH_WAVE]);
+                               cdf_s[threadIdx.x / ESTH_WAVE], &xch);
 }
 
 /* =========================================================== Vose alias
