@@ -111,6 +111,8 @@ struct esth_batch {
     /* device */
     esth_bfilter* d_F = nullptr;
     uint2* d_sub_all = nullptr;
+    uint2* d_chunks = nullptr;   /* (filter, first pair) per 64 particle pairs */
+    uint32_t nchunks = 0;
     std::vector<esth_bclass> cls;
     float *state = nullptr, *tmp = nullptr, *w = nullptr, *part = nullptr, *bt_w = nullptr, *bt_s = nullptr;
     float* R[2] = {nullptr, nullptr};
@@ -262,6 +264,8 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
     BA(b->frames[1], sizeof(float) * b->frame_stride * n);
     BA(b->d_F, sizeof(esth_bfilter) * n);
     BA(b->d_sub_all, sizeof(uint2) * Gtot);
+    for (int i = 0; i < n; ++i) b->nchunks += (b->hf[i].N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
+    BA(b->d_chunks, sizeof(uint2) * b->nchunks);
 #undef BA
     if (rc != ESTH_OK) { batch_free(b); return rc; }
     /* filter table, sub-filter tables, RNG streams */
@@ -299,6 +303,11 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
     }
     hipError_t err = hipMemcpy(b->d_F, b->hf.data(), sizeof(esth_bfilter) * n, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(b->d_sub_all, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice);
+    std::vector<uint2> chunks;
+    chunks.reserve(b->nchunks);
+    for (int i = 0; i < n; ++i)
+        for (uint32_t q = 0; q < b->hf[i].N / 2; q += ESTH_WAVE) chunks.push_back(make_uint2((uint32_t)i, q));
+    if (err == hipSuccess) err = hipMemcpy(b->d_chunks, chunks.data(), sizeof(uint2) * chunks.size(), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(b->streams, hs.data(), sizeof(esth_mtgp_stream) * hs.size(), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(b->status[0], st.data(), sizeof(uint32_t) * st.size(), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(b->w, 0, sizeof(float) * b->Ntot);
@@ -349,10 +358,13 @@ template <int LOGM, int S>
 static void launch_kb_sort(esth_batch* b, const esth_bclass& c, int log_w) {
     hipLaunchKernelGGL((kb_sort<LOGM, S>), dim3((c.n + 3) / 4), dim3(256), 0, b->stream, b->d_F, c.d_sub, c.n, log_w);
 }
+/* argmax_blocks: the first launch after the sorts also runs every filter's
+ * argmax (one work-group each, ahead of the resampling work-groups) */
 template <int LOGM, int S>
-static void launch_kb_resample(esth_batch* b, const esth_bclass& c, const float* R, int log_w) {
-    hipLaunchKernelGGL((kb_resample<LOGM, S>), dim3((c.n + 3) / 4), dim3(256), 0, b->stream, b->d_F, c.d_sub, c.n, R,
-                       b->d.resampling_threshold, b->d.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0, log_w);
+static void launch_kb_resample(esth_batch* b, const esth_bclass& c, const float* R, int log_w, uint32_t argmax_blocks) {
+    hipLaunchKernelGGL((kb_resample<LOGM, S>), dim3(argmax_blocks + (c.n + 3) / 4), dim3(256), 0, b->stream, b->d_F,
+                       c.d_sub, c.n, R, b->d.resampling_threshold, b->d.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0,
+                       log_w, b->d.topology, argmax_blocks);
 }
 
 #define ESTH_BCLASS(FN, S, ...)                        \
@@ -372,37 +384,39 @@ static int batch_kernels_s(esth_batch* b, int p) {
     const float* frames = b->frames[p];
     hipStream_t s = b->stream;
     const uint32_t nsub = b->nsub;
-    const int grid_w = (int)(((uint64_t)nsub * ESTH_WAVE + 255) / 256);
+    const int grid_w = (int)(((uint64_t)b->nchunks * ESTH_WAVE + 255) / 256);
     switch (b->d.model) {
     case ESTH_MODEL_ROBOTARM5:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
         break;
     case ESTH_MODEL_UNICYCLE:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
         break;
     case ESTH_MODEL_UNICYCLE10:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
         break;
     default:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub, R, frames, log_w);
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
         break;
     }
     for (const esth_bclass& c : b->cls) ESTH_BCLASS(launch_kb_sort, S, b, c, log_w);
-    hipLaunchKernelGGL(kb_argmax<S>, dim3(b->n), dim3(256), 0, s, b->d_F, log_w);
     const uint32_t t = b->d.topology == ESTH_TOPO_NONE ? 0 : b->d.num_transfer;
-    if (t > 0) {
-        if (b->d.topology == ESTH_TOPO_ALL2ALL) hipLaunchKernelGGL(kb_topt, dim3(b->n), dim3(1024), 0, s, b->d_F);
-        const uint32_t nslots = (uint32_t)b->nq * t;
-        const uint64_t total = (uint64_t)nsub * nslots;
-        hipLaunchKernelGGL(kb_exchange, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, b->d_F, b->d_sub_all, nsub,
-                           S, nslots, b->d.topology);
-    }
+    if (t > 0 && b->d.topology == ESTH_TOPO_ALL2ALL) hipLaunchKernelGGL(kb_topt, dim3(b->n), dim3(1024), 0, s, b->d_F);
     if (b->d.resampler == ESTH_RESAMPLE_ALIAS) {
+        hipLaunchKernelGGL(kb_argmax<S>, dim3(b->n), dim3(256), 0, s, b->d_F, log_w);
+        if (t > 0) {
+            const uint32_t nslots = (uint32_t)b->nq * t;
+            const uint64_t total = (uint64_t)nsub * nslots;
+            hipLaunchKernelGGL(kb_exchange, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, b->d_F, b->d_sub_all,
+                               nsub, S, nslots, b->d.topology);
+        }
         for (const esth_bclass& c : b->cls)
             hipLaunchKernelGGL(kb_alias<S>, dim3(c.n), dim3(1u << c.logm), 0, s, b->d_F, c.d_sub, R,
                                b->d.resampling_threshold, b->d.adaptive, log_w);
     } else {
-        for (const esth_bclass& c : b->cls) ESTH_BCLASS(launch_kb_resample, S, b, c, R, log_w);
+        /* the exchange's receiving side runs inside the resampling */
+        hipLaunchKernelGGL(kb_argmax<S>, dim3(b->n), dim3(256), 0, s, b->d_F, log_w);
+        for (const esth_bclass& c : b->cls) ESTH_BCLASS(launch_kb_resample, S, b, c, R, log_w, 0u);
     }
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;

@@ -458,17 +458,18 @@ __device__ __forceinline__ void esth_argmax_block(const float2* __restrict__ win
     double a[2 + S];
 #pragma unroll
     for (int c = 0; c < 2 + S; ++c) a[c] = 0.0;
-    /* the same per-work-item order of sub-filters, 4 of them loaded at once */
-    for (uint32_t g0 = (uint32_t)l; g0 < G; g0 += 4u * (uint32_t)NT) {
-        float pg[4][2 + S];
+    /* the same per-work-item order of sub-filters, U of them loaded at once */
+    constexpr int U = (2 + S) <= 5 ? 4 : 2;
+    for (uint32_t g0 = (uint32_t)l; g0 < G; g0 += (uint32_t)U * (uint32_t)NT) {
+        float pg[U][2 + S];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < U; ++j) {
             const uint32_t gi = g0 + (uint32_t)(j * NT);
 #pragma unroll
             for (int c = 0; c < 2 + S; ++c) pg[j][c] = gi < G ? part[(size_t)gi * (2 + S) + c] : 0.0f;
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < U; ++j) {
             if (g0 + (uint32_t)(j * NT) >= G) break;
             const double scale = log_w ? (double)esth_expf(pg[j][0] - gmax) : 1.0;
 #pragma unroll
@@ -558,14 +559,20 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
         }
     }
     /* gather (particle_set of block_sort, pf_generic.cl:281-283) and partial
-     * sums; each lane's E consecutive outputs leave as vector stores */
+     * sums; each lane's E consecutive outputs leave as vector stores; the
+     * planes' loads are issued together, PB planes at a time */
+    constexpr int PB = (S * E <= 64) ? S : 1;
+    float in[PB][E];
 #pragma unroll
     for (int c = 0; c < S; ++c) {
-        const float* src = st_in + (size_t)c * N + base;
+        if (c % PB == 0) {
+#pragma unroll
+            for (int j = 0; j < PB; ++j) esth_load_run<E>(st_in + (size_t)(c + j) * N + base + le, in[j]);
+        }
         float* dst = st_out + (size_t)c * N + base;
         float acc = 0.0f;
         float v[E];
-        esth_gather_lds<M, E>(src, le, stage, idx, v);
+        esth_permute_lds<M, E>(in[c % PB], le, stage, idx, v);
 #pragma unroll
         for (int r = 0; r < E; ++r) acc = esth_fma(wt[r], v[r], acc);
         if (valid) esth_store_run<E>(dst + le, v);
@@ -729,15 +736,15 @@ struct esth_xch {
 
 /* one sub-filter per wave (or 64/m per wave); every wave of the work-group
  * must call it (one barrier inside); cdf_w: this wave's 64*E floats of LDS;
- * xch (optional): the exchange to apply on the way in */
+ * recv: apply the exchange xch on the way in */
 template <int LOGM, int S>
 __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                                                   const float* __restrict__ st_tmp,
-                                                  const float* w,   /* xch->w_out may write it */
+                                                  const float* w,   /* xch.w_out may write it */
                                                   const float* __restrict__ rnd, uint32_t N, uint32_t g,
                                                   bool valid, float thr, int systematic, int log_w,
                                                   int* __restrict__ sel, float* cdf_w,
-                                                  const esth_xch* xch = nullptr) {
+                                                  bool recv = false, esth_xch xch = esth_xch{}) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS, LOGE = TL::LOGE, LOGL = TL::LOGL;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
@@ -745,23 +752,30 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
     const int le = laneg * E;
     const size_t base = (size_t)g * M;
     const bool go = valid && !((rnd[N + g] - 1.0f) > thr);   /* gate, pf_generic.cl:403 */
-    float x[E];
-    esth_load_run<E>(w + base + le, x);
     int rsrc[E];   /* compact best-t index of a received slot, or -1 */
 #pragma unroll
     for (int r = 0; r < E; ++r) rsrc[r] = -1;
-    if (xch && xch->t > 0) {
-        const uint32_t t = xch->t, nrecv = xch->nq * t;
+    recv = recv && xch.t > 0;
+    if (recv) {
+        const uint32_t t = xch.t, nrecv = xch.nq * t;
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             const uint32_t pos = (uint32_t)(le + r);
             if (pos >= (uint32_t)M - nrecv) {
                 const uint32_t j = (uint32_t)M - 1u - pos, q = j / t, k = j - q * t;
-                rsrc[r] = (int)esth_recv_src(g, q, k, xch->G, t, xch->topo, xch->dimx, xch->dimy, xch->top);
-                x[r] = xch->bt_w[rsrc[r]];
-                if (valid) xch->w_out[base + pos] = x[r];
+                rsrc[r] = (int)esth_recv_src(g, q, k, xch.G, t, xch.topo, xch.dimx, xch.dimy, xch.top);
             }
         }
+    }
+    float x[E];
+    esth_load_run<E>(w + base + le, x);
+    if (recv) {
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+            if (rsrc[r] >= 0) {
+                x[r] = xch.bt_w[rsrc[r]];
+                if (valid) xch.w_out[base + le + r] = x[r];
+            }
     }
     if (log_w) {
         float mx = x[0];
@@ -819,19 +833,28 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
         pick[r] = mn & (M - 1);
     }
     /* gather the selected particles through the cdf's LDS region, plane by
-     * plane (every lane takes part; only resampled sub-filters store) */
+     * plane (every lane takes part; only resampled sub-filters store); the
+     * planes' loads are issued together, PB planes at a time */
+    constexpr int PB = (S * E <= 64) ? S : 1;
 #pragma unroll
-    for (int c = 0; c < S; ++c) {
-        float in[E], v[E];
-        esth_load_run<E>(st_tmp + (size_t)c * N + base + le, in);
-        if (xch && xch->t > 0) {
-            const size_t Gt = (size_t)xch->G * xch->t;
+    for (int c0 = 0; c0 < S; c0 += PB) {
+        float in[PB][E];
 #pragma unroll
-            for (int r = 0; r < E; ++r)
-                if (rsrc[r] >= 0) in[r] = xch->bt_s[(size_t)c * Gt + (size_t)rsrc[r]];
+        for (int c = 0; c < PB; ++c) esth_load_run<E>(st_tmp + (size_t)(c0 + c) * N + base + le, in[c]);
+        if (recv) {
+            const size_t Gt = (size_t)xch.G * xch.t;
+#pragma unroll
+            for (int c = 0; c < PB; ++c)
+#pragma unroll
+                for (int r = 0; r < E; ++r)
+                    if (rsrc[r] >= 0) in[c][r] = xch.bt_s[(size_t)(c0 + c) * Gt + (size_t)rsrc[r]];
         }
-        esth_permute_lds<M, E>(in, le, cdf, pick, v);
-        if (go) esth_store_run<E>(st_main + (size_t)c * N + base + le, v);
+#pragma unroll
+        for (int c = 0; c < PB; ++c) {
+            float v[E];
+            esth_permute_lds<M, E>(in[c], le, cdf, pick, v);
+            if (go) esth_store_run<E>(st_main + (size_t)(c0 + c) * N + base + le, v);
+        }
     }
     if (sel && valid) {
 #pragma unroll
@@ -889,7 +912,7 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
     const bool valid = g < G;
     if (!valid) g = G - 1;
     esth_resample_sub<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, nullptr,
-                               cdf_s[threadIdx.x / ESTH_WAVE], &xch);
+                               cdf_s[threadIdx.x / ESTH_WAVE], true, xch);
 }
 
 /* =========================================================== Vose alias
@@ -1106,19 +1129,21 @@ __device__ __forceinline__ uint32_t esth_wave_id() {
     return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE);
 }
 
+/* one wave per chunk of 64 particle pairs of one filter, chunk = (filter,
+ * first pair): sample+weight has no sub-filter structure, so the chunks tile
+ * a filter's particles whatever its sub-filter size, and a wave never mixes
+ * filters (its frame reads stay uniform) */
 template <int MODEL>
 __global__ __launch_bounds__(256) void kb_sample_weight(const esth_bfilter* __restrict__ F,
-                                                        const uint2* __restrict__ sub, uint32_t nsub,
+                                                        const uint2* __restrict__ chunks, uint32_t nchunks,
                                                         const float* __restrict__ R,
                                                         const float* __restrict__ frames, int log_w) {
     const uint32_t wv = esth_wave_id();
-    if (wv >= nsub) return;
-    const uint2 e = sub[wv];
-    const esth_bfilter& f = F[e.x];
-    const uint32_t lane = threadIdx.x & (ESTH_WAVE - 1);
-    const uint32_t base = e.y * f.m;
-    for (uint32_t q = lane; q < f.m / 2; q += ESTH_WAVE)
-        esth_sample_weight_pair<MODEL>(f.state, f.N, R + f.r_off, frames + f.f_off, log_w, f.w, base + 2 * q);
+    if (wv >= nchunks) return;
+    const uint2 c = chunks[wv];
+    const esth_bfilter& f = F[c.x];
+    const uint32_t p = 2u * (c.y + (threadIdx.x & (ESTH_WAVE - 1)));
+    if (p < f.N) esth_sample_weight_pair<MODEL>(f.state, f.N, R + f.r_off, frames + f.f_off, log_w, f.w, p);
 }
 
 template <int LOGM, int S>
@@ -1204,14 +1229,25 @@ template <int LOGM, int S>
 __global__ __launch_bounds__(256) void kb_resample(const esth_bfilter* __restrict__ F,
                                                    const uint2* __restrict__ sub, uint32_t nsub,
                                                    const float* __restrict__ R, float thr, int systematic,
-                                                   int log_w) {
+                                                   int log_w, int topo, uint32_t argmax_blocks) {
     __shared__ float cdf_s[4][ESTH_WAVE * esth_tile<LOGM>::E];
-    const uint32_t wv = esth_wave_id();
+    __shared__ float2 sc[128];
+    __shared__ float smax[4];
+    __shared__ double sd[4 * (2 + S)];
+    if (blockIdx.x < argmax_blocks) {   /* filter blockIdx.x's argmax + posterior, beside the resampling */
+        const esth_bfilter& f = F[blockIdx.x];
+        esth_argmax_block<S, 256>((const float2*)f.winner, f.G, log_w, f.part, f.tmp, f.N, f.m, (float2*)f.result,
+                                  f.est, sc, smax, sd);
+        return;
+    }
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(((blockIdx.x - argmax_blocks) * blockDim.x + threadIdx.x) /
+                                                       ESTH_WAVE);
     const bool valid = wv < nsub;
     const uint2 e = sub[valid ? wv : nsub - 1];
     const esth_bfilter& f = F[e.x];
+    const esth_xch x{f.bt_w, f.bt_s, f.top, f.w, f.G, f.t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, f.dimx, f.dimy};
     esth_resample_sub<LOGM, S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, valid, thr, systematic, log_w,
-                               nullptr, cdf_s[threadIdx.x / ESTH_WAVE]);
+                               nullptr, cdf_s[threadIdx.x / ESTH_WAVE], topo != ESTH_TOPO_NONE, x);
 }
 
 template <int S>

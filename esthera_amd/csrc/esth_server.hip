@@ -164,6 +164,7 @@ struct esth_wave_group {
     int key = 0;
     uint32_t f0 = 0, nf = 0;        /* filters [f0, f0 + nf) of the table */
     uint64_t sub0 = 0, nsub = 0;    /* their sub-filters in sub_all */
+    uint64_t chk0 = 0, nchk = 0;    /* their 64-pair chunks (sample+weight) */
     uint64_t cls0[11] = {0}, ncls[11] = {0};   /* per log2(m) class, in sub_cls */
 };
 
@@ -562,8 +563,9 @@ static void srv_sort(const esth_bfilter* F, const uint2* sub, uint32_t n, int lo
 }
 template <int LOGM, int S>
 static void srv_resample(const esth_bfilter* F, const uint2* sub, uint32_t n, const float* R, float thr, int sys,
-                         int log_w, hipStream_t st) {
-    hipLaunchKernelGGL((kb_resample<LOGM, S>), dim3((n + 3) / 4), dim3(256), 0, st, F, sub, n, R, thr, sys, log_w);
+                         int log_w, int topo, uint32_t argmax_blocks, hipStream_t st) {
+    hipLaunchKernelGGL((kb_resample<LOGM, S>), dim3(argmax_blocks + (n + 3) / 4), dim3(256), 0, st, F, sub, n, R, thr,
+                       sys, log_w, topo, argmax_blocks);
 }
 
 #define ESTH_SRV_LOGM(FN, S, LM, ...)                  \
@@ -578,45 +580,54 @@ static void srv_resample(const esth_bfilter* F, const uint2* sub, uint32_t n, co
 
 template <int S>
 static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wave_group& g, const esth_bfilter* F,
-                         const uint2* sub_all, const uint2* sub_cls, const float* R, const float* frames) {
+                         const uint2* sub_all, const uint2* sub_cls, const uint2* chunks, const float* R,
+                         const float* frames) {
     hipStream_t st = s->stream;
     const int log_w = k.weights == ESTH_WEIGHTS_LOG;
     const esth_bfilter* Fg = F + g.f0;
     const uint32_t nsub = (uint32_t)g.nsub;
     const uint2* sa = sub_all + g.sub0;
-    const int grid_w = (int)(((uint64_t)nsub * ESTH_WAVE + 255) / 256);
+    const uint2* ch = chunks + g.chk0;
+    const uint32_t nch = (uint32_t)g.nchk;
+    const int grid_w = (int)(((uint64_t)nch * ESTH_WAVE + 255) / 256);
     switch (k.model) {
     case ESTH_MODEL_ROBOTARM5:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, st, Fg, sa, nsub, R, frames, log_w);
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
         break;
     case ESTH_MODEL_UNICYCLE:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, st, Fg, sa, nsub, R, frames, log_w);
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
         break;
     case ESTH_MODEL_UNICYCLE10:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, st, Fg, sa, nsub, R, frames, log_w);
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
         break;
     default:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, st, Fg, sa, nsub, R, frames, log_w);
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
         break;
     }
     for (int lm = 6; lm <= 10; ++lm)
         if (g.ncls[lm]) ESTH_SRV_LOGM(srv_sort, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], log_w, st);
-    hipLaunchKernelGGL(kb_argmax<S>, dim3(g.nf), dim3(256), 0, st, Fg, log_w);
-    if (k.t > 0) {
-        if (k.topology == ESTH_TOPO_ALL2ALL) hipLaunchKernelGGL(kb_topt, dim3(g.nf), dim3(1024), 0, st, Fg);
-        const uint32_t nslots = (uint32_t)((k.topology == ESTH_TOPO_TORUS ? 4 : 1) * k.t);
-        const uint64_t total = (uint64_t)nsub * nslots;
-        hipLaunchKernelGGL(kb_exchange, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, Fg, sa, nsub, S, nslots,
-                           k.topology);
+    if (k.t > 0 && k.topology == ESTH_TOPO_ALL2ALL) hipLaunchKernelGGL(kb_topt, dim3(g.nf), dim3(1024), 0, st, Fg);
+    if (k.resampler == ESTH_RESAMPLE_ALIAS) {
+        hipLaunchKernelGGL(kb_argmax<S>, dim3(g.nf), dim3(256), 0, st, Fg, log_w);
+        if (k.t > 0) {
+            const uint32_t nslots = (uint32_t)((k.topology == ESTH_TOPO_TORUS ? 4 : 1) * k.t);
+            const uint64_t total = (uint64_t)nsub * nslots;
+            hipLaunchKernelGGL(kb_exchange, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, Fg, sa, nsub, S,
+                               nslots, k.topology);
+        }
+        for (int lm = 6; lm <= 10; ++lm)
+            if (g.ncls[lm])
+                hipLaunchKernelGGL(kb_alias<S>, dim3((unsigned)g.ncls[lm]), dim3(1u << lm), 0, st, Fg,
+                                   sub_cls + g.cls0[lm], R, k.thr, k.adaptive, log_w);
+        return ESTH_OK;
     }
+    /* exchange inside the resampling, argmax in the first resampling launch */
+    uint32_t am = g.nf;
     for (int lm = 6; lm <= 10; ++lm) {
         if (!g.ncls[lm]) continue;
-        if (k.resampler == ESTH_RESAMPLE_ALIAS)
-            hipLaunchKernelGGL(kb_alias<S>, dim3((unsigned)g.ncls[lm]), dim3(1u << lm), 0, st, Fg, sub_cls + g.cls0[lm], R,
-                               k.thr, k.adaptive, log_w);
-        else
-            ESTH_SRV_LOGM(srv_resample, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], R, k.thr,
-                          k.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0, log_w, st);
+        ESTH_SRV_LOGM(srv_resample, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], R, k.thr,
+                      k.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0, log_w, k.topology, am, st);
+        am = 0;
     }
     return ESTH_OK;
 }
@@ -662,17 +673,19 @@ static int launch_wave(esth_server* s, bool* launched) {
         return a.t->group < b.t->group;
     });
     const size_t nw = upd.size();
-    uint64_t words = 0, nsub = 0, nstreams = 0, frames = 0;
+    uint64_t words = 0, nsub = 0, nstreams = 0, frames = 0, nchk = 0;
     for (auto& u : upd) {
         words += u.t->words;
         nsub += u.t->d.num_blocks;
+        nchk += (u.t->bf.N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
         nstreams += (uint64_t)u.t->K;
         frames += u.t->frame_floats;
     }
     const size_t o_F = 0;
     const size_t o_all = align_up(o_F + sizeof(esth_bfilter) * nw, ESTH_SRV_ALIGN);
     const size_t o_cls = align_up(o_all + sizeof(uint2) * nsub, ESTH_SRV_ALIGN);
-    const size_t o_str = align_up(o_cls + sizeof(uint2) * nsub, ESTH_SRV_ALIGN);
+    const size_t o_chk = align_up(o_cls + sizeof(uint2) * nsub, ESTH_SRV_ALIGN);
+    const size_t o_str = align_up(o_chk + sizeof(uint2) * nchk, ESTH_SRV_ALIGN);
     const size_t o_frm = align_up(o_str + sizeof(esth_mtgp_stream) * nstreams, ESTH_SRV_ALIGN);
     const size_t o_ini = align_up(o_frm + sizeof(float) * frames, ESTH_SRV_ALIGN);
     const size_t total = align_up(o_ini + sizeof(esth_sinit) * inits.size(), ESTH_SRV_ALIGN);
@@ -702,12 +715,13 @@ static int launch_wave(esth_server* s, bool* launched) {
     esth_bfilter* hF = (esth_bfilter*)(h + o_F);
     uint2* hall = (uint2*)(h + o_all);
     uint2* hcls = (uint2*)(h + o_cls);
+    uint2* hchk = (uint2*)(h + o_chk);
     esth_mtgp_stream* hst = (esth_mtgp_stream*)(h + o_str);
     float* hfr = (float*)(h + o_frm);
     esth_sinit* hin = (esth_sinit*)(h + o_ini);
 
     std::vector<esth_wave_group> groups;
-    uint64_t r_off = 0, f_off = 0, sub_pos = 0, str_pos = 0;
+    uint64_t r_off = 0, f_off = 0, sub_pos = 0, str_pos = 0, chk_pos = 0;
     for (size_t i = 0; i < nw; ++i) {
         esth_strack* t = upd[i].t;
         if (groups.empty() || groups.back().key != t->group) {
@@ -715,6 +729,7 @@ static int launch_wave(esth_server* s, bool* launched) {
             g.key = t->group;
             g.f0 = (uint32_t)i;
             g.sub0 = sub_pos;
+            g.chk0 = chk_pos;
             groups.push_back(g);
         }
         esth_wave_group& g = groups.back();
@@ -727,6 +742,8 @@ static int launch_wave(esth_server* s, bool* launched) {
         const uint32_t li = (uint32_t)(i - g.f0);
         for (uint32_t b = 0; b < t->d.num_blocks; ++b) hall[sub_pos + b] = make_uint2(li, b);
         sub_pos += t->d.num_blocks;
+        for (uint32_t q = 0; q < t->bf.N / 2; q += ESTH_WAVE) hchk[chk_pos++] = make_uint2(li, q);
+        g.nchk = chk_pos - g.chk0;
         g.nf++;
         g.nsub += t->d.num_blocks;
         g.ncls[t->logm] += t->d.num_blocks;
@@ -788,12 +805,13 @@ static int launch_wave(esth_server* s, bool* launched) {
         const esth_sgroup_key& k = s->keys[g.key];
         const uint2* sa = (const uint2*)(dt + o_all);
         const uint2* sc = (const uint2*)(dt + o_cls);
+        const uint2* ck = (const uint2*)(dt + o_chk);
         const float* fr = (const float*)(dt + o_frm);
         int rc;
         switch (upd[g.f0].t->S) {
-        case ESTH_UNI_S: rc = group_kernels<ESTH_UNI_S>(s, k, g, dF, sa, sc, s->R[p], fr); break;
-        case ESTH_RA5_S: rc = group_kernels<ESTH_RA5_S>(s, k, g, dF, sa, sc, s->R[p], fr); break;
-        case ESTH_DH6_S: rc = group_kernels<ESTH_DH6_S>(s, k, g, dF, sa, sc, s->R[p], fr); break;
+        case ESTH_UNI_S: rc = group_kernels<ESTH_UNI_S>(s, k, g, dF, sa, sc, ck, s->R[p], fr); break;
+        case ESTH_RA5_S: rc = group_kernels<ESTH_RA5_S>(s, k, g, dF, sa, sc, ck, s->R[p], fr); break;
+        case ESTH_DH6_S: rc = group_kernels<ESTH_DH6_S>(s, k, g, dF, sa, sc, ck, s->R[p], fr); break;
         default: rc = esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension");
         }
         ESTH_TRY(rc);
