@@ -34,7 +34,8 @@ class FilterDesc(C.Structure):
     _fields_ = [("model", C.c_int), ("num_particles", C.c_uint32), ("num_blocks", C.c_uint32),
                 ("num_transfer", C.c_uint32), ("topology", C.c_int), ("torus_dimx", C.c_int),
                 ("torus_dimy", C.c_int), ("resampler", C.c_int), ("resampling_threshold", C.c_float),
-                ("adaptive", C.c_int), ("weights", C.c_int), ("rng", C.c_int), ("seed", C.c_uint32)]
+                ("adaptive", C.c_int), ("weights", C.c_int), ("rng", C.c_int), ("seed", C.c_uint32),
+                ("carry_weights", C.c_int)]
 
 
 class Estimate(C.Structure):
@@ -325,12 +326,12 @@ class Engine:
 
 
 def filter_desc(model, m=256, G=16, t=1, topology=TOPO_RING, resampler=RESAMPLE_MULTINOMIAL, threshold=1.0,
-                adaptive=0, weights=WEIGHTS_LINEAR, rng=RNG_MTGP_COMPAT, seed=1234, dimx=0, dimy=0):
+                adaptive=0, weights=WEIGHTS_LINEAR, rng=RNG_MTGP_COMPAT, seed=1234, dimx=0, dimy=0, carry=0):
     d = FilterDesc()
     d.model, d.num_particles, d.num_blocks, d.num_transfer = model, m, G, t
     d.topology, d.torus_dimx, d.torus_dimy = topology, dimx, dimy
     d.resampler, d.resampling_threshold, d.adaptive = resampler, threshold, adaptive
-    d.weights, d.rng, d.seed = weights, rng, seed
+    d.weights, d.rng, d.seed, d.carry_weights = weights, rng, seed, carry
     return d
 
 

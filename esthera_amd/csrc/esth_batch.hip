@@ -233,6 +233,7 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
         if (f.dimx <= 0) f.dimx = 1;
         if (f.dimy <= 0) f.dimy = 1;
         f.uoff = (uint32_t)(N * (uint64_t)V);
+        f.flags = d.carry_weights ? 1u : 0u;
         f.r_off = b->words;
         b->Ntot += N;
         b->words += b->rsize[i] * (uint64_t)b->K[i];
@@ -350,6 +351,12 @@ extern "C" int esth_batch_init_uniform(esth_batch* b, const float* states) {
                       states[(size_t)i * b->S + c]);
     }
     ESTH_HIP(hipMemcpy(b->state, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    /* carried weights restart equal (0 in log mode, 1 linear) */
+    std::vector<float> w0((size_t)b->Ntot, 0.0f);
+    for (int i = 0; i < b->n; ++i)
+        if (b->descs[i].carry_weights && b->d.weights != ESTH_WEIGHTS_LOG)
+            std::fill(w0.begin() + b->woff[i], w0.begin() + b->woff[i] + b->hf[i].N, 1.0f);
+    ESTH_HIP(hipMemcpy(b->w, w0.data(), sizeof(float) * w0.size(), hipMemcpyHostToDevice));
     return ESTH_OK;
 }
 

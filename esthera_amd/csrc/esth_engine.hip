@@ -240,20 +240,20 @@ int esth_launch_mtgp(const uint32_t* d_params, const uint32_t* d_status_in, uint
 }
 
 int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame, int log_w,
-                              float* w, hipStream_t s) {
+                              float* w, int carry, hipStream_t s) {
     const int grid = grid_for(N / 2, 256);
     switch (model) {
     case ESTH_MODEL_ROBOTARM5:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
         break;
     case ESTH_MODEL_UNICYCLE:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
         break;
     case ESTH_MODEL_DH6:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_DH6>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_DH6>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
         break;
     case ESTH_MODEL_UNICYCLE10:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w);
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
         break;
     default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
     }
@@ -347,14 +347,14 @@ int esth_launch_exchange(float* st, float* w, int S, uint32_t N, uint32_t m, uin
 }
 
 int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w, const float* rnd,
-                         uint32_t N, uint32_t m, uint32_t G, float thr, int adaptive, int log_w, int* sel,
+                         uint32_t N, uint32_t m, uint32_t G, float thr, int adaptive, int log_w, int* sel, int carry,
                          hipStream_t s) {
     if (resampler == ESTH_RESAMPLE_ALIAS) {
         switch (S) {
-        case 1: hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr); break;
-        case ESTH_UNI_S: hipLaunchKernelGGL(k_alias<ESTH_UNI_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr); break;
-        case ESTH_RA5_S: hipLaunchKernelGGL(k_alias<ESTH_RA5_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr); break;
-        case ESTH_DH6_S: hipLaunchKernelGGL(k_alias<ESTH_DH6_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr); break;
+        case 1: hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry); break;
+        case ESTH_UNI_S: hipLaunchKernelGGL(k_alias<ESTH_UNI_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry); break;
+        case ESTH_RA5_S: hipLaunchKernelGGL(k_alias<ESTH_RA5_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry); break;
+        case ESTH_DH6_S: hipLaunchKernelGGL(k_alias<ESTH_DH6_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry); break;
         default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
         }
         ESTH_HIP(hipGetLastError());
@@ -377,19 +377,20 @@ template <int LOGM, int S>
 static void launch_resample_argmax_t(float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
                                      uint32_t G, float thr, int systematic, int log_w, const float2* winner,
                                      const float* part, const float* best_src, size_t plane_stride, uint32_t g_stride,
-                                     float2* result, double* est, const esth_xch& xch, hipStream_t s) {
+                                     float2* result, double* est, const esth_xch& xch, int carry, hipStream_t s) {
     using TL = esth_tile<LOGM>;
     const uint64_t waves = (G + (ESTH_WAVE / TL::LPS) - 1) / (ESTH_WAVE / TL::LPS);
     hipLaunchKernelGGL((k_resample_argmax<LOGM, S>), dim3(1 + grid_for(waves * ESTH_WAVE, 256)), dim3(256), 0, s,
                        st_main, st_tmp, w, rnd, N, G, thr, systematic, log_w, winner, part, best_src, plane_stride,
-                       g_stride, result, est, xch);
+                       g_stride, result, est, xch, carry);
 }
 
 int esth_launch_resample_argmax(int resampler, int S, float* st_main, const float* st_tmp, float* w,
                                 const float* rnd, uint32_t N, uint32_t m, uint32_t G, float thr, int log_w,
                                 const float2* winner, const float* part, const float* best_src, size_t plane_stride,
                                 uint32_t g_stride, float2* result, double* est, uint32_t t, int topo, int dimx,
-                                int dimy, const float* bt_w, const float* bt_s, uint32_t* top, hipStream_t s) {
+                                int dimy, const float* bt_w, const float* bt_s, uint32_t* top, int carry,
+                                hipStream_t s) {
     if (resampler == ESTH_RESAMPLE_ALIAS) return esth_fail(ESTH_ERR_UNSUPPORTED, "fused argmax: scan resamplers only");
     const int lm = log2u(m);
     const int sys = resampler == ESTH_RESAMPLE_SYSTEMATIC;
@@ -401,7 +402,7 @@ int esth_launch_resample_argmax(int resampler, int S, float* st_main, const floa
         }
         xch = {bt_w, bt_s, top, w, G, t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, dimx, dimy};
     }
-#define ESTH_RA_ARGS st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, winner, part, best_src, plane_stride, g_stride, result, est, xch, s
+#define ESTH_RA_ARGS st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, winner, part, best_src, plane_stride, g_stride, result, est, xch, carry, s
     switch (S) {
     case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_UNI_S, ESTH_RA_ARGS); break;
     case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_RA5_S, ESTH_RA_ARGS); break;
@@ -460,7 +461,7 @@ extern "C" int esth_stage_sample_weight(int model, float* d_state, uint32_t N, c
     float* dfr = nullptr;
     ESTH_HIP(hipMalloc((void**)&dfr, sizeof(float) * ESTH_FRAME_FLOATS));
     ESTH_HIP(hipMemcpy(dfr, frame, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice));
-    int rc = esth_launch_sample_weight(model, d_state, N, d_random, dfr, weights == ESTH_WEIGHTS_LOG, d_w, 0);
+    int rc = esth_launch_sample_weight(model, d_state, N, d_random, dfr, weights == ESTH_WEIGHTS_LOG, d_w, 0, 0);
     ESTH_HIP(hipDeviceSynchronize());
     hipFree(dfr);
     return rc;
@@ -538,7 +539,7 @@ extern "C" int esth_stage_resample(int resampler, float* dst, const float* src, 
     int s_, v_;
     ESTH_TRY(esth_check_desc(&d, &s_, &v_));
     int rc = esth_launch_resample(resampler, S, dst, src, d_w, d_random, m * G, m, G, thr, adaptive,
-                                  weights == ESTH_WEIGHTS_LOG, d_sel, 0);
+                                  weights == ESTH_WEIGHTS_LOG, d_sel, 0, 0);
     ESTH_HIP(hipDeviceSynchronize());
     return rc;
 }
@@ -547,7 +548,7 @@ extern "C" int esth_stage_alias_table(const float* d_w, uint32_t m, uint32_t G, 
     if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES) return esth_fail(ESTH_ERR_INVALID, "m=%u", m);
     if (!d_prob || !d_alias) return esth_fail(ESTH_ERR_INVALID, "null output");
     hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, 0, (float*)nullptr, (const float*)nullptr, d_w,
-                       (const float*)nullptr, m * G, m, 1.0f, 0, weights == ESTH_WEIGHTS_LOG, (int*)nullptr, d_prob, d_alias);
+                       (const float*)nullptr, m * G, m, 1.0f, 0, weights == ESTH_WEIGHTS_LOG, (int*)nullptr, d_prob, d_alias, 0);
     ESTH_HIP(hipGetLastError());
     ESTH_HIP(hipDeviceSynchronize());
     return ESTH_OK;

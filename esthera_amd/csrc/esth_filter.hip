@@ -187,6 +187,10 @@ extern "C" int esth_filter_init(esth_filter* f, const float* soa) {
     ESTH_HIP(hipSetDevice(f->e->device));
     ESTH_HIP(hipStreamSynchronize(f->stream));
     ESTH_HIP(hipMemcpy(f->state, soa, sizeof(float) * (size_t)f->S * f->N, hipMemcpyHostToDevice));
+    if (f->d.carry_weights) {   /* a new particle set starts from equal weights */
+        std::vector<float> w0(f->N, f->d.weights == ESTH_WEIGHTS_LOG ? 0.0f : 1.0f);
+        ESTH_HIP(hipMemcpy(f->w, w0.data(), sizeof(float) * f->N, hipMemcpyHostToDevice));
+    }
     return ESTH_OK;
 }
 
@@ -254,7 +258,7 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
     float* R = f->R[p];
     if (timing) ESTH_HIP(hipEventRecord(ev[1], s));
     /* sampling_importance with Box-Muller fused (pf_generic.c:521-533) */
-    ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, R, d_frame, log_w, f->w, s));
+    ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, R, d_frame, log_w, f->w, d.carry_weights, s));
     if (timing) ESTH_HIP(hipEventRecord(ev[2], s));
     /* block_sort (pf_generic.c:572-585) */
     ESTH_TRY(esth_launch_sort(f->S, f->m, f->w, f->state, f->tmp, N, f->G, f->winner, nullptr, f->t, f->bt_w,
@@ -280,10 +284,10 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
                                              f->G, d.resampling_threshold, log_w, f->winner, f->part,
                                              bt ? f->bt_s : f->tmp, bt ? (size_t)f->G * f->t : (size_t)N,
                                              bt ? f->t : f->m, f->result, f->est, f->t, d.topology, f->dimx,
-                                             f->dimy, f->bt_w, f->bt_s, f->top, s));
+                                             f->dimy, f->bt_w, f->bt_s, f->top, d.carry_weights, s));
     } else {
         ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m, f->G,
-                                      d.resampling_threshold, d.adaptive, log_w, nullptr, s));
+                                      d.resampling_threshold, d.adaptive, log_w, nullptr, d.carry_weights, s));
     }
     if (all) ESTH_HIP(hipEventRecord(ev[6], s));
     return ESTH_OK;

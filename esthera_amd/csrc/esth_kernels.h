@@ -211,11 +211,13 @@ __device__ __forceinline__ void esth_sample_pair_regs(float (&s0)[esth_model_tra
     }
 }
 
+/* carry: multiply the particles' carried weights by the likelihoods (add the
+ * log-likelihoods in log mode) instead of overwriting them */
 template <int MODEL>
 __device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ state, uint32_t N,
                                                         const float* __restrict__ R,
                                                         const float* __restrict__ frame, int log_w,
-                                                        float* __restrict__ w, uint32_t p) {
+                                                        float* __restrict__ w, uint32_t p, int carry = 0) {
     using T = esth_model_traits<MODEL>;
     float s0[T::S], s1[T::S], u0[T::V], u1[T::V];
 #pragma unroll
@@ -236,6 +238,10 @@ __device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ stat
     for (int c = 0; c < T::S; ++c)
         *reinterpret_cast<float2*>(state + (size_t)c * N + p) = make_float2(s0[c], s1[c]);
     float2 wo = log_w ? make_float2(lw0, lw1) : make_float2(esth_expf(lw0), esth_expf(lw1));
+    if (carry) {
+        const float2 wp = *reinterpret_cast<const float2*>(w + p);
+        wo = log_w ? make_float2(wp.x + wo.x, wp.y + wo.y) : make_float2(wp.x * wo.x, wp.y * wo.y);
+    }
     *reinterpret_cast<float2*>(w + p) = wo;
 }
 
@@ -243,9 +249,9 @@ template <int MODEL>
 __global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state, uint32_t N,
                                                        const float* __restrict__ R,
                                                        const float* __restrict__ frame, int log_w,
-                                                       float* __restrict__ w) {
+                                                       float* __restrict__ w, int carry) {
     const uint32_t p = 2u * (blockIdx.x * blockDim.x + threadIdx.x);
-    if (p < N) esth_sample_weight_pair<MODEL>(state, N, R, frame, log_w, w, p);
+    if (p < N) esth_sample_weight_pair<MODEL>(state, N, R, frame, log_w, w, p, carry);
 }
 
 /* ============================================================ block sort
@@ -744,7 +750,8 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                                                   const float* __restrict__ rnd, uint32_t N, uint32_t g,
                                                   bool valid, float thr, int systematic, int log_w,
                                                   int* __restrict__ sel, float* cdf_w,
-                                                  bool recv = false, esth_xch xch = esth_xch{}) {
+                                                  bool recv = false, esth_xch xch = esth_xch{},
+                                                  int carry = 0, float* w_mut = nullptr) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS, LOGE = TL::LOGE, LOGL = TL::LOGL;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
@@ -854,7 +861,15 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
             float v[E];
             esth_permute_lds<M, E>(in[c], le, cdf, pick, v);
             if (go) esth_store_run<E>(st_main + (size_t)(c0 + c) * N + base + le, v);
+            /* carried weights: a skipped sub-filter keeps its sorted, exchanged particles */
+            else if (carry && valid) esth_store_run<E>(st_main + (size_t)(c0 + c) * N + base + le, in[c]);
         }
+    }
+    if (carry && go) {   /* resampled: equal weights again */
+        float w0[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) w0[r] = log_w ? 0.0f : 1.0f;
+        esth_store_run<E>(w_mut + base + le, w0);
     }
     if (sel && valid) {
 #pragma unroll
@@ -896,7 +911,7 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
                                                          const float* __restrict__ part,
                                                          const float* __restrict__ best_src, size_t plane_stride,
                                                          uint32_t g_stride, float2* __restrict__ result,
-                                                         double* __restrict__ est, esth_xch xch) {
+                                                         double* __restrict__ est, esth_xch xch, int carry) {
     using TL = esth_tile<LOGM>;
     __shared__ float cdf_s[4][ESTH_WAVE * TL::E];
     __shared__ float2 sc[128];
@@ -912,7 +927,7 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
     const bool valid = g < G;
     if (!valid) g = G - 1;
     esth_resample_sub<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, nullptr,
-                               cdf_s[threadIdx.x / ESTH_WAVE], true, xch);
+                               cdf_s[threadIdx.x / ESTH_WAVE], true, xch, carry, const_cast<float*>(w));
 }
 
 /* =========================================================== Vose alias
@@ -956,13 +971,15 @@ __device__ __forceinline__ uint32_t esth_block_rank(bool pred, uint32_t* cnt, in
     return before + r;
 }
 
-/* one work-group of M work-items on sub-filter g */
+/* one work-group of M work-items on sub-filter g; carry: a skipped
+ * sub-filter keeps its sorted, exchanged particles (and weights), a
+ * resampled one restarts from equal weights */
 template <int S>
 __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, const float* __restrict__ st_tmp,
-                                               const float* __restrict__ w, const float* __restrict__ rnd,
+                                               const float* w, const float* __restrict__ rnd,
                                                uint32_t N, uint32_t M, uint32_t g, float thr, int adaptive,
                                                int log_w, int* __restrict__ sel, float* __restrict__ prob_out,
-                                               uint32_t* __restrict__ alias_out) {
+                                               uint32_t* __restrict__ alias_out, int carry = 0) {
     __shared__ float prob[ESTH_MAX_NUM_PARTICLES];
     __shared__ uint32_t alias[ESTH_MAX_NUM_PARTICLES];
     __shared__ uint32_t sl[ESTH_MAX_NUM_PARTICLES];
@@ -976,6 +993,9 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
     if (!table_only && !adaptive) {
         if ((rnd[N + g] - 1.0f) > thr) {
             if (sel) sel[idx] = -1;
+#pragma unroll
+            for (int c = 0; c < S; ++c)
+                if (carry) st_main[(size_t)c * N + idx] = st_tmp[(size_t)c * N + idx];
             return;
         }
     }
@@ -1005,6 +1025,9 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
         float recip_ess = esth_sum_reduce4(scratch, M, l);
         if (!(recip_ess > thr)) {
             if (sel) sel[idx] = -1;
+#pragma unroll
+            for (int c = 0; c < S; ++c)
+                if (carry) st_main[(size_t)c * N + idx] = st_tmp[(size_t)c * N + idx];
             return;
         }
     }
@@ -1105,17 +1128,20 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
 #pragma unroll
     for (int c = 0; c < S; ++c) st_main[(size_t)c * N + idx] = st_tmp[(size_t)c * N + sb + pick];
     if (sel) sel[idx] = (int)pick;
+    /* every item has read its weight (before the first barrier) */
+    if (carry) const_cast<float*>(w)[idx] = log_w ? 0.0f : 1.0f;
 }
 
 template <int S>
 __global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
                                                 const float* __restrict__ st_tmp,
-                                                const float* __restrict__ w,
+                                                const float* w,
                                                 const float* __restrict__ rnd, uint32_t N, uint32_t M,
                                                 float thr, int adaptive, int log_w,
                                                 int* __restrict__ sel, float* __restrict__ prob_out,
-                                                uint32_t* __restrict__ alias_out) {
-    esth_alias_sub<S>(st_main, st_tmp, w, rnd, N, M, blockIdx.x, thr, adaptive, log_w, sel, prob_out, alias_out);
+                                                uint32_t* __restrict__ alias_out, int carry) {
+    esth_alias_sub<S>(st_main, st_tmp, w, rnd, N, M, blockIdx.x, thr, adaptive, log_w, sel, prob_out, alias_out,
+                      carry);
 }
 
 
@@ -1143,7 +1169,8 @@ __global__ __launch_bounds__(256) void kb_sample_weight(const esth_bfilter* __re
     const uint2 c = chunks[wv];
     const esth_bfilter& f = F[c.x];
     const uint32_t p = 2u * (c.y + (threadIdx.x & (ESTH_WAVE - 1)));
-    if (p < f.N) esth_sample_weight_pair<MODEL>(f.state, f.N, R + f.r_off, frames + f.f_off, log_w, f.w, p);
+    if (p < f.N)
+        esth_sample_weight_pair<MODEL>(f.state, f.N, R + f.r_off, frames + f.f_off, log_w, f.w, p, (int)(f.flags & 1u));
 }
 
 template <int LOGM, int S>
@@ -1247,7 +1274,8 @@ __global__ __launch_bounds__(256) void kb_resample(const esth_bfilter* __restric
     const esth_bfilter& f = F[e.x];
     const esth_xch x{f.bt_w, f.bt_s, f.top, f.w, f.G, f.t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, f.dimx, f.dimy};
     esth_resample_sub<LOGM, S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, valid, thr, systematic, log_w,
-                               nullptr, cdf_s[threadIdx.x / ESTH_WAVE], topo != ESTH_TOPO_NONE, x);
+                               nullptr, cdf_s[threadIdx.x / ESTH_WAVE], topo != ESTH_TOPO_NONE, x,
+                               (int)(f.flags & 1u), f.w);
 }
 
 template <int S>
@@ -1256,12 +1284,12 @@ __global__ __launch_bounds__(1024) void kb_alias(const esth_bfilter* __restrict_
     const uint2 e = sub[blockIdx.x];
     const esth_bfilter& f = F[e.x];
     esth_alias_sub<S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, f.m, e.y, thr, adaptive, log_w, nullptr,
-                      nullptr, nullptr);
+                      nullptr, nullptr, (int)(f.flags & 1u));
 }
 
 /* ============================================================== serving
  * Start-up of tracks admitted in a wave: the state planes set to the initial
- * state, the weights to zero and the track's K MTGP32 statuses seeded on the
+ * state, the weights to w0 and the track's K MTGP32 statuses seeded on the
  * device as mtgp32_init_state (mtgp32-fast.c:119-137) does on the host, with
  * stream j of parameter set j seeded seed + j + 1 (mtgp32-opencl.c:47).
  * grid (blocks per track, tracks). */
@@ -1270,6 +1298,7 @@ struct esth_sinit {
     float* w;
     uint32_t* status;   /* K consecutive 351-word statuses */
     uint32_t N, S, K, seed;
+    float w0;           /* initial weight: 0, or 1 for carried linear weights */
     float init[ESTH_MAX_STATE_DIM];
 };
 
@@ -1303,7 +1332,7 @@ static __global__ __launch_bounds__(256) void ks_init(const esth_sinit* __restri
         float* pl = r.state + (size_t)c * r.N;
         for (uint32_t i = i0; i < r.N; i += stride) pl[i] = v;
     }
-    for (uint32_t i = i0; i < r.N; i += stride) r.w[i] = 0.0f;
+    for (uint32_t i = i0; i < r.N; i += stride) r.w[i] = r.w0;
     if (blockIdx.x == 0)
         for (uint32_t j = threadIdx.x; j < r.K; j += blockDim.x)
             esth_mtgp_seed_dev(params + (size_t)j * ESTH_MTGP_PARAM_WORDS, r.seed + j + 1u,

@@ -68,7 +68,7 @@ struct esth_bfilter {
     uint32_t N, G, m, t;
     int32_t dimx, dimy;
     uint32_t uoff;     /* N * V: first uniform of the step's words */
-    uint32_t pad;
+    uint32_t flags;    /* bit 0: carried weights */
 };
 
 /* one MTGP32 stream of a launch */

@@ -319,6 +319,7 @@ static uint64_t track_layout(esth_strack* t, uint64_t off, bool assign, unsigned
         if (f.dimx <= 0) f.dimx = 1;
         if (f.dimy <= 0) f.dimy = 1;
         f.uoff = (uint32_t)(N * (uint64_t)t->V);
+        f.flags = d.carry_weights ? 1u : 0u;
         t->soff = (off + o_st) / sw;
     }
     return p;
@@ -778,6 +779,7 @@ static int launch_wave(esth_server* s, bool* launched) {
         r.S = (uint32_t)t->S;
         r.K = (uint32_t)t->K;
         r.seed = t->d.seed;
+        r.w0 = (t->d.carry_weights && t->d.weights != ESTH_WEIGHTS_LOG) ? 1.0f : 0.0f;
         memcpy(r.init, t->init, sizeof(float) * (size_t)t->S);
         t->need_init = false;
     }

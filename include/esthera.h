@@ -164,6 +164,13 @@ typedef struct {
     int      weights;              /* ESTH_WEIGHTS_* */
     int      rng;                  /* ESTH_RNG_* */
     uint32_t seed;                 /* MTGP seed; stream j is seeded with seed + j + 1 */
+    int      carry_weights;        /* 1: a sub-filter that is not resampled keeps its sorted,
+                                      exchanged particles with their weights, and the next step
+                                      multiplies each weight by the new likelihood (adds in log
+                                      mode); a resampled sub-filter restarts from equal weights
+                                      (the TODO of resampling-valias.cl:302-303).  0: the
+                                      reference (weights restart every step; a skipped sub-filter
+                                      keeps its unsorted pre-exchange state, pf_generic.cl:403-404) */
 } esth_filter_desc;
 
 /* Fill a descriptor with the reference's defaults (pf_generic.c:1072-1084):

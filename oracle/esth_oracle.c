@@ -703,13 +703,21 @@ void or_filter_destroy(or_filter* f) {
     free(f);
 }
 
+/* with carried weights a new particle set starts from equal weights */
+static void reset_weights(or_filter* f) {
+    const float w0 = (f->d.carry_weights && f->d.weights != ESTH_WEIGHTS_LOG) ? 1.0f : 0.0f;
+    for (size_t i = 0; i < f->N; ++i) f->w[i] = w0;
+}
+
 void or_filter_init_uniform(or_filter* f, const float* s) {
     for (int c = 0; c < f->S; ++c)
         for (size_t i = 0; i < f->N; ++i) f->state[(size_t)c * f->N + i] = s[c];
+    reset_weights(f);
 }
 
 void or_filter_set_state(or_filter* f, const float* soa) {
     memcpy(f->state, soa, sizeof(float) * (size_t)f->S * f->N);
+    reset_weights(f);
 }
 
 static void posterior_f64(const double* st, const double* lw, size_t N, int S, double* mean) {
@@ -733,17 +741,30 @@ int or_filter_step(or_filter* f, const void* ctrl, const void* meas, float dt) {
     /* generate_normal_random, pf_generic.c:191-232 */
     or_mtgp_fill(f->params, f->K, 0, f->status, (int)f->size, (uint32_t*)f->R, 1);
     or_boxmuller(f->R, (int64_t)N * f->V);
+    const int carry = d->carry_weights != 0;
+    const int logw = d->weights == ESTH_WEIGHTS_LOG;
+    float* w_prev = NULL;   /* carried weights, aligned with the particles */
+    if (carry) {
+        w_prev = (float*)malloc(sizeof(float) * N);
+        memcpy(w_prev, f->w, sizeof(float) * N);
+    }
     /* FP64 twin from the same prior and normals (lock-step posterior) */
     if (f->f64) {
     double* pre = f->st64;
     float* prior = (float*)malloc(sizeof(float) * (size_t)f->S * N);
     memcpy(prior, f->state, sizeof(float) * (size_t)f->S * N);
     or_sample_weight_f64(d->model, prior, (uint32_t)N, f->R, f->mparams, ctrl, meas, dt, pre, f->lw64);
+    if (carry)
+        for (size_t i = 0; i < N; ++i) f->lw64[i] += logw ? (double)w_prev[i] : log((double)w_prev[i]);
     posterior_f64(pre, f->lw64, N, f->S, f->mean64);
     free(prior);
     }
     /* sampling_importance, pf_generic.c:521-533 */
     or_sample_weight(d->model, f->state, (uint32_t)N, f->R, frame, d->weights, f->w);
+    if (carry) {   /* w = w_prev * likelihood (w_prev + log-likelihood) */
+        for (size_t i = 0; i < N; ++i) f->w[i] = logw ? w_prev[i] + f->w[i] : w_prev[i] * f->w[i];
+        free(w_prev);
+    }
     /* FP64 posterior of the FP32 path's particles and weights */
     if (f->f64) {
         double* lw = f->lw64;
@@ -764,9 +785,24 @@ int or_filter_step(or_filter* f, const void* ctrl, const void* meas, float dt) {
         if (or_exchange(f->tmp, f->w, f->S, m, G, d->num_transfer, d->topology, f->dimx, f->dimy))
             return -1;
     /* resampling, pf_generic.c:732-757, uniforms at rand_offset = Nn */
+    int* sel = carry ? (int*)malloc(sizeof(int) * N) : NULL;
     if (or_resample(d->resampler, f->state, f->tmp, f->S, f->w, f->R + N * f->V, m, G,
-                    d->resampling_threshold, d->adaptive, d->weights, NULL))
+                    d->resampling_threshold, d->adaptive, d->weights, sel)) {
+        free(sel);
         return -1;
+    }
+    if (carry) {   /* skipped: keep the sorted, exchanged particles and weights; resampled: equal weights */
+        for (uint32_t g = 0; g < G; ++g) {
+            const size_t base = (size_t)g * m;
+            if (sel[base] < 0) {
+                for (int c = 0; c < f->S; ++c)
+                    memcpy(f->state + (size_t)c * N + base, f->tmp + (size_t)c * N + base, sizeof(float) * m);
+            } else {
+                for (uint32_t l = 0; l < m; ++l) f->w[base + l] = logw ? 0.0f : 1.0f;
+            }
+        }
+        free(sel);
+    }
     return 0;
 }
 

@@ -12,13 +12,15 @@ import esthera_amd as ea
 pytestmark = pytest.mark.gpu
 
 
-def make_filters(nf, seed, sizes, topology=ea.TOPO_RING, t=1, resampler=ea.RESAMPLE_SYSTEMATIC):
+def make_filters(nf, seed, sizes, topology=ea.TOPO_RING, t=1, resampler=ea.RESAMPLE_SYSTEMATIC, threshold=1.0,
+                 carry=0):
     rng = np.random.default_rng(seed)
     descs, params, scen = [], [], []
     for i in range(nf):
         m, G = sizes[i % len(sizes)]
         d = ea.filter_desc(ea.MODEL_UNICYCLE, m=m, G=G, t=t, topology=topology, resampler=resampler,
-                           weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=int(rng.integers(1, 1 << 30)))
+                           threshold=threshold, weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER,
+                           seed=int(rng.integers(1, 1 << 30)), carry=carry)
         p, ctrl, meas, truth = ea.unicycle_scenario(seed * 1000 + i, 8, 20.0, 8)
         descs.append(d)
         params.append(p)
@@ -39,6 +41,7 @@ CASES = [
                                resampler=ea.RESAMPLE_MULTINOMIAL)),
     ("ring-alias", dict(sizes=[(64, 4), (256, 3), (1024, 2)], resampler=ea.RESAMPLE_ALIAS)),
     ("all2all", dict(sizes=[(128, 8), (256, 5)], topology=ea.TOPO_ALL2ALL, t=2)),
+    ("ring-carry-gate", dict(sizes=[(64, 3), (256, 4), (128, 2)], threshold=0.5, carry=1)),
 ]
 
 

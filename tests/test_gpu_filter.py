@@ -74,6 +74,12 @@ CASES = [
                                      adaptive=1, threshold=0.01)),
     ("dh6-torus-log", dict(model=ea.MODEL_DH6, m=512, G=8, t=1, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG)),
     ("unicycle10-reference-linear", dict(model=ea.MODEL_UNICYCLE10, m=128, G=16, t=1)),
+    ("unicycle-carry-gate-log", dict(model=ea.MODEL_UNICYCLE, m=256, G=16, t=1, topology=ea.TOPO_TORUS,
+                                     resampler=ea.RESAMPLE_SYSTEMATIC, threshold=0.5, weights=ea.WEIGHTS_LOG,
+                                     carry=1)),
+    ("robotarm-carry-gate-linear", dict(model=ea.MODEL_ROBOTARM5, m=128, G=8, t=1, threshold=0.4, carry=1)),
+    ("unicycle-alias-adaptive-carry", dict(model=ea.MODEL_UNICYCLE, m=256, G=8, t=2, resampler=ea.RESAMPLE_ALIAS,
+                                           adaptive=1, threshold=0.02, weights=ea.WEIGHTS_LOG, carry=1)),
     ("unicycle10-alias-log", dict(model=ea.MODEL_UNICYCLE10, m=256, G=8, t=2, resampler=ea.RESAMPLE_ALIAS,
                                   weights=ea.WEIGHTS_LOG)),
     ("unicycle-small-m", dict(model=ea.MODEL_UNICYCLE, m=16, G=12, t=2, weights=ea.WEIGHTS_LOG,

@@ -14,9 +14,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GiB = 1 << 30
 
 
-def unicycle_track(seed, m, G, L=8, topology=ea.TOPO_TORUS, t=1, resampler=ea.RESAMPLE_SYSTEMATIC):
+def unicycle_track(seed, m, G, L=8, topology=ea.TOPO_TORUS, t=1, resampler=ea.RESAMPLE_SYSTEMATIC, threshold=1.0,
+                   carry=0):
     d = ea.filter_desc(ea.MODEL_UNICYCLE, m=m, G=G, t=t, topology=topology, resampler=resampler,
-                       weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=seed)
+                       weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=seed, threshold=threshold,
+                       carry=carry)
     p, ctrl, meas, truth = ea.unicycle_scenario(seed, L, 20.0, 40)
     return d, p, [(ctrl[k], meas[k], 0.1) for k in range(40)], np.zeros(3, np.float32)
 
@@ -55,8 +57,9 @@ def test_server_mix_equals_tracks_alone(engine):
     specs = [unicycle_track(11, 256, 16), unicycle_track(12, 64, 40), robotarm_track(13, 128, 8),
              unicycle_track(14, 1024, 4, L=32), unicycle_track(15, 512, 6, topology=ea.TOPO_RING, t=2,
                                                                resampler=ea.RESAMPLE_ALIAS),
-             robotarm_track(16, 256, 3), unicycle_track(17, 128, 9, topology=ea.TOPO_ALL2ALL, t=2)]
-    nsteps = [9, 14, 7, 11, 6, 12, 8]
+             robotarm_track(16, 256, 3), unicycle_track(17, 128, 9, topology=ea.TOPO_ALL2ALL, t=2),
+             unicycle_track(18, 256, 6, threshold=0.5, carry=1)]
+    nsteps = [9, 14, 7, 11, 6, 12, 8, 10]
     srv = ea.Server(engine, 64, GiB)
     tracks = [None] * len(specs)
     sent = [0] * len(specs)

@@ -163,6 +163,58 @@ def test_unicycle10_model_against_fp64_twin():
     np.testing.assert_allclose(s32, [1.0 + np.cos(0.5), 2.0 + np.sin(0.5), 0.5], rtol=1e-6)
 
 
+def _carry_case(threshold, weights):
+    """Zero process noise and one common initial state: every particle follows
+    the same deterministic path, so every weight is the same function of the
+    steps (no sort, exchange or resampling can change any particle)."""
+    import ctypes as C
+    import esthera_amd as ea
+    p = ea.UnicycleParams()
+    p.num_landmarks = 4
+    p.sigma_x = p.sigma_y = p.sigma_theta = 0.0
+    p.sigma_range, p.sigma_bearing = 0.5, 0.1
+    for j, (x, y) in enumerate([(5.0, 1.0), (-3.0, 4.0), (2.0, -6.0), (7.0, 7.0)]):
+        p.lx[j], p.ly[j] = x, y
+    ctrl = ea.UnicycleControl(1.0, 0.1)
+    meas = ea.UnicycleMeasurement()
+    for j in range(4):
+        meas.range[j], meas.bearing[j] = 5.0 + j, 0.3 * j - 0.4
+    d = ea.filter_desc(ea.MODEL_UNICYCLE, m=64, G=4, t=1, topology=ea.TOPO_RING, threshold=threshold,
+                       weights=weights, seed=5, carry=1)
+    f = O.Filter(d, p, C.sizeof(p))
+    f.init_uniform(np.zeros(3, np.float32))
+    # FP64 path of the common particle
+    x = np.zeros(3)
+    lls = []
+    for k in range(4):
+        f.step(ctrl, meas, 0.1)
+        x = np.array([x[0] + 0.1 * np.cos(x[2]), x[1] + 0.1 * np.sin(x[2]), x[2] + 0.01])
+        acc = 0.0
+        for j in range(4):
+            dx, dy = p.lx[j] - x[0], p.ly[j] - x[1]
+            er = (meas.range[j] - np.hypot(dx, dy)) / 0.5
+            eb = np.angle(np.exp(1j * (meas.bearing[j] - (np.arctan2(dy, dx) - x[2])))) / 0.1
+            acc += er * er + eb * eb
+        lls.append(-0.5 * acc)
+    st, w = f.read(3)
+    return np.array(lls), st.reshape(3, -1), w
+
+
+def test_carried_weights_accumulate_without_resampling():
+    lls, st, w = _carry_case(-1.0, 1)     # the gate u - 1 > -1 skips every sub-filter
+    assert np.allclose(st, st[:, :1], atol=0)          # still one common particle
+    np.testing.assert_allclose(w, lls.sum(), rtol=2e-5)
+    lls, st, w = _carry_case(-1.0, 0)     # linear: the product of the likelihoods
+    np.testing.assert_allclose(w, np.exp(lls.sum()), rtol=1e-4)
+
+
+def test_carried_weights_restart_after_resampling():
+    lls, st, w = _carry_case(1.0, 1)      # every sub-filter resampled: weights restart every step
+    np.testing.assert_allclose(w, 0.0, atol=0)
+    lls, st, w = _carry_case(1.0, 0)
+    np.testing.assert_allclose(w, 1.0, atol=0)
+
+
 def test_boxmuller_closed_form():
     rng = np.random.default_rng(1)
     u = (1.0 + rng.random(10000, dtype=np.float32)).astype(np.float32)
