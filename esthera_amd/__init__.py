@@ -18,15 +18,15 @@ LIB_PATH = os.path.join(HERE, "libesthera.so")
 
 # --------------------------------------------------------------- constants
 ESTH_OK = 0
-MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6, MODEL_UNICYCLE10 = 0, 1, 2, 3
+MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6, MODEL_UNICYCLE10, MODEL_ROBOTARM5_CAMERA = 0, 1, 2, 3, 4
 TOPO_NONE, TOPO_RING, TOPO_TORUS, TOPO_ALL2ALL = 0, 1, 2, 3
 RESAMPLE_MULTINOMIAL, RESAMPLE_SYSTEMATIC, RESAMPLE_ALIAS = 0, 1, 2
 WEIGHTS_LINEAR, WEIGHTS_LOG = 0, 1
 RNG_MTGP_COMPAT, RNG_MTGP_PER_FILTER = 0, 1
 FRAME_FLOATS = 336
 MAX_LANDMARKS = 64
-STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12, MODEL_UNICYCLE10: 3}
-NOISE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 6, MODEL_UNICYCLE10: 3}
+STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12, MODEL_UNICYCLE10: 3, MODEL_ROBOTARM5_CAMERA: 9}
+NOISE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 6, MODEL_UNICYCLE10: 3, MODEL_ROBOTARM5_CAMERA: 9}
 
 
 # ------------------------------------------------------------- structures
@@ -88,6 +88,12 @@ class Unicycle10Measurement(C.Structure):
     _fields_ = [("range", C.c_float * 10)]
 
 
+class CameraParams(C.Structure):
+    """Camera intrinsics of robotarm5_camera: focal lengths, principal point,
+    distortion kc1, kc2 (radial), kc3, kc4 (tangential)."""
+    _fields_ = [("fc", C.c_float * 2), ("cc", C.c_float * 2), ("kc", C.c_float * 4)]
+
+
 class Result(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("track_id", C.c_uint64), ("status", C.c_int), ("kind", C.c_int),
                 ("submit_us", C.c_double), ("done_us", C.c_double), ("estimate", Estimate)]
@@ -116,11 +122,12 @@ class ServeReport(C.Structure):
 
 
 PARAMS_TYPE = {MODEL_ROBOTARM5: None, MODEL_UNICYCLE: UnicycleParams, MODEL_DH6: DH6Params,
-               MODEL_UNICYCLE10: Unicycle10Params}
+               MODEL_UNICYCLE10: Unicycle10Params, MODEL_ROBOTARM5_CAMERA: CameraParams}
 CONTROL_TYPE = {MODEL_ROBOTARM5: RobotArm5Control, MODEL_UNICYCLE: UnicycleControl, MODEL_DH6: DH6Control,
-                MODEL_UNICYCLE10: UnicycleControl}
+                MODEL_UNICYCLE10: UnicycleControl, MODEL_ROBOTARM5_CAMERA: RobotArm5Control}
 MEAS_TYPE = {MODEL_ROBOTARM5: RobotArm5Measurement, MODEL_UNICYCLE: UnicycleMeasurement,
-             MODEL_DH6: DH6Measurement, MODEL_UNICYCLE10: Unicycle10Measurement}
+             MODEL_DH6: DH6Measurement, MODEL_UNICYCLE10: Unicycle10Measurement,
+             MODEL_ROBOTARM5_CAMERA: RobotArm5Measurement}
 
 
 class EsthError(RuntimeError):
@@ -167,8 +174,10 @@ _SIGS = {
     "esth_unicycle10_filter": ([C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p, C.c_uint32, C.c_int,
                                 _P, _P], C.c_int),
     "esth_unicycle10_params_default": ([_P], None),
+    "esth_camera_params_default": ([_P], None),
     "esth_unicycle_scenario": ([C.c_uint32, C.c_int, C.c_float, C.c_int, C.c_float, _P, _P, _P, _P], C.c_int),
     "esth_dh6_scenario": ([C.c_uint32, C.c_int, C.c_float, _P, _P, _P], C.c_int),
+    "esth_robotarm5_camera_scenario": ([C.c_uint32, C.c_int, C.c_float, _P, _P, _P, _P], C.c_int),
     "esth_batch_create": ([_P, C.c_int, _P, _P, C.c_size_t, _P], C.c_int),
     "esth_batch_destroy": ([_P], C.c_int),
     "esth_batch_init_uniform": ([_P, _P], C.c_int),
@@ -627,6 +636,15 @@ def dh6_scenario(seed, steps, dt=0.01):
     return p, meas, truth.reshape(steps + 1, 12)
 
 
+def robotarm5_camera_scenario(seed, steps, dt=0.1, params=None):
+    """Seeded camera-observed arm trace: (controls, measurements, truth rows (steps+1) x 9)."""
+    ctrl = (RobotArm5Control * steps)()
+    meas = (RobotArm5Measurement * steps)()
+    truth = np.zeros((steps + 1) * 9, np.float32)
+    check(lib().esth_robotarm5_camera_scenario(seed, steps, dt, ptr(params), ctrl, meas, ptr(truth)))
+    return ctrl, meas, truth.reshape(steps + 1, 9)
+
+
 def load_robotarm_trace(path, max_rows=100000):
     meas = (RobotArm5Measurement * max_rows)()
     ctrl = (RobotArm5Control * max_rows)()
@@ -653,6 +671,13 @@ def load_unicycle10_trace(path, max_rows=100000):
 def unicycle10_params_default():
     p = Unicycle10Params()
     lib().esth_unicycle10_params_default(C.byref(p))
+    return p
+
+
+def camera_params_default():
+    """The reference's camera calibration (pf_model_soa_packed_code.cuh:33-43)."""
+    p = CameraParams()
+    lib().esth_camera_params_default(C.byref(p))
     return p
 
 

@@ -402,6 +402,9 @@ static int batch_kernels_s(esth_batch* b, int p) {
     case ESTH_MODEL_UNICYCLE10:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
         break;
+    case ESTH_MODEL_ROBOTARM5_CAMERA:
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5_CAMERA>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
+        break;
     default:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
         break;

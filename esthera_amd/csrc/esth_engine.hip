@@ -61,6 +61,7 @@ extern "C" int esth_model_lookup(const char* name) {
     if (!strcmp(name, "unicycle")) return ESTH_MODEL_UNICYCLE;
     if (!strcmp(name, "dh6")) return ESTH_MODEL_DH6;
     if (!strcmp(name, "unicycle10")) return ESTH_MODEL_UNICYCLE10;
+    if (!strcmp(name, "robotarm5_camera")) return ESTH_MODEL_ROBOTARM5_CAMERA;
     return -1;
 }
 
@@ -70,6 +71,7 @@ int esth_params_size(int model, size_t* sz) {
     case ESTH_MODEL_UNICYCLE: *sz = sizeof(esth_unicycle_params); return ESTH_OK;
     case ESTH_MODEL_DH6: *sz = sizeof(esth_dh6_params); return ESTH_OK;
     case ESTH_MODEL_UNICYCLE10: *sz = sizeof(esth_unicycle10_params); return ESTH_OK;
+    case ESTH_MODEL_ROBOTARM5_CAMERA: *sz = sizeof(esth_camera_params); return ESTH_OK;
     default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
     }
 }
@@ -82,10 +84,11 @@ extern "C" int esth_model_info(int model, int* S, int* V, size_t* ps, size_t* cs
     size_t p = 0;
     esth_params_size(model, &p);
     if (ps) *ps = p;
-    if (cs) *cs = model == ESTH_MODEL_ROBOTARM5 ? sizeof(esth_robotarm5_control)
-                  : model == ESTH_MODEL_DH6      ? sizeof(esth_dh6_control)
-                                                 : sizeof(esth_unicycle_control);
-    if (ms) *ms = model == ESTH_MODEL_ROBOTARM5  ? sizeof(esth_robotarm5_measurement)
+    const bool arm = model == ESTH_MODEL_ROBOTARM5 || model == ESTH_MODEL_ROBOTARM5_CAMERA;
+    if (cs) *cs = arm                       ? sizeof(esth_robotarm5_control)
+                  : model == ESTH_MODEL_DH6 ? sizeof(esth_dh6_control)
+                                            : sizeof(esth_unicycle_control);
+    if (ms) *ms = arm                            ? sizeof(esth_robotarm5_measurement)
                   : model == ESTH_MODEL_UNICYCLE ? sizeof(esth_unicycle_measurement)
                   : model == ESTH_MODEL_DH6      ? sizeof(esth_dh6_measurement)
                                                  : sizeof(esth_unicycle10_measurement);
@@ -105,6 +108,19 @@ extern "C" void esth_unicycle10_params_default(esth_unicycle10_params* p) {
     p->noise_v = 0.5f;
     p->noise_omega = 0.1f;
     p->noise_gamma = 0.01f;
+}
+
+/* pf_model_soa_packed_code.cuh:33-43 */
+extern "C" void esth_camera_params_default(esth_camera_params* p) {
+    if (!p) return;
+    p->fc[0] = 408.65831f;
+    p->fc[1] = 409.69059f;
+    p->cc[0] = 166.32611f;
+    p->cc[1] = 108.93539f;
+    p->kc[0] = -0.21561f;
+    p->kc[1] = 1.60704f;
+    p->kc[2] = -0.01291f;
+    p->kc[3] = 0.00007f;
 }
 
 extern "C" int esth_model_build_frame(int model, const void* params, const void* control,
@@ -254,6 +270,9 @@ int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* 
         break;
     case ESTH_MODEL_UNICYCLE10:
         hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
+        break;
+    case ESTH_MODEL_ROBOTARM5_CAMERA:
+        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_ROBOTARM5_CAMERA>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
         break;
     default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
     }

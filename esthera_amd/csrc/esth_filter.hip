@@ -778,3 +778,65 @@ extern "C" int esth_dh6_scenario(uint32_t seed, int steps, float dt, esth_dh6_pa
     }
     return ESTH_OK;
 }
+
+/* the base's pixel position for an arm state (pinhole + distortion of the
+ * robotarm5_camera model), in double */
+static void camera_project(const double* q, double x, double y, const esth_camera_params* p, double* u, double* v) {
+    double cx = x * cos(q[0]) + y * sin(q[0]);
+    double cy = -x * sin(q[0]) + y * cos(q[0]);
+    double cz = 0.0;
+    for (int i = 1; i < 5; ++i) {
+        const double nb = cy * cos(q[i]) - cz * sin(q[i]);
+        const double nc = cy * sin(q[i]) + cz * cos(q[i]);
+        cy = nb;
+        cz = nc - 3.0;
+    }
+    const double xn = cx / cz, yn = cy / cz, r2 = xn * xn + yn * yn;
+    const double radial = 1.0 + p->kc[0] * r2 + p->kc[1] * r2 * r2;
+    const double xd = xn * radial + 2.0 * p->kc[2] * xn * yn + p->kc[3] * (r2 + 2.0 * xn * xn);
+    const double yd = yn * radial + p->kc[2] * (r2 + 2.0 * yn * yn) + 2.0 * p->kc[3] * xn * yn;
+    *u = p->fc[0] * xd + p->cc[0];
+    *v = p->fc[1] * yd + p->cc[1];
+}
+
+extern "C" int esth_robotarm5_camera_scenario(uint32_t seed, int steps, float dt, const esth_camera_params* params,
+                                              esth_robotarm5_control* ctrl, esth_robotarm5_measurement* meas,
+                                              float* truth) {
+    if (steps < 0 || !(dt > 0.0f)) return esth_fail(ESTH_ERR_INVALID, "scenario args");
+    esth_camera_params dflt;
+    esth_camera_params_default(&dflt);
+    const esth_camera_params* p = params ? params : &dflt;
+    esth_hostrng rng{0xca3e000000000000ull ^ (uint64_t)seed};
+    double q[5], x = 0.0, y = 0.0, vx = 0.0, vy = 0.0, ph[5];
+    for (int i = 0; i < 5; ++i) { q[i] = 0.75 * M_PI; ph[i] = 2.0 * M_PI * rng.uniform(); }
+    if (truth) {
+        for (int i = 0; i < 5; ++i) truth[i] = (float)q[i];
+        truth[5] = truth[6] = truth[7] = truth[8] = 0.0f;
+    }
+    for (int k = 0; k < steps; ++k) {
+        for (int i = 0; i < 5; ++i) {
+            const double rate = 0.2 * sin(0.05 * (double)k + ph[i]);
+            if (ctrl) ctrl[k].angles[i] = (float)rate;
+            q[i] += rate * dt + 0.01 * rng.normal();
+        }
+        vx = fmin(0.5, fmax(-0.5, vx + 0.1 * rng.normal()));
+        vy = fmin(0.5, fmax(-0.5, vy + 0.1 * rng.normal()));
+        x += vx * dt;
+        y += vy * dt;
+        if (fabs(x) > 1.0) { x = copysign(2.0, x) - x; vx = -vx; }
+        if (fabs(y) > 1.0) { y = copysign(2.0, y) - y; vy = -vy; }
+        if (meas) {
+            double u, v;
+            camera_project(q, x, y, p, &u, &v);
+            for (int i = 0; i < 5; ++i) meas[k].angles[i] = (float)(q[i] + 0.05 * rng.normal());
+            meas[k].x = (float)(u + rng.normal());
+            meas[k].y = (float)(v + rng.normal());
+        }
+        if (truth) {
+            float* tr = truth + (size_t)(k + 1) * 9;
+            for (int i = 0; i < 5; ++i) tr[i] = (float)q[i];
+            tr[5] = (float)x; tr[6] = (float)y; tr[7] = (float)vx; tr[8] = (float)vy;
+        }
+    }
+    return ESTH_OK;
+}

@@ -176,6 +176,13 @@ struct esth_model_traits<ESTH_MODEL_UNICYCLE10> {
     static constexpr int S = ESTH_U10_S, V = ESTH_U10_V;
     __device__ static float iw(float* s, const float* z, const float* fr) { return esth_unicycle10_sampling_iw(s, z, fr); }
 };
+template <>
+struct esth_model_traits<ESTH_MODEL_ROBOTARM5_CAMERA> {
+    static constexpr int S = ESTH_RA5_S, V = ESTH_RA5_V;
+    __device__ static float iw(float* s, const float* z, const float* fr) {
+        return esth_robotarm5_camera_sampling_iw(s, z, fr);
+    }
+};
 
 /* particles p (s0) and p+1 (s1) given their uniform pairs (u0[k], u1[k]) =
  * (R[k N + p], R[k N + p + 1]): Box-Muller, propagation and log-weight */
@@ -460,6 +467,7 @@ __device__ __forceinline__ void esth_argmax_block(const float2* __restrict__ win
     if (log_w) {
         gmax = smax[0];
         for (int i = 1; i < NW; ++i) gmax = smax[i] > gmax ? smax[i] : gmax;
+        gmax = esth_max_shift(gmax);
     }
     double a[2 + S];
 #pragma unroll
@@ -552,7 +560,7 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
     float sw = 0.0f;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-        wt[r] = log_w ? esth_expf(key[r] - mx) : key[r];
+        wt[r] = log_w ? esth_expf(key[r] - esth_max_shift(mx)) : key[r];
         sw = sw + wt[r];
     }
     float sx[S];
@@ -790,7 +798,7 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
         for (int r = 1; r < E; ++r) mx = x[r] > mx ? x[r] : mx;
         mx = esth_bfly_max<1, LPS>(mx);
 #pragma unroll
-        for (int r = 0; r < E; ++r) x[r] = esth_expf(x[r] - mx);
+        for (int r = 0; r < E; ++r) x[r] = esth_expf(x[r] - esth_max_shift(mx));
     }
     /* register tree: lvl[k][j], k = 0..LOGE */
     float lvl[LOGE + 1][E];
@@ -1010,7 +1018,7 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
         __syncthreads();
         mx = red[0];
         for (int i = 1; i < nw; ++i) mx = red[i] > mx ? red[i] : mx;
-        p = esth_expf(p - mx);
+        p = esth_expf(p - esth_max_shift(mx));
     }
     scratch[l] = p;
     __syncthreads();

@@ -102,6 +102,10 @@ ESTH_HD float esth_expf(float x) {
     return (x != x) ? x : y;
 }
 
+/* the shift of a log-domain normalisation exp(x - shift) by the maximum m:
+ * m itself, or 0 when every x is -inf (weights 0 rather than NaN) */
+ESTH_HD float esth_max_shift(float m) { return m == -INFINITY ? 0.0f : m; }
+
 /* natural log for x > 0 (x <= 0 returns -inf / NaN). */
 ESTH_HD float esth_logf(float x) {
     if (!(x > 0.0f)) return (x == 0.0f) ? esth_u2f(0xff800000u) : esth_u2f(0x7fc00000u);

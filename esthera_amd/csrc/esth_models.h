@@ -41,19 +41,22 @@
 
 ESTH_HD float esth_ra5_arm_length(int i) { return i == 0 ? 0.0f : 3.0f; } /* pf_model.cl:13 */
 
-ESTH_HD float esth_robotarm5_sampling_iw(float* s, const float* z, const float* fr) {
+/* The base's random walk and the joint chain (pf_model.cl:70-117):
+ * zp = noise of x, y, vx, vy; za = noise of the five joint angles.  Leaves
+ * the propagated particle in s, the camera-frame point of the base in cam
+ * and returns the weighted squared angle-measurement errors. */
+ESTH_HD float esth_ra5_chain(float* s, const float* za, const float* zp, const float* fr, float* cam) {
     const float dt = fr[0];
     float value = 0.0f;
-    /* position / velocity random walk, pf_model.cl:70-74 */
     float x = s[5], y = s[6], vx = s[7], vy = s[8];
-    x = x + ((vx * dt) + (z[0] * ESTH_RA5_NOISE_X));
-    y = y + ((vy * dt) + (z[1] * ESTH_RA5_NOISE_Y));
-    vx = vx + (z[2] * ESTH_RA5_NOISE_V_X);
-    vy = vy + (z[3] * ESTH_RA5_NOISE_V_Y);
+    x = x + ((vx * dt) + (zp[0] * ESTH_RA5_NOISE_X));
+    y = y + ((vy * dt) + (zp[1] * ESTH_RA5_NOISE_Y));
+    vx = vx + (zp[2] * ESTH_RA5_NOISE_V_X);
+    vy = vy + (zp[3] * ESTH_RA5_NOISE_V_Y);
     float camX, camY, camZ;
     /* joint 0, pf_model.cl:76-90 */
     {
-        float a = s[0] + ((fr[1] * dt) + (z[4] * ESTH_RA5_NOISE_ANGLES));
+        float a = s[0] + ((fr[1] * dt) + (za[0] * ESTH_RA5_NOISE_ANGLES));
         float sf, cf;
         esth_sincosf(a, &sf, &cf);
         camX = (x * cf) + (y * sf);
@@ -65,7 +68,7 @@ ESTH_HD float esth_robotarm5_sampling_iw(float* s, const float* z, const float* 
     }
     /* joints 1..4, pf_model.cl:91-117 */
     for (int i = 1; i < 5; ++i) {
-        float a = s[i] + ((fr[1 + i] * dt) + (z[4 + i] * ESTH_RA5_NOISE_ANGLES));
+        float a = s[i] + ((fr[1 + i] * dt) + (za[i] * ESTH_RA5_NOISE_ANGLES));
         float sf, cf;
         esth_sincosf(a, &sf, &cf);
         float e = a - fr[6 + i];
@@ -77,10 +80,53 @@ ESTH_HD float esth_robotarm5_sampling_iw(float* s, const float* z, const float* 
         s[i] = a;
     }
     s[5] = x; s[6] = y; s[7] = vx; s[8] = vy;
+    cam[0] = camX; cam[1] = camY; cam[2] = camZ;
+    return value;
+}
+
+ESTH_HD float esth_robotarm5_sampling_iw(float* s, const float* z, const float* fr) {
+    float cam[3];
+    float value = esth_ra5_chain(s, z + 4, z, fr, cam);
     /* camera error, pf_model.cl:124-125 */
-    value = value + (((camX - fr[11]) * (camX - fr[11])) * ESTH_RA5_NOISE_CAMERA_X);
-    value = value + (((camY - fr[12]) * (camY - fr[12])) * ESTH_RA5_NOISE_CAMERA_Y);
+    value = value + (((cam[0] - fr[11]) * (cam[0] - fr[11])) * ESTH_RA5_NOISE_CAMERA_X);
+    value = value + (((cam[1] - fr[12]) * (cam[1] - fr[12])) * ESTH_RA5_NOISE_CAMERA_Y);
     return -value;
+}
+
+/* ----------------------------------------------------- robotarm5_camera
+ * The arm seen through a calibrated camera
+ * (src/models/robotarm/cuda/pf_model_soa_packed_code.cuh:33-43,57-143): the
+ * same walk and joint chain, noise in the CUDA order (0..4 joint angles,
+ * 5 x, 6 y, 7 vx, 8 vy, :79-83), and the camera-frame point projected to
+ * pixels by the pinhole + Brown-Conrady model before it is compared with the
+ * measurement (:119-131):
+ *   xn = X/Z, yn = Y/Z, r2 = xn^2 + yn^2, k = 1 + kc1 r2 + kc2 r2^2,
+ *   u = fc_x (k xn + 2 kc3 xn yn + kc4 (r2 + 2 xn^2)) + cc_x,
+ *   v = fc_y (k yn + kc3 (r2 + 2 yn^2) + 2 kc4 xn yn) + cc_y,
+ *   log w = -(angle terms + 0.1 (u - mu)^2 + 0.1 (v - mv)^2).
+ * A point whose projection overflows gives +inf, never NaN: a NaN cost is
+ * taken as +inf (weight 0), so no NaN key reaches the sort.
+ * State planes and frame as robotarm5, measurement x, y in pixels; frame
+ * [13] fc_x, [14] fc_y, [15] cc_x, [16] cc_y, [17..20] kc1..kc4. */
+#define ESTH_RA5C_FLOATS 21
+
+ESTH_HD float esth_robotarm5_camera_sampling_iw(float* s, const float* z, const float* fr) {
+    float cam[3];
+    float value = esth_ra5_chain(s, z, z + 5, fr, cam);
+    const float xn = cam[0] / cam[2];
+    const float yn = cam[1] / cam[2];
+    const float xx = xn * xn, yy = yn * yn, xy = xn * yn;
+    const float r2 = xx + yy;
+    const float kc1 = fr[17], kc2 = fr[18], kc3 = fr[19], kc4 = fr[20];
+    const float radial = esth_fma(esth_fma(kc2, r2, kc1), r2, 1.0f);
+    const float tx = esth_fma(2.0f * kc3, xy, kc4 * esth_fma(2.0f, xx, r2));
+    const float ty = esth_fma(kc3, esth_fma(2.0f, yy, r2), (2.0f * kc4) * xy);
+    const float u = esth_fma(fr[13], esth_fma(radial, xn, tx), fr[15]);
+    const float v = esth_fma(fr[14], esth_fma(radial, yn, ty), fr[16]);
+    const float eu = u - fr[11], ev = v - fr[12];
+    value = value + ((eu * eu) * ESTH_RA5_NOISE_CAMERA_X);
+    value = value + ((ev * ev) * ESTH_RA5_NOISE_CAMERA_Y);
+    return value == value ? -value : -INFINITY;
 }
 
 /* ------------------------------------------------------------- unicycle
@@ -246,6 +292,7 @@ ESTH_HD int esth_model_dims(int model, int* S, int* V) {
     case ESTH_MODEL_UNICYCLE:  *S = ESTH_UNI_S; *V = ESTH_UNI_V; return 0;
     case ESTH_MODEL_DH6:       *S = ESTH_DH6_S; *V = ESTH_DH6_V; return 0;
     case ESTH_MODEL_UNICYCLE10: *S = ESTH_U10_S; *V = ESTH_U10_V; return 0;
+    case ESTH_MODEL_ROBOTARM5_CAMERA: *S = ESTH_RA5_S; *V = ESTH_RA5_V; return 0;
     default: return -1;
     }
 }
@@ -259,15 +306,25 @@ ESTH_HD int esth_build_frame_n(int model, const void* params, const void* contro
         return -1;
     if (model == ESTH_MODEL_DH6 && ESTH_DH6_HDR + 24 > nfloats) return -1;
     if (model == ESTH_MODEL_UNICYCLE10 && ESTH_U10_FLOATS > nfloats) return -1;
+    if (model == ESTH_MODEL_ROBOTARM5_CAMERA && ESTH_RA5C_FLOATS > nfloats) return -1;
     if (nfloats < 16) return -1;
     fr[0] = dt;
-    if (model == ESTH_MODEL_ROBOTARM5) {
+    if (model == ESTH_MODEL_ROBOTARM5 || model == ESTH_MODEL_ROBOTARM5_CAMERA) {
         const esth_robotarm5_control* c = (const esth_robotarm5_control*)control;
         const esth_robotarm5_measurement* m = (const esth_robotarm5_measurement*)measurement;
         if (!c || !m) return -1;
         for (int i = 0; i < 5; ++i) { fr[1 + i] = c->angles[i]; fr[6 + i] = m->angles[i]; }
         fr[11] = m->x;
         fr[12] = m->y;
+        if (model == ESTH_MODEL_ROBOTARM5_CAMERA) {
+            const esth_camera_params* p = (const esth_camera_params*)params;
+            if (!p) return -1;
+            fr[13] = p->fc[0];
+            fr[14] = p->fc[1];
+            fr[15] = p->cc[0];
+            fr[16] = p->cc[1];
+            for (int i = 0; i < 4; ++i) fr[17 + i] = p->kc[i];
+        }
         return 0;
     }
     if (model == ESTH_MODEL_UNICYCLE) {
@@ -348,6 +405,7 @@ ESTH_HD int esth_model_frame_floats(int model, const void* params) {
         return params ? ESTH_UNI_HDR + 5 * ((const esth_unicycle_params*)params)->num_landmarks : ESTH_FRAME_FLOATS;
     if (model == ESTH_MODEL_DH6) return ESTH_DH6_HDR + 24;
     if (model == ESTH_MODEL_UNICYCLE10) return ESTH_U10_FLOATS;
+    if (model == ESTH_MODEL_ROBOTARM5_CAMERA) return ESTH_RA5C_FLOATS;
     return 16;
 }
 

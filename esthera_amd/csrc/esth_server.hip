@@ -601,6 +601,9 @@ static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wa
     case ESTH_MODEL_UNICYCLE10:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
         break;
+    case ESTH_MODEL_ROBOTARM5_CAMERA:
+        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5_CAMERA>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
+        break;
     default:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
         break;

@@ -71,12 +71,18 @@ int esth_version(void);
  *              60-74,313-362): velocity motion model with noisy speed, turn
  *              rate and heading drift, 10 fixed range sensors above the
  *              plane; state x, y, theta (3); 3 normals per particle.
+ *  robotarm5_camera: robotarm5 observed through a calibrated camera
+ *              (src/models/robotarm/cuda/pf_model_soa_packed_code.cuh:33-43,
+ *              57-143): the base point is projected to pixels by the pinhole
+ *              + radial/tangential distortion model before the comparison;
+ *              noise in the CUDA order (angles first); state as robotarm5.
  */
 #define ESTH_MODEL_ROBOTARM5  0
 #define ESTH_MODEL_UNICYCLE   1
 #define ESTH_MODEL_DH6        2
 #define ESTH_MODEL_UNICYCLE10 3
-#define ESTH_NUM_MODELS       4
+#define ESTH_MODEL_ROBOTARM5_CAMERA 4
+#define ESTH_NUM_MODELS       5
 
 /* robotarm5 (pf_model.h:34-56) */
 typedef struct { float angles[5]; } esth_robotarm5_control;
@@ -116,7 +122,15 @@ typedef struct { float range[ESTH_UNICYCLE10_SENSORS]; } esth_unicycle10_measure
 /* the reference's sensor positions and noise constants (particle_filter.cu:60-74) */
 void esth_unicycle10_params_default(esth_unicycle10_params* p);
 
-/* Model id for "robotarm5" | "unicycle" | "dh6" | "unicycle10"; -1 if unknown. */
+/* robotarm5_camera: control and measurement are robotarm5's (x, y in pixels);
+ * the parameters are the camera's intrinsics: focal lengths, principal point
+ * and distortion kc1, kc2 (radial), kc3, kc4 (tangential). */
+typedef struct { float fc[2], cc[2], kc[4]; } esth_camera_params;
+/* the reference's calibration (pf_model_soa_packed_code.cuh:33-43) */
+void esth_camera_params_default(esth_camera_params* p);
+
+/* Model id for "robotarm5" | "unicycle" | "dh6" | "unicycle10" |
+ * "robotarm5_camera"; -1 if unknown. */
 int esth_model_lookup(const char* name);
 /* Sizes for a model: state floats per particle, normals per particle, and
  * the byte sizes of its params / control / measurement structs. */
@@ -279,6 +293,16 @@ int esth_unicycle_scenario(uint32_t seed, int num_landmarks, float box, int step
  * initial q, qd (also drawn from `seed`), row k the state after step k. */
 int esth_dh6_scenario(uint32_t seed, int steps, float dt, esth_dh6_params* params,
                       esth_dh6_measurement* meas, float* truth);
+/* robotarm5_camera: the arm's joints start at 0.75 pi and follow smooth
+ * controls (|rate| <= 0.2 rad/s) with 0.01 rad process noise per step; the
+ * base walks near the origin (|x|, |y| <= 1, speeds <= 0.5) so that the
+ * camera sees it; measurements: joint angles with 0.05 rad noise and the
+ * base's projection through `params` (NULL: the reference's calibration)
+ * with 1 px noise.  truth: (steps+1)*9 floats, row 0 the initial state, row
+ * k the state after step k. */
+int esth_robotarm5_camera_scenario(uint32_t seed, int steps, float dt, const esth_camera_params* params,
+                                   esth_robotarm5_control* controls, esth_robotarm5_measurement* meas,
+                                   float* truth);
 
 /* ------------------------------------------------------------ batches
  * Many independent filters stepped together: every stage is one launch over

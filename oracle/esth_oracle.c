@@ -128,6 +128,7 @@ int or_sample_weight(int model, float* state, uint32_t N, const float* normals, 
         if (model == ESTH_MODEL_ROBOTARM5) lw = esth_robotarm5_sampling_iw(s, z, frame);
         else if (model == ESTH_MODEL_UNICYCLE) lw = esth_unicycle_sampling_iw(s, z, frame);
         else if (model == ESTH_MODEL_UNICYCLE10) lw = esth_unicycle10_sampling_iw(s, z, frame);
+        else if (model == ESTH_MODEL_ROBOTARM5_CAMERA) lw = esth_robotarm5_camera_sampling_iw(s, z, frame);
         else lw = esth_dh6_sampling_iw(s, z, frame);
         for (int c = 0; c < S; ++c) state[(size_t)c * N + i] = s[c];
         w[i] = (weights == ESTH_WEIGHTS_LOG) ? lw : esth_expf(lw);
@@ -138,17 +139,19 @@ int or_sample_weight(int model, float* state, uint32_t N, const float* normals, 
 /* ---- FP64 twins: the models written directly from their definitions with
  * libm double functions, independent of esth_math.h / esth_models.h. */
 
-/* robotarm5: pf_model.cl:52-131 in double */
-static double ra5_f64(double* s, const double* z, const esth_robotarm5_control* c,
-                      const esth_robotarm5_measurement* m, double dt) {
+/* robotarm5's walk and joint chain (pf_model.cl:70-117) in double; za / zp:
+ * noise of the joint angles / of x, y, vx, vy.  Returns the angle cost and
+ * the camera-frame point in cam. */
+static double ra5_chain_f64(double* s, const double* za, const double* zp, const esth_robotarm5_control* c,
+                            const esth_robotarm5_measurement* m, double dt, double cam[3]) {
     static const double L[5] = {0.0, 3.0, 3.0, 3.0, 3.0};
-    double x = s[5] + s[7] * dt + z[0] * 0.2;
-    double y = s[6] + s[8] * dt + z[1] * 0.2;
-    double vx = s[7] + z[2] * 2.0;
-    double vy = s[8] + z[3] * 2.0;
+    double x = s[5] + s[7] * dt + zp[0] * 0.2;
+    double y = s[6] + s[8] * dt + zp[1] * 0.2;
+    double vx = s[7] + zp[2] * 2.0;
+    double vy = s[8] + zp[3] * 2.0;
     double value = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     for (int i = 0; i < 5; ++i) {
-        double a = s[i] + (double)c->angles[i] * dt + z[4 + i] * 0.2;
+        double a = s[i] + (double)c->angles[i] * dt + za[i] * 0.2;
         double sf = sin(a), cf = cos(a);
         double e = a - (double)m->angles[i];
         value += e * e * (double)0.1f;
@@ -165,8 +168,40 @@ static double ra5_f64(double* s, const double* z, const esth_robotarm5_control* 
         s[i] = a;
     }
     s[5] = x; s[6] = y; s[7] = vx; s[8] = vy;
-    value += (cx - m->x) * (cx - m->x) * (double)0.1f;
-    value += (cy - m->y) * (cy - m->y) * (double)0.1f;
+    cam[0] = cx; cam[1] = cy; cam[2] = cz;
+    return value;
+}
+
+/* robotarm5: pf_model.cl:52-131 in double */
+static double ra5_f64(double* s, const double* z, const esth_robotarm5_control* c,
+                      const esth_robotarm5_measurement* m, double dt) {
+    double cam[3];
+    double value = ra5_chain_f64(s, z + 4, z, c, m, dt, cam);
+    value += (cam[0] - m->x) * (cam[0] - m->x) * (double)0.1f;
+    value += (cam[1] - m->y) * (cam[1] - m->y) * (double)0.1f;
+    return -value;
+}
+
+/* robotarm5_camera (pf_model_soa_packed_code.cuh:57-143) in double, the
+ * projection in the Brown-Conrady form with tangential coefficients
+ * p1 = kc3, p2 = kc4:
+ *   xd = x (1 + k1 r^2 + k2 r^4) + 2 p1 x y + p2 (r^2 + 2 x^2),
+ *   yd = y (1 + k1 r^2 + k2 r^4) + p1 (r^2 + 2 y^2) + 2 p2 x y,
+ *   (u, v) = (fx xd + cx, fy yd + cy). */
+static double ra5c_f64(double* s, const double* z, const esth_camera_params* p, const esth_robotarm5_control* c,
+                       const esth_robotarm5_measurement* m, double dt) {
+    double cam[3];
+    double value = ra5_chain_f64(s, z, z + 5, c, m, dt, cam);
+    const double x = cam[0] / cam[2], y = cam[1] / cam[2];
+    const double r2 = x * x + y * y, r4 = r2 * r2;
+    const double k1 = p->kc[0], k2 = p->kc[1], p1 = p->kc[2], p2 = p->kc[3];
+    const double radial = 1.0 + k1 * r2 + k2 * r4;
+    const double xd = x * radial + 2.0 * p1 * x * y + p2 * (r2 + 2.0 * x * x);
+    const double yd = y * radial + p1 * (r2 + 2.0 * y * y) + 2.0 * p2 * x * y;
+    const double u = (double)p->fc[0] * xd + (double)p->cc[0];
+    const double v = (double)p->fc[1] * yd + (double)p->cc[1];
+    value += (u - m->x) * (u - m->x) * (double)0.1f;
+    value += (v - m->y) * (v - m->y) * (double)0.1f;
     return -value;
 }
 
@@ -273,6 +308,9 @@ int or_sample_weight_f64(int model, const float* state_in, uint32_t N, const flo
         else if (model == ESTH_MODEL_UNICYCLE10)
             lw = u10_f64(s, z, (const esth_unicycle10_params*)params, (const esth_unicycle_control*)ctrl,
                          (const esth_unicycle10_measurement*)meas, dt);
+        else if (model == ESTH_MODEL_ROBOTARM5_CAMERA)
+            lw = ra5c_f64(s, z, (const esth_camera_params*)params, (const esth_robotarm5_control*)ctrl,
+                          (const esth_robotarm5_measurement*)meas, dt);
         else
             lw = dh6_f64(s, z, (const esth_dh6_params*)params, (const esth_dh6_measurement*)meas, dt);
         for (int c = 0; c < S; ++c) state_out[(size_t)c * N + i] = s[c];
@@ -546,7 +584,7 @@ static void alias_probs(const float* wblk, uint32_t m, int weights, float* prob)
     float scratch[ESTH_MAX_NUM_PARTICLES], p[ESTH_MAX_NUM_PARTICLES];
     float mx = (weights == ESTH_WEIGHTS_LOG) ? block_max(wblk, m) : 0.0f;
     for (uint32_t l = 0; l < m; ++l) {
-        p[l] = (weights == ESTH_WEIGHTS_LOG) ? esth_expf(wblk[l] - mx) : wblk[l];
+        p[l] = (weights == ESTH_WEIGHTS_LOG) ? esth_expf(wblk[l] - esth_max_shift(mx)) : wblk[l];
         scratch[l] = p[l];
     }
     float sum = sum_reduce4(scratch, m);
@@ -606,7 +644,7 @@ int or_resample(int resampler, float* dst, const float* src, int S, const float*
             float cdf[ESTH_MAX_NUM_PARTICLES];
             float mx = (weights == ESTH_WEIGHTS_LOG) ? block_max(w + base, m) : 0.0f;
             for (uint32_t l = 0; l < m; ++l)
-                cdf[l] = (weights == ESTH_WEIGHTS_LOG) ? esth_expf(w[base + l] - mx) : w[base + l];
+                cdf[l] = (weights == ESTH_WEIGHTS_LOG) ? esth_expf(w[base + l] - esth_max_shift(mx)) : w[base + l];
             float cum = prefix_sum(cdf, m);
             if (resampler == ESTH_RESAMPLE_MULTINOMIAL) {
                 for (uint32_t l = 0; l < m; ++l) {
@@ -723,6 +761,7 @@ void or_filter_set_state(or_filter* f, const float* soa) {
 static void posterior_f64(const double* st, const double* lw, size_t N, int S, double* mean) {
     double mx = -INFINITY;
     for (size_t i = 0; i < N; ++i) mx = lw[i] > mx ? lw[i] : mx;
+    if (mx == -INFINITY) mx = 0.0;   /* every weight 0 */
     double sw = 0.0, acc[ESTH_MAX_STATE_DIM] = {0};
     for (size_t i = 0; i < N; ++i) {
         double wi = exp(lw[i] - mx);

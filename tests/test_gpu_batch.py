@@ -78,6 +78,45 @@ def test_batch_equals_filters_alone(name, kw, graph, engine):
     b.close()
 
 
+def test_batch_camera_filters_equal_alone(engine):
+    """robotarm5_camera filters with their own calibrations in one batch."""
+    sizes = [(64, 4), (256, 2), (128, 3)]
+    nf, steps = 5, 4
+    descs, params, scen, inits = [], [], [], []
+    for i in range(nf):
+        m, G = sizes[i % len(sizes)]
+        descs.append(ea.filter_desc(ea.MODEL_ROBOTARM5_CAMERA, m=m, G=G, t=1, topology=ea.TOPO_RING,
+                                    resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG,
+                                    rng=ea.RNG_MTGP_PER_FILTER, seed=100 + i))
+        p = ea.camera_params_default()
+        p.fc[0] *= 1.0 + 0.05 * i
+        p.kc[1] *= 1.0 - 0.1 * i
+        ctrl, meas, truth = ea.robotarm5_camera_scenario(50 + i, steps, 0.1, p)
+        params.append(p)
+        scen.append((ctrl, meas))
+        inits.append(truth[0])
+    b = ea.Batch(engine, descs, params)
+    b.init_uniform(np.array(inits, np.float32))
+    alone = []
+    for i in range(nf):
+        f = ea.Filter(engine, descs[i], params[i])
+        f.init_uniform(inits[i])
+        alone.append(f)
+    for k in range(steps):
+        ctrls = (ea.RobotArm5Control * nf)(*[sc[0][k] for sc in scen])
+        meas = (ea.RobotArm5Measurement * nf)(*[sc[1][k] for sc in scen])
+        b.step(ctrls, meas, np.full(nf, 0.1, np.float32))
+        for i, f in enumerate(alone):
+            f.step(scen[i][0][k], scen[i][1][k], 0.1)
+    for i, f in enumerate(alone):
+        sb, wb = b.particles(i)
+        sa, wa = f.particles()
+        assert np.array_equal(sb.view(np.uint32), sa.view(np.uint32)), f"filter {i} state differs"
+        assert np.array_equal(wb.view(np.uint32), wa.view(np.uint32)), f"filter {i} weights differ"
+        f.close()
+    b.close()
+
+
 def test_batch_info_and_memory(engine):
     descs, params, scen = make_filters(4, 5, [(256, 4), (512, 2)])
     used0, _ = engine.memory()

@@ -27,6 +27,9 @@ def make_case(model, steps):
         meas, ctrl, dts, truth = ea.load_unicycle10_trace(os.path.join(HERE, "golden", "unicycle_input.txt"))
         init = np.array([20.0, 40.0, -np.pi / 2], np.float32)
         return ea.unicycle10_params_default(), ctrl, meas, init, truth, dts
+    if model == ea.MODEL_ROBOTARM5_CAMERA:
+        ctrl, meas, truth = ea.robotarm5_camera_scenario(21, steps, 0.1)
+        return ea.camera_params_default(), ctrl, meas, truth[0].copy(), truth[1:], 0.1
     meas, ctrl, dts, truth = ea.load_robotarm_trace(os.path.join(HERE, "golden", "robotarm_input5.txt"))
     a0 = np.float32(0.75 * np.pi)
     init = np.array([a0] * 5 + [7.0, 0.0, 0.0, 0.0], np.float32)
@@ -82,6 +85,13 @@ CASES = [
                                            adaptive=1, threshold=0.02, weights=ea.WEIGHTS_LOG, carry=1)),
     ("unicycle10-alias-log", dict(model=ea.MODEL_UNICYCLE10, m=256, G=8, t=2, resampler=ea.RESAMPLE_ALIAS,
                                   weights=ea.WEIGHTS_LOG)),
+    ("camera-ring-log", dict(model=ea.MODEL_ROBOTARM5_CAMERA, m=256, G=16, t=1, weights=ea.WEIGHTS_LOG)),
+    ("camera-reference-linear", dict(model=ea.MODEL_ROBOTARM5_CAMERA, m=64, G=16, t=1)),
+    ("camera-alias-log", dict(model=ea.MODEL_ROBOTARM5_CAMERA, m=256, G=8, t=2, resampler=ea.RESAMPLE_ALIAS,
+                              weights=ea.WEIGHTS_LOG)),
+    ("camera-torus-carry-gate", dict(model=ea.MODEL_ROBOTARM5_CAMERA, m=512, G=16, t=4, topology=ea.TOPO_TORUS,
+                                     resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, carry=1,
+                                     threshold=0.5)),
     ("unicycle-small-m", dict(model=ea.MODEL_UNICYCLE, m=16, G=12, t=2, weights=ea.WEIGHTS_LOG,
                               rng=ea.RNG_MTGP_PER_FILTER)),
 ]

@@ -165,6 +165,11 @@ def _model_case(model, N, seed):
             os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "unicycle_input.txt"))
         params, ctrl, meas, dt = ea.unicycle10_params_default(), ctrls[2], meass[2], float(dts[2])
         st = np.concatenate([truth[1, c] + rng.normal(0, 0.3, N).astype(np.float32) for c in range(3)])
+    elif model == ea.MODEL_ROBOTARM5_CAMERA:
+        params = ea.camera_params_default()
+        ctrls, meass, truth = ea.robotarm5_camera_scenario(3, 4, 0.1)
+        ctrl, meas, dt = ctrls[2], meass[2], 0.1
+        st = np.concatenate([truth[2, c] + rng.normal(0, 0.05, N).astype(np.float32) for c in range(9)])
     else:
         params, meass, truth = ea.dh6_scenario(7, 3)
         ctrl, meas = ea.DH6Control(), meass[2]
@@ -174,7 +179,8 @@ def _model_case(model, N, seed):
     return params, ctrl, meas, st.astype(np.float32), R, dt, S, V
 
 
-@pytest.mark.parametrize("model", [ea.MODEL_ROBOTARM5, ea.MODEL_UNICYCLE, ea.MODEL_DH6, ea.MODEL_UNICYCLE10])
+@pytest.mark.parametrize("model", [ea.MODEL_ROBOTARM5, ea.MODEL_UNICYCLE, ea.MODEL_DH6, ea.MODEL_UNICYCLE10,
+                                   ea.MODEL_ROBOTARM5_CAMERA])
 @pytest.mark.parametrize("weights", [ea.WEIGHTS_LINEAR, ea.WEIGHTS_LOG])
 def test_sample_weight_bit_exact(model, weights, engine):
     N = 4096

@@ -163,6 +163,95 @@ def test_unicycle10_model_against_fp64_twin():
     np.testing.assert_allclose(s32, [1.0 + np.cos(0.5), 2.0 + np.sin(0.5), 0.5], rtol=1e-6)
 
 
+def test_camera_model_against_fp64_twin():
+    """The FP32 hook of robotarm5_camera against the double twin (projection
+    written in the Brown-Conrady form with libm) on a seeded camera trace."""
+    import esthera_amd as ea
+    p = ea.camera_params_default()
+    ctrl, meas, truth = ea.robotarm5_camera_scenario(11, 60, 0.1)
+    rng = np.random.default_rng(6)
+    N = 2048
+    M = ea.MODEL_ROBOTARM5_CAMERA
+    for k in (1, 30, 59):
+        st = np.concatenate([truth[k - 1, c] + rng.normal(0, 0.05, N) for c in range(9)]).astype(np.float32)
+        normals = rng.normal(size=9 * N).astype(np.float32)
+        fr = O.build_frame(M, p, ctrl[k - 1], meas[k - 1], 0.1)
+        s32, w32 = O.sample_weight(M, st, N, normals, fr, 1)
+        s64, lw64 = O.sample_weight_f64(M, st, N, normals, p, ctrl[k - 1], meas[k - 1], 0.1, 9)
+        np.testing.assert_allclose(s32, s64, rtol=2e-6, atol=4e-6)
+        np.testing.assert_allclose(w32, lw64, rtol=2e-4, atol=2e-3)
+
+
+def _camera_lw(p, angles, x, y, meas_xy, meas_angles):
+    """log-weight of one noise-free particle at dt = 0 (the state is kept)."""
+    import esthera_amd as ea
+    M = ea.MODEL_ROBOTARM5_CAMERA
+    c = ea.RobotArm5Control()
+    m = ea.RobotArm5Measurement()
+    for i in range(5):
+        m.angles[i] = meas_angles[i]
+    m.x, m.y = meas_xy
+    fr = O.build_frame(M, p, c, m, 0.0)
+    st = np.array(list(angles) + [x, y, 0.0, 0.0], np.float32)
+    s32, w32 = O.sample_weight(M, st, 1, np.zeros(9, np.float32), fr, 1)
+    np.testing.assert_array_equal(s32, st)
+    return float(w32[0])
+
+
+def test_camera_optical_axis_maps_to_principal_point():
+    """All joints at 0 and the base at the origin put the point on the optical
+    axis (X = Y = 0, Z = -12): its pixel is the principal point whatever the
+    distortion, so a measurement there costs nothing and one (3, 4) px away
+    costs 0.1 (9 + 16)."""
+    import esthera_amd as ea
+    p = ea.camera_params_default()
+    cc = (p.cc[0], p.cc[1])
+    assert _camera_lw(p, [0.0] * 5, 0.0, 0.0, cc, [0.0] * 5) == 0.0
+    lw = _camera_lw(p, [0.0] * 5, 0.0, 0.0, (cc[0] + 3.0, cc[1] + 4.0), [0.0] * 5)
+    np.testing.assert_allclose(lw, -0.1 * 25.0, rtol=1e-5)
+    # angle terms: 0.1 per squared radian, independent of the camera
+    lw = _camera_lw(p, [0.0] * 5, 0.0, 0.0, cc, [0.5, 0.0, 0.0, 0.0, -1.0])
+    np.testing.assert_allclose(lw, -0.1 * 1.25, rtol=1e-5)
+
+
+@pytest.mark.parametrize("xy", [(0.6, -0.4), (-0.9, 0.8), (1.5, 2.0)])
+def test_camera_projection_matches_bouguet_form(xy):
+    """With all joints at 0 the camera point is (x, y, -12).  The expected
+    pixel is the reference's literal expression (tangential x term
+    kc4 (3 xn^2 + yn^2), pf_model_soa_packed_code.cuh:119-131) in double: a
+    measurement 2 px off it in u and 1 px in v costs 0.1 (4 + 1); a swap of
+    kc3 / kc4 or fc_x / fc_y would move the prediction by pixels."""
+    import esthera_amd as ea
+    p = ea.camera_params_default()
+    x, y = xy
+    xn, yn = x / -12.0, y / -12.0
+    k1, k2, k3, k4 = (float(v) for v in p.kc)
+    r2 = xn * xn + yn * yn
+    dxx = 2 * k3 * xn * yn + k4 * (3 * xn * xn + yn * yn)
+    dxy = k3 * (3 * yn * yn + xn * xn) + 2 * k4 * xn * yn
+    u = p.fc[0] * ((1 + k1 * r2 + k2 * r2 * r2) * xn + dxx) + p.cc[0]
+    v = p.fc[1] * ((1 + k1 * r2 + k2 * r2 * r2) * yn + dxy) + p.cc[1]
+    lw = _camera_lw(p, [0.0] * 5, x, y, (u + 2.0, v - 1.0), [0.0] * 5)
+    np.testing.assert_allclose(lw, -0.1 * 5.0, rtol=2e-3)
+    # the pinhole limit: no distortion, unit focal length, no offset
+    q = ea.CameraParams()
+    q.fc[0] = q.fc[1] = 1.0
+    lw = _camera_lw(q, [0.0] * 5, x, y, (xn, yn + 0.5), [0.0] * 5)
+    np.testing.assert_allclose(lw, -0.1 * 0.25, rtol=1e-4)
+
+
+def test_camera_nan_cost_is_zero_weight():
+    """Base at the origin, joints (0, 0, pi, 0, pi): joint 2 turns the first
+    link's offset back onto the base's plane and joint 4 the third's, so the
+    point reaches Z = 0 with X = 0, and X / Z is 0 / 0.  Its weight is 0
+    (log -inf), never NaN."""
+    import esthera_amd as ea
+    p = ea.camera_params_default()
+    a = [0.0, 0.0, np.pi, 0.0, np.pi]
+    lw = _camera_lw(p, a, 0.0, 0.0, (p.cc[0], p.cc[1]), a)
+    assert lw == -np.inf
+
+
 def _carry_case(threshold, weights):
     """Zero process noise and one common initial state: every particle follows
     the same deterministic path, so every weight is the same function of the
