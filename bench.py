@@ -73,6 +73,31 @@ def run_plain(ea, eng, resampler, steps, warmup, graph=False, timing=1, every=1)
     return t1 - t0, stage, err
 
 
+def run_layouts(ea, eng):
+    """Device time per step of the headline filter (unicycle, S = 3) and of the
+    robot arm at the same size (S = 9) with the state resident as SOA, AOS
+    and PACKED (the layout autotuner's measurement)."""
+    import numpy as np
+    out = {}
+    names = ["soa", "aos", "packed"]
+    params, ctrl, meas, truth = ea.unicycle_scenario(42, L, 50.0, 1)
+    d = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=G, t=T, topology=ea.TOPO_TORUS, resampler=ea.RESAMPLE_SYSTEMATIC,
+                       weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
+    us, best = ea.autotune_layout(eng, d, params, ctrl[0], meas[0], 0.1)
+    out["unicycle_2m"] = {"us_per_step": dict(zip(names, map(float, us))), "best": names[best]}
+    c = ea.RobotArm5Control()
+    m = ea.RobotArm5Measurement()
+    for i in range(5):
+        c.angles[i] = 0.05
+        m.angles[i] = 2.35
+    m.x, m.y = 1.0, 2.0
+    d = ea.filter_desc(ea.MODEL_ROBOTARM5, m=M, G=G, t=T, topology=ea.TOPO_TORUS, resampler=ea.RESAMPLE_SYSTEMATIC,
+                       weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
+    us, best = ea.autotune_layout(eng, d, None, c, m, 0.0589)
+    out["robotarm_2m"] = {"us_per_step": dict(zip(names, map(float, us))), "best": names[best]}
+    return out
+
+
 def cpu_baseline(budget_s=15.0, max_steps=200):
     """The oracle's FP32 restatement of the reference step (the same MTGP
     stream, bitonic network, scan and resampler), OpenMP over sub-filters, on
@@ -258,6 +283,7 @@ def main():
     ap.add_argument("--graph-only", action="store_true", help="only the HIP-graph replay run (profiling)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 10k-filter batched workload")
     ap.add_argument("--no-served", action="store_true", help="skip the served request-stream workload")
+    ap.add_argument("--no-layouts", action="store_true", help="skip the state-layout comparison")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -320,6 +346,8 @@ def main():
         out["alias"] = {"value": N * max(1, args.steps // 2) / asecs, "unit": "particles/s",
                         "ms_per_step": asecs * 1e3 / max(1, args.steps // 2),
                         "stages_us": stage_dict(astage, fused=False)}
+    if not args.no_layouts:
+        out["layouts"] = run_layouts(ea, eng)
     if not args.no_batched:
         out["batched"] = run_batched(ea, eng)
     if not args.no_served:

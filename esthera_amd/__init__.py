@@ -23,6 +23,7 @@ TOPO_NONE, TOPO_RING, TOPO_TORUS, TOPO_ALL2ALL = 0, 1, 2, 3
 RESAMPLE_MULTINOMIAL, RESAMPLE_SYSTEMATIC, RESAMPLE_ALIAS = 0, 1, 2
 WEIGHTS_LINEAR, WEIGHTS_LOG = 0, 1
 RNG_MTGP_COMPAT, RNG_MTGP_PER_FILTER = 0, 1
+LAYOUT_SOA, LAYOUT_AOS, LAYOUT_PACKED = 0, 1, 2
 FRAME_FLOATS = 336
 MAX_LANDMARKS = 64
 STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12, MODEL_UNICYCLE10: 3, MODEL_ROBOTARM5_CAMERA: 9}
@@ -35,7 +36,7 @@ class FilterDesc(C.Structure):
                 ("num_transfer", C.c_uint32), ("topology", C.c_int), ("torus_dimx", C.c_int),
                 ("torus_dimy", C.c_int), ("resampler", C.c_int), ("resampling_threshold", C.c_float),
                 ("adaptive", C.c_int), ("weights", C.c_int), ("rng", C.c_int), ("seed", C.c_uint32),
-                ("carry_weights", C.c_int)]
+                ("carry_weights", C.c_int), ("state_layout", C.c_int)]
 
 
 class Estimate(C.Structure):
@@ -191,6 +192,8 @@ _SIGS = {
     "esth_batch_info": ([_P, _P, _P, _P], C.c_int),
     "esth_autotune": ([_P, _P, _P, C.c_size_t, _P, _P, C.c_float, _P], C.c_int),
     "esth_autotune_pick": ([_P, C.c_uint32, C.c_uint32, _P], C.c_int),
+    "esth_autotune_layout": ([_P, _P, _P, C.c_size_t, _P, _P, C.c_float, _P, _P], C.c_int),
+    "esth_state_convert": ([C.c_int, C.c_size_t, C.c_int, _P, C.c_int, _P], C.c_int),
     "esth_server_create": ([_P, C.c_int, C.c_size_t, _P], C.c_int),
     "esth_server_configure": ([_P, C.c_int, C.c_int], C.c_int),
     "esth_server_wave_stats": ([_P, _P, _P, _P], C.c_int),
@@ -335,12 +338,14 @@ class Engine:
 
 
 def filter_desc(model, m=256, G=16, t=1, topology=TOPO_RING, resampler=RESAMPLE_MULTINOMIAL, threshold=1.0,
-                adaptive=0, weights=WEIGHTS_LINEAR, rng=RNG_MTGP_COMPAT, seed=1234, dimx=0, dimy=0, carry=0):
+                adaptive=0, weights=WEIGHTS_LINEAR, rng=RNG_MTGP_COMPAT, seed=1234, dimx=0, dimy=0, carry=0,
+                layout=LAYOUT_SOA):
     d = FilterDesc()
     d.model, d.num_particles, d.num_blocks, d.num_transfer = model, m, G, t
     d.topology, d.torus_dimx, d.torus_dimy = topology, dimx, dimy
     d.resampler, d.resampling_threshold, d.adaptive = resampler, threshold, adaptive
     d.weights, d.rng, d.seed, d.carry_weights = weights, rng, seed, carry
+    d.state_layout = layout
     return d
 
 
@@ -600,6 +605,25 @@ def autotune(engine, desc, params, control, measurement, dt):
     psize = C.sizeof(params) if params is not None else 0
     check(lib().esth_autotune(engine.h, ptr(desc), ptr(params), psize, ptr(control), ptr(measurement), dt, ptr(us)))
     return us
+
+
+def autotune_layout(engine, desc, params, control, measurement, dt):
+    """Device time per step (us) of desc's filter in layouts SOA, AOS, PACKED, and the fastest."""
+    us = np.zeros(3, np.float32)
+    best = C.c_int()
+    psize = C.sizeof(params) if params is not None else 0
+    check(lib().esth_autotune_layout(engine.h, ptr(desc), ptr(params), psize, ptr(control), ptr(measurement), dt,
+                                     ptr(us), C.byref(best)))
+    return us, best.value
+
+
+def state_convert(state, state_dim, n, src_layout, dst_layout):
+    """A host buffer of n particles of state_dim floats from one LAYOUT_* to another."""
+    src = np.ascontiguousarray(state, dtype=np.float32).reshape(-1)
+    assert src.size == state_dim * n
+    dst = np.empty_like(src)
+    check(lib().esth_state_convert(state_dim, n, src_layout, ptr(src), dst_layout, ptr(dst)))
+    return dst
 
 
 def autotune_pick(us_per_mparticle, particles, min_m=256):

@@ -183,6 +183,8 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
             return esth_fail(ESTH_ERR_INVALID, "batch: filter %d differs from filter 0 in model/topology/t/resampler/weights", i);
         if (d.rng != ESTH_RNG_MTGP_PER_FILTER)
             return esth_fail(ESTH_ERR_INVALID, "batch: filter %d must use ESTH_RNG_MTGP_PER_FILTER", i);
+        if (d.state_layout != ESTH_LAYOUT_SOA)
+            return esth_fail(ESTH_ERR_UNSUPPORTED, "batch: filter %d: batches keep their state in ESTH_LAYOUT_SOA", i);
         if (d.num_particles < 64) return esth_fail(ESTH_ERR_UNSUPPORTED, "batch: sub-filters need m >= 64 (filter %d)", i);
     }
     ESTH_HIP(hipSetDevice(e->device));
@@ -646,6 +648,43 @@ extern "C" int esth_autotune(esth_engine* e, const esth_filter_desc* base, const
         esth_filter_destroy(f);
         if (rc != ESTH_OK) return rc;
         us_per_mparticle[k] = us[1] + us[2] + us[3] + us[4] + us[5];
+    }
+    return ESTH_OK;
+}
+
+/* Device time per step of `base`'s filter in each state layout: 3 warm-up
+ * steps, then 10 steps timed stage by stage (rng wait excluded). */
+extern "C" int esth_autotune_layout(esth_engine* e, const esth_filter_desc* base, const void* params,
+                                    size_t params_size, const void* control, const void* measurement, float dt,
+                                    float us_per_step[3], int* best) {
+    if (!e || !base || !us_per_step) return esth_fail(ESTH_ERR_INVALID, "null");
+    for (int L = ESTH_LAYOUT_SOA; L <= ESTH_LAYOUT_PACKED; ++L) {
+        esth_filter_desc d = *base;
+        d.state_layout = L;
+        esth_filter* f = nullptr;
+        ESTH_TRY(esth_filter_create(e, &d, params, params_size, &f));
+        int S, V;
+        esth_model_dims(d.model, &S, &V);
+        std::vector<float> zero(S, 0.0f);
+        int rc = esth_filter_init_uniform(f, zero.data());
+        for (int i = 0; rc == ESTH_OK && i < 3; ++i) rc = esth_filter_step(f, control, measurement, dt);
+        if (rc == ESTH_OK) rc = esth_filter_sync(f);
+        if (rc == ESTH_OK) rc = esth_filter_set_timing(f, 2);
+        for (int i = 0; rc == ESTH_OK && i < 10; ++i) rc = esth_filter_step(f, control, measurement, dt);
+        float us[6];
+        if (rc == ESTH_OK) rc = esth_filter_stage_times(f, us);
+        esth_filter_destroy(f);
+        if (rc != ESTH_OK) return rc;
+        float tot = 0.0f;
+        for (int k = 1; k < 6; ++k)
+            if (us[k] == us[k]) tot += us[k];
+        us_per_step[L] = tot;
+    }
+    if (best) {
+        int b = ESTH_LAYOUT_SOA;
+        for (int L = ESTH_LAYOUT_AOS; L <= ESTH_LAYOUT_PACKED; ++L)
+            if (us_per_step[L] < us_per_step[b]) b = L;
+        *best = b;
     }
     return ESTH_OK;
 }

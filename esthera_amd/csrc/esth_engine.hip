@@ -167,6 +167,20 @@ int esth_check_desc(const esth_filter_desc* d, int* S, int* V) {
         return esth_fail(ESTH_ERR_INVALID, "bad weights mode %d", d->weights);
     if (d->rng != ESTH_RNG_MTGP_COMPAT && d->rng != ESTH_RNG_MTGP_PER_FILTER)
         return esth_fail(ESTH_ERR_INVALID, "bad rng mode %d", d->rng);
+    if (d->state_layout < ESTH_LAYOUT_SOA || d->state_layout > ESTH_LAYOUT_PACKED)
+        return esth_fail(ESTH_ERR_INVALID, "bad state layout %d", d->state_layout);
+    return ESTH_OK;
+}
+
+extern "C" int esth_state_convert(int S, size_t n, int from, const float* src, int to, float* dst) {
+    if (S < 1 || S > ESTH_MAX_STATE_DIM) return esth_fail(ESTH_ERR_INVALID, "state dimension %d", S);
+    if (from < ESTH_LAYOUT_SOA || from > ESTH_LAYOUT_PACKED || to < ESTH_LAYOUT_SOA || to > ESTH_LAYOUT_PACKED)
+        return esth_fail(ESTH_ERR_INVALID, "bad layout %d -> %d", from, to);
+    if (!src || !dst) return esth_fail(ESTH_ERR_INVALID, "null buffer");
+    const size_t total = (size_t)S * n;
+    if (src < dst + total && dst < src + total) return esth_fail(ESTH_ERR_INVALID, "overlapping buffers");
+    for (int c = 0; c < S; ++c)
+        for (size_t i = 0; i < n; ++i) dst[esth_state_index(to, S, n, c, i)] = src[esth_state_index(from, S, n, c, i)];
     return ESTH_OK;
 }
 
@@ -255,24 +269,38 @@ int esth_launch_mtgp(const uint32_t* d_params, const uint32_t* d_status_in, uint
     return ESTH_OK;
 }
 
+template <int MODEL>
+static void launch_sample_weight_t(int layout, int grid, float* state, uint32_t N, const float* R, const float* frame,
+                                   int log_w, float* w, int carry, hipStream_t s) {
+    if (layout == ESTH_LAYOUT_AOS)
+        hipLaunchKernelGGL((k_sample_weight<MODEL, ESTH_LAYOUT_AOS>), dim3(grid), dim3(256), 0, s, state, N, R, frame,
+                           log_w, w, carry);
+    else if (layout == ESTH_LAYOUT_PACKED)
+        hipLaunchKernelGGL((k_sample_weight<MODEL, ESTH_LAYOUT_PACKED>), dim3(grid), dim3(256), 0, s, state, N, R,
+                           frame, log_w, w, carry);
+    else
+        hipLaunchKernelGGL((k_sample_weight<MODEL, ESTH_LAYOUT_SOA>), dim3(grid), dim3(256), 0, s, state, N, R, frame,
+                           log_w, w, carry);
+}
+
 int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame, int log_w,
-                              float* w, int carry, hipStream_t s) {
+                              float* w, int carry, int layout, hipStream_t s) {
     const int grid = grid_for(N / 2, 256);
     switch (model) {
     case ESTH_MODEL_ROBOTARM5:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
+        launch_sample_weight_t<ESTH_MODEL_ROBOTARM5>(layout, grid, state, N, R, frame, log_w, w, carry, s);
         break;
     case ESTH_MODEL_UNICYCLE:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
+        launch_sample_weight_t<ESTH_MODEL_UNICYCLE>(layout, grid, state, N, R, frame, log_w, w, carry, s);
         break;
     case ESTH_MODEL_DH6:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_DH6>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
+        launch_sample_weight_t<ESTH_MODEL_DH6>(layout, grid, state, N, R, frame, log_w, w, carry, s);
         break;
     case ESTH_MODEL_UNICYCLE10:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
+        launch_sample_weight_t<ESTH_MODEL_UNICYCLE10>(layout, grid, state, N, R, frame, log_w, w, carry, s);
         break;
     case ESTH_MODEL_ROBOTARM5_CAMERA:
-        hipLaunchKernelGGL(k_sample_weight<ESTH_MODEL_ROBOTARM5_CAMERA>, dim3(grid), dim3(256), 0, s, state, N, R, frame, log_w, w, carry);
+        launch_sample_weight_t<ESTH_MODEL_ROBOTARM5_CAMERA>(layout, grid, state, N, R, frame, log_w, w, carry, s);
         break;
     default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
     }
@@ -284,20 +312,21 @@ static int log2u(uint32_t m) { int k = 0; while ((1u << k) < m) ++k; return k; }
 
 template <int LOGM, int S>
 static void launch_sort_t(float* w, const float* st_in, float* st_out, uint32_t N, uint32_t G, float2* winner,
-                          int* perm, uint32_t t, float* bt_w, float* bt_s, float* part, int log_w, hipStream_t s) {
+                          int* perm, uint32_t t, float* bt_w, float* bt_s, float* part, int log_w, int layout,
+                          hipStream_t s) {
     using TL = esth_tile<LOGM>;
     const uint64_t waves = (G + (ESTH_WAVE / TL::LPS) - 1) / (ESTH_WAVE / TL::LPS);
     hipLaunchKernelGGL((k_sort<LOGM, S>), dim3(grid_for(waves * ESTH_WAVE, 256)), dim3(256), 0, s, w, st_in,
-                       st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w);
+                       st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, layout);
 }
 
 template <int LOGM, int S>
 static void launch_resample_t(float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
-                              uint32_t G, float thr, int systematic, int log_w, int* sel, hipStream_t s) {
+                              uint32_t G, float thr, int systematic, int log_w, int* sel, int layout, hipStream_t s) {
     using TL = esth_tile<LOGM>;
     const uint64_t waves = (G + (ESTH_WAVE / TL::LPS) - 1) / (ESTH_WAVE / TL::LPS);
     hipLaunchKernelGGL((k_resample<LOGM, S>), dim3(grid_for(waves * ESTH_WAVE, 256)), dim3(256), 0, s, st_main,
-                       st_tmp, w, rnd, N, G, thr, systematic, log_w, sel);
+                       st_tmp, w, rnd, N, G, thr, systematic, log_w, sel, layout);
 }
 
 #define ESTH_DISPATCH_LOGM(FN, S, ...)                 \
@@ -317,9 +346,9 @@ static void launch_resample_t(float* st_main, const float* st_tmp, const float* 
 
 int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_out, uint32_t N, uint32_t G,
                      float2* winner, int* perm, uint32_t t, float* bt_w, float* bt_s, float* part, int log_w,
-                     hipStream_t s) {
+                     int layout, hipStream_t s) {
     const int lm = log2u(m);
-#define ESTH_SORT_ARGS w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, s
+#define ESTH_SORT_ARGS w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, layout, s
     switch (S) {
     case 1: ESTH_DISPATCH_LOGM(launch_sort_t, 1, ESTH_SORT_ARGS); break;
     case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_UNI_S, ESTH_SORT_ARGS); break;
@@ -367,13 +396,13 @@ int esth_launch_exchange(float* st, float* w, int S, uint32_t N, uint32_t m, uin
 
 int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w, const float* rnd,
                          uint32_t N, uint32_t m, uint32_t G, float thr, int adaptive, int log_w, int* sel, int carry,
-                         hipStream_t s) {
+                         int layout, hipStream_t s) {
     if (resampler == ESTH_RESAMPLE_ALIAS) {
         switch (S) {
-        case 1: hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry); break;
-        case ESTH_UNI_S: hipLaunchKernelGGL(k_alias<ESTH_UNI_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry); break;
-        case ESTH_RA5_S: hipLaunchKernelGGL(k_alias<ESTH_RA5_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry); break;
-        case ESTH_DH6_S: hipLaunchKernelGGL(k_alias<ESTH_DH6_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry); break;
+        case 1: hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry, layout); break;
+        case ESTH_UNI_S: hipLaunchKernelGGL(k_alias<ESTH_UNI_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry, layout); break;
+        case ESTH_RA5_S: hipLaunchKernelGGL(k_alias<ESTH_RA5_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry, layout); break;
+        case ESTH_DH6_S: hipLaunchKernelGGL(k_alias<ESTH_DH6_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry, layout); break;
         default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
         }
         ESTH_HIP(hipGetLastError());
@@ -382,10 +411,10 @@ int esth_launch_resample(int resampler, int S, float* st_main, const float* st_t
     const int lm = log2u(m);
     const int sys = resampler == ESTH_RESAMPLE_SYSTEMATIC;
     switch (S) {
-    case 1: ESTH_DISPATCH_LOGM(launch_resample_t, 1, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
-    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_UNI_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
-    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_RA5_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
-    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_DH6_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, s); break;
+    case 1: ESTH_DISPATCH_LOGM(launch_resample_t, 1, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s); break;
+    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_UNI_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s); break;
+    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_RA5_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s); break;
+    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_DH6_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s); break;
     default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
     }
     ESTH_HIP(hipGetLastError());
@@ -396,12 +425,13 @@ template <int LOGM, int S>
 static void launch_resample_argmax_t(float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
                                      uint32_t G, float thr, int systematic, int log_w, const float2* winner,
                                      const float* part, const float* best_src, size_t plane_stride, uint32_t g_stride,
-                                     float2* result, double* est, const esth_xch& xch, int carry, hipStream_t s) {
+                                     float2* result, double* est, const esth_xch& xch, int carry, int layout,
+                                     hipStream_t s) {
     using TL = esth_tile<LOGM>;
     const uint64_t waves = (G + (ESTH_WAVE / TL::LPS) - 1) / (ESTH_WAVE / TL::LPS);
     hipLaunchKernelGGL((k_resample_argmax<LOGM, S>), dim3(1 + grid_for(waves * ESTH_WAVE, 256)), dim3(256), 0, s,
                        st_main, st_tmp, w, rnd, N, G, thr, systematic, log_w, winner, part, best_src, plane_stride,
-                       g_stride, result, est, xch, carry);
+                       g_stride, result, est, xch, carry, layout);
 }
 
 int esth_launch_resample_argmax(int resampler, int S, float* st_main, const float* st_tmp, float* w,
@@ -409,7 +439,7 @@ int esth_launch_resample_argmax(int resampler, int S, float* st_main, const floa
                                 const float2* winner, const float* part, const float* best_src, size_t plane_stride,
                                 uint32_t g_stride, float2* result, double* est, uint32_t t, int topo, int dimx,
                                 int dimy, const float* bt_w, const float* bt_s, uint32_t* top, int carry,
-                                hipStream_t s) {
+                                int layout, hipStream_t s) {
     if (resampler == ESTH_RESAMPLE_ALIAS) return esth_fail(ESTH_ERR_UNSUPPORTED, "fused argmax: scan resamplers only");
     const int lm = log2u(m);
     const int sys = resampler == ESTH_RESAMPLE_SYSTEMATIC;
@@ -421,7 +451,7 @@ int esth_launch_resample_argmax(int resampler, int S, float* st_main, const floa
         }
         xch = {bt_w, bt_s, top, w, G, t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, dimx, dimy};
     }
-#define ESTH_RA_ARGS st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, winner, part, best_src, plane_stride, g_stride, result, est, xch, carry, s
+#define ESTH_RA_ARGS st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, winner, part, best_src, plane_stride, g_stride, result, est, xch, carry, layout, s
     switch (S) {
     case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_UNI_S, ESTH_RA_ARGS); break;
     case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_RA5_S, ESTH_RA_ARGS); break;
@@ -480,7 +510,8 @@ extern "C" int esth_stage_sample_weight(int model, float* d_state, uint32_t N, c
     float* dfr = nullptr;
     ESTH_HIP(hipMalloc((void**)&dfr, sizeof(float) * ESTH_FRAME_FLOATS));
     ESTH_HIP(hipMemcpy(dfr, frame, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice));
-    int rc = esth_launch_sample_weight(model, d_state, N, d_random, dfr, weights == ESTH_WEIGHTS_LOG, d_w, 0, 0);
+    int rc = esth_launch_sample_weight(model, d_state, N, d_random, dfr, weights == ESTH_WEIGHTS_LOG, d_w, 0,
+                                       ESTH_LAYOUT_SOA, 0);
     ESTH_HIP(hipDeviceSynchronize());
     hipFree(dfr);
     return rc;
@@ -497,7 +528,7 @@ extern "C" int esth_stage_sort(float* d_w, const float* d_in, float* d_out, int 
     ESTH_HIP(hipMalloc((void**)&part, sizeof(float) * (size_t)G * (2 + S)));
     if (!d_winner) ESTH_HIP(hipMalloc((void**)&win, sizeof(float2) * G));
     int rc = esth_launch_sort(S, m, d_w, d_in, d_out, N, G, d_winner ? (float2*)d_winner : win, d_perm, 0, nullptr,
-                              nullptr, part, 0, 0);
+                              nullptr, part, 0, ESTH_LAYOUT_SOA, 0);
     ESTH_HIP(hipDeviceSynchronize());
     hipFree(part);
     if (win) hipFree(win);
@@ -558,7 +589,7 @@ extern "C" int esth_stage_resample(int resampler, float* dst, const float* src, 
     int s_, v_;
     ESTH_TRY(esth_check_desc(&d, &s_, &v_));
     int rc = esth_launch_resample(resampler, S, dst, src, d_w, d_random, m * G, m, G, thr, adaptive,
-                                  weights == ESTH_WEIGHTS_LOG, d_sel, 0, 0);
+                                  weights == ESTH_WEIGHTS_LOG, d_sel, 0, ESTH_LAYOUT_SOA, 0);
     ESTH_HIP(hipDeviceSynchronize());
     return rc;
 }
@@ -567,7 +598,8 @@ extern "C" int esth_stage_alias_table(const float* d_w, uint32_t m, uint32_t G, 
     if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES) return esth_fail(ESTH_ERR_INVALID, "m=%u", m);
     if (!d_prob || !d_alias) return esth_fail(ESTH_ERR_INVALID, "null output");
     hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, 0, (float*)nullptr, (const float*)nullptr, d_w,
-                       (const float*)nullptr, m * G, m, 1.0f, 0, weights == ESTH_WEIGHTS_LOG, (int*)nullptr, d_prob, d_alias, 0);
+                       (const float*)nullptr, m * G, m, 1.0f, 0, weights == ESTH_WEIGHTS_LOG, (int*)nullptr, d_prob, d_alias, 0,
+                       ESTH_LAYOUT_SOA);
     ESTH_HIP(hipGetLastError());
     ESTH_HIP(hipDeviceSynchronize());
     return ESTH_OK;

@@ -182,11 +182,35 @@ extern "C" int esth_filter_init_uniform(esth_filter* f, const float* s) {
     return esth_filter_init(f, h.data());
 }
 
+/* the device state from / to host SOA planes, through the filter's layout */
+static int upload_state(esth_filter* f, const float* soa) {
+    const size_t n = (size_t)f->S * f->N;
+    if (f->d.state_layout == ESTH_LAYOUT_SOA) {
+        ESTH_HIP(hipMemcpy(f->state, soa, sizeof(float) * n, hipMemcpyHostToDevice));
+        return ESTH_OK;
+    }
+    std::vector<float> h(n);
+    ESTH_TRY(esth_state_convert(f->S, f->N, ESTH_LAYOUT_SOA, soa, f->d.state_layout, h.data()));
+    ESTH_HIP(hipMemcpy(f->state, h.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    return ESTH_OK;
+}
+
+static int download_state(esth_filter* f, float* soa) {
+    const size_t n = (size_t)f->S * f->N;
+    if (f->d.state_layout == ESTH_LAYOUT_SOA) {
+        ESTH_HIP(hipMemcpy(soa, f->state, sizeof(float) * n, hipMemcpyDeviceToHost));
+        return ESTH_OK;
+    }
+    std::vector<float> h(n);
+    ESTH_HIP(hipMemcpy(h.data(), f->state, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return esth_state_convert(f->S, f->N, f->d.state_layout, h.data(), ESTH_LAYOUT_SOA, soa);
+}
+
 extern "C" int esth_filter_init(esth_filter* f, const float* soa) {
     if (!f || !soa) return esth_fail(ESTH_ERR_INVALID, "null filter/state");
     ESTH_HIP(hipSetDevice(f->e->device));
     ESTH_HIP(hipStreamSynchronize(f->stream));
-    ESTH_HIP(hipMemcpy(f->state, soa, sizeof(float) * (size_t)f->S * f->N, hipMemcpyHostToDevice));
+    ESTH_TRY(upload_state(f, soa));
     if (f->d.carry_weights) {   /* a new particle set starts from equal weights */
         std::vector<float> w0(f->N, f->d.weights == ESTH_WEIGHTS_LOG ? 0.0f : 1.0f);
         ESTH_HIP(hipMemcpy(f->w, w0.data(), sizeof(float) * f->N, hipMemcpyHostToDevice));
@@ -258,11 +282,12 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
     float* R = f->R[p];
     if (timing) ESTH_HIP(hipEventRecord(ev[1], s));
     /* sampling_importance with Box-Muller fused (pf_generic.c:521-533) */
-    ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, R, d_frame, log_w, f->w, d.carry_weights, s));
+    ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, R, d_frame, log_w, f->w, d.carry_weights,
+                                       d.state_layout, s));
     if (timing) ESTH_HIP(hipEventRecord(ev[2], s));
     /* block_sort (pf_generic.c:572-585) */
     ESTH_TRY(esth_launch_sort(f->S, f->m, f->w, f->state, f->tmp, N, f->G, f->winner, nullptr, f->t, f->bt_w,
-                              f->bt_s, f->part, log_w, s));
+                              f->bt_s, f->part, log_w, d.state_layout, s));
     if (all) ESTH_HIP(hipEventRecord(ev[3], s));
     /* reduce (pf_generic.c:604-615), result kept on the device.  With a scan
      * resampler it rides in the resampling launch (below); its best particle
@@ -284,10 +309,12 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
                                              f->G, d.resampling_threshold, log_w, f->winner, f->part,
                                              bt ? f->bt_s : f->tmp, bt ? (size_t)f->G * f->t : (size_t)N,
                                              bt ? f->t : f->m, f->result, f->est, f->t, d.topology, f->dimx,
-                                             f->dimy, f->bt_w, f->bt_s, f->top, d.carry_weights, s));
+                                             f->dimy, f->bt_w, f->bt_s, f->top, d.carry_weights, d.state_layout,
+                                             s));
     } else {
         ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m, f->G,
-                                      d.resampling_threshold, d.adaptive, log_w, nullptr, d.carry_weights, s));
+                                      d.resampling_threshold, d.adaptive, log_w, nullptr, d.carry_weights,
+                                      d.state_layout, s));
     }
     if (all) ESTH_HIP(hipEventRecord(ev[6], s));
     return ESTH_OK;
@@ -452,12 +479,13 @@ extern "C" int esth_filter_estimate(esth_filter* f, esth_estimate* out) {
 extern "C" int esth_filter_read_particles(esth_filter* f, float* soa, float* w) {
     if (!f) return esth_fail(ESTH_ERR_INVALID, "null");
     ESTH_HIP(hipStreamSynchronize(f->stream));
-    if (soa) ESTH_HIP(hipMemcpy(soa, f->state, sizeof(float) * (size_t)f->S * f->N, hipMemcpyDeviceToHost));
+    if (soa) ESTH_TRY(download_state(f, soa));
     if (w) ESTH_HIP(hipMemcpy(w, f->w, sizeof(float) * f->N, hipMemcpyDeviceToHost));
     return ESTH_OK;
 }
 
-/* snapshot: magic, S, N, K, steps | state | w | status */
+/* snapshot: magic, S, N, K, steps, model, m, G | state (SOA planes, whatever
+ * the filter's layout) | w | status */
 #define ESTH_SNAP_MAGIC 0x45535448u
 extern "C" int esth_filter_snapshot_size(esth_filter* f, size_t* bytes) {
     if (!f || !bytes) return esth_fail(ESTH_ERR_INVALID, "null");
@@ -475,7 +503,7 @@ extern "C" int esth_filter_snapshot(esth_filter* f, void* buf, size_t bytes) {
     hdr[0] = ESTH_SNAP_MAGIC; hdr[1] = (uint64_t)f->S; hdr[2] = f->N; hdr[3] = (uint64_t)f->K; hdr[4] = f->steps;
     hdr[5] = f->d.model; hdr[6] = f->m; hdr[7] = f->G;
     char* p = (char*)buf + 8 * sizeof(uint64_t);
-    ESTH_HIP(hipMemcpy(p, f->state, sizeof(float) * (size_t)f->S * f->N, hipMemcpyDeviceToHost));
+    ESTH_TRY(download_state(f, (float*)p));
     p += sizeof(float) * (size_t)f->S * f->N;
     ESTH_HIP(hipMemcpy(p, f->w, sizeof(float) * f->N, hipMemcpyDeviceToHost));
     p += sizeof(float) * f->N;
@@ -495,7 +523,7 @@ extern "C" int esth_filter_restore(esth_filter* f, const void* buf, size_t bytes
     ESTH_HIP(hipStreamSynchronize(f->stream));
     ESTH_HIP(hipStreamSynchronize(f->rng_stream));
     const char* p = (const char*)buf + 8 * sizeof(uint64_t);
-    ESTH_HIP(hipMemcpy(f->state, p, sizeof(float) * (size_t)f->S * f->N, hipMemcpyHostToDevice));
+    ESTH_TRY(upload_state(f, (const float*)p));
     p += sizeof(float) * (size_t)f->S * f->N;
     ESTH_HIP(hipMemcpy(f->w, p, sizeof(float) * f->N, hipMemcpyHostToDevice));
     p += sizeof(float) * f->N;

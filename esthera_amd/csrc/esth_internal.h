@@ -51,10 +51,10 @@ const uint32_t* esth_param_table();
 int esth_launch_mtgp(const uint32_t* d_params, const uint32_t* d_status_in, uint32_t* d_status_out,
                      const esth_mtgp_stream* d_streams, int nstreams, void* d_out, int float_mode, hipStream_t s);
 int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame,
-                              int log_w, float* w, int carry, hipStream_t s);
+                              int log_w, float* w, int carry, int layout, hipStream_t s);
 int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_out, uint32_t N,
                      uint32_t G, float2* winner, int* perm, uint32_t t, float* bt_w, float* bt_s,
-                     float* part, int log_w, hipStream_t s);
+                     float* part, int log_w, int layout, hipStream_t s);
 int esth_launch_argmax(int S, const float2* winner, uint32_t G, int log_w, const float* part,
                        const float* sorted, uint32_t N, uint32_t m, float2* result, double* est,
                        hipStream_t s);
@@ -63,7 +63,7 @@ int esth_launch_exchange(float* st, float* w, int S, uint32_t N, uint32_t m, uin
                          uint32_t* top, hipStream_t s);
 int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w,
                          const float* rnd, uint32_t N, uint32_t m, uint32_t G, float thr,
-                         int adaptive, int log_w, int* sel, int carry, hipStream_t s);
+                         int adaptive, int log_w, int* sel, int carry, int layout, hipStream_t s);
 /* scan resampling with the step's argmax + posterior and the exchange (its
  * receiving side, plus the all-to-all top t) in the same launch; the best
  * particle is read from best_src[c * plane_stride + g * g_stride] */
@@ -72,7 +72,7 @@ int esth_launch_resample_argmax(int resampler, int S, float* st_main, const floa
                                 const float2* winner, const float* part, const float* best_src, size_t plane_stride,
                                 uint32_t g_stride, float2* result, double* est, uint32_t t, int topo, int dimx,
                                 int dimy, const float* bt_w, const float* bt_s, uint32_t* top, int carry,
-                                hipStream_t s);
+                                int layout, hipStream_t s);
 /* the global top t of the compact best-t buffer (all-to-all exchange) */
 int esth_launch_topt(const float* bt_w, uint32_t n, uint32_t t, uint32_t* top, hipStream_t s);
 

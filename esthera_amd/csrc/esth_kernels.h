@@ -220,19 +220,99 @@ __device__ __forceinline__ void esth_sample_pair_regs(float (&s0)[esth_model_tra
 
 /* carry: multiply the particles' carried weights by the likelihoods (add the
  * log-likelihoods in log mode) instead of overwriting them */
-template <int MODEL>
+/* particles p and p+1 (p even) of a state in layout L: SOA one float2 per
+ * plane; AOS the pair's 2S consecutive floats as float2s; PACKED, per group
+ * of w planes, the pair's 2w consecutive floats (float4s where w is 2 or 4:
+ * a group starts at a multiple of 4N floats and the pair at 2pw) */
+template <int S, int L>
+__device__ __forceinline__ void esth_load_pair(const float* __restrict__ st, size_t N, uint32_t p, float (&s0)[S],
+                                               float (&s1)[S]) {
+    if constexpr (L == ESTH_LAYOUT_SOA) {
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+            const float2 v = *reinterpret_cast<const float2*>(st + (size_t)c * N + p);
+            s0[c] = v.x;
+            s1[c] = v.y;
+        }
+    } else if constexpr (L == ESTH_LAYOUT_AOS) {
+        const float* b = st + (size_t)p * S;
+        float run[2 * S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const float2 v = *reinterpret_cast<const float2*>(b + 2 * k);
+            run[2 * k] = v.x;
+            run[2 * k + 1] = v.y;
+        }
+#pragma unroll
+        for (int c = 0; c < S; ++c) { s0[c] = run[c]; s1[c] = run[S + c]; }
+    } else {
+#pragma unroll
+        for (int q = 0; q < S; q += 4) {
+            const int w = S - q < 4 ? S - q : 4;
+            const float* b = st + (size_t)q * N + (size_t)p * w;
+            float run[8];
+            if (w == 4 || w == 2) {
+#pragma unroll
+                for (int k = 0; k < w / 2; ++k) {
+                    const float4 v = *reinterpret_cast<const float4*>(b + 4 * k);
+                    run[4 * k] = v.x; run[4 * k + 1] = v.y; run[4 * k + 2] = v.z; run[4 * k + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < w; ++k) {
+                    const float2 v = *reinterpret_cast<const float2*>(b + 2 * k);
+                    run[2 * k] = v.x;
+                    run[2 * k + 1] = v.y;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < w; ++j) { s0[q + j] = run[j]; s1[q + j] = run[w + j]; }
+        }
+    }
+}
+
+template <int S, int L>
+__device__ __forceinline__ void esth_store_pair(float* __restrict__ st, size_t N, uint32_t p, const float (&s0)[S],
+                                                const float (&s1)[S]) {
+    if constexpr (L == ESTH_LAYOUT_SOA) {
+#pragma unroll
+        for (int c = 0; c < S; ++c) *reinterpret_cast<float2*>(st + (size_t)c * N + p) = make_float2(s0[c], s1[c]);
+    } else if constexpr (L == ESTH_LAYOUT_AOS) {
+        float run[2 * S];
+#pragma unroll
+        for (int c = 0; c < S; ++c) { run[c] = s0[c]; run[S + c] = s1[c]; }
+        float* b = st + (size_t)p * S;
+#pragma unroll
+        for (int k = 0; k < S; ++k) *reinterpret_cast<float2*>(b + 2 * k) = make_float2(run[2 * k], run[2 * k + 1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < S; q += 4) {
+            const int w = S - q < 4 ? S - q : 4;
+            float run[8];
+#pragma unroll
+            for (int j = 0; j < w; ++j) { run[j] = s0[q + j]; run[w + j] = s1[q + j]; }
+            float* b = st + (size_t)q * N + (size_t)p * w;
+            if (w == 4 || w == 2) {
+#pragma unroll
+                for (int k = 0; k < w / 2; ++k)
+                    *reinterpret_cast<float4*>(b + 4 * k) =
+                        make_float4(run[4 * k], run[4 * k + 1], run[4 * k + 2], run[4 * k + 3]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < w; ++k) *reinterpret_cast<float2*>(b + 2 * k) = make_float2(run[2 * k], run[2 * k + 1]);
+            }
+        }
+    }
+}
+
+template <int MODEL, int L = ESTH_LAYOUT_SOA>
 __device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ state, uint32_t N,
                                                         const float* __restrict__ R,
                                                         const float* __restrict__ frame, int log_w,
                                                         float* __restrict__ w, uint32_t p, int carry = 0) {
     using T = esth_model_traits<MODEL>;
     float s0[T::S], s1[T::S], u0[T::V], u1[T::V];
-#pragma unroll
-    for (int c = 0; c < T::S; ++c) {
-        float2 v = *reinterpret_cast<const float2*>(state + (size_t)c * N + p);
-        s0[c] = v.x;
-        s1[c] = v.y;
-    }
+    esth_load_pair<T::S, L>(state, N, p, s0, s1);
 #pragma unroll
     for (int k = 0; k < T::V; ++k) {
         float2 u = *reinterpret_cast<const float2*>(R + (size_t)k * N + p);
@@ -241,9 +321,7 @@ __device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ stat
     }
     float lw0, lw1;
     esth_sample_pair_regs<MODEL>(s0, s1, u0, u1, frame, lw0, lw1);
-#pragma unroll
-    for (int c = 0; c < T::S; ++c)
-        *reinterpret_cast<float2*>(state + (size_t)c * N + p) = make_float2(s0[c], s1[c]);
+    esth_store_pair<T::S, L>(state, N, p, s0, s1);
     float2 wo = log_w ? make_float2(lw0, lw1) : make_float2(esth_expf(lw0), esth_expf(lw1));
     if (carry) {
         const float2 wp = *reinterpret_cast<const float2*>(w + p);
@@ -252,13 +330,13 @@ __device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ stat
     *reinterpret_cast<float2*>(w + p) = wo;
 }
 
-template <int MODEL>
+template <int MODEL, int L>
 __global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state, uint32_t N,
                                                        const float* __restrict__ R,
                                                        const float* __restrict__ frame, int log_w,
                                                        float* __restrict__ w, int carry) {
     const uint32_t p = 2u * (blockIdx.x * blockDim.x + threadIdx.x);
-    if (p < N) esth_sample_weight_pair<MODEL>(state, N, R, frame, log_w, w, p, carry);
+    if (p < N) esth_sample_weight_pair<MODEL, L>(state, N, R, frame, log_w, w, p, carry);
 }
 
 /* ============================================================ block sort
@@ -535,12 +613,42 @@ __global__ __launch_bounds__(1024) void k_argmax(const float2* __restrict__ winn
 /* One sub-filter (one wave, or 64/m sub-filters per wave): sort its weights
  * (block_sort, pf_generic.cl:252-291) and write the sorted weights, the
  * gathered state, the compact best t, the winner and the posterior partials. */
+/* E consecutive particles i0.. of plane c in a state of any layout: SOA
+ * through esth_load_run's vector loads, AOS / PACKED one float each at the
+ * layout's particle stride (S, or the plane's group width) */
+template <int E, int S>
+__device__ __forceinline__ void esth_load_plane_run(const float* __restrict__ st, int layout, size_t N, int c,
+                                                    size_t i0, float (&out)[E]) {
+    if (layout == ESTH_LAYOUT_SOA) {
+        esth_load_run<E>(st + (size_t)c * N + i0, out);
+        return;
+    }
+    const size_t b = esth_state_index(layout, S, N, c, i0);
+    const size_t stride = esth_state_index(layout, S, N, c, i0 + 1) - b;
+#pragma unroll
+    for (int r = 0; r < E; ++r) out[r] = st[b + (size_t)r * stride];
+}
+
+template <int E, int S>
+__device__ __forceinline__ void esth_store_plane_run(float* __restrict__ st, int layout, size_t N, int c, size_t i0,
+                                                     const float (&v)[E]) {
+    if (layout == ESTH_LAYOUT_SOA) {
+        esth_store_run<E>(st + (size_t)c * N + i0, v);
+        return;
+    }
+    const size_t b = esth_state_index(layout, S, N, c, i0);
+    const size_t stride = esth_state_index(layout, S, N, c, i0 + 1) - b;
+#pragma unroll
+    for (int r = 0; r < E; ++r) st[b + (size_t)r * stride] = v[r];
+}
+
 template <int LOGM, int S>
 __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float* __restrict__ st_in,
                                               float* __restrict__ st_out, uint32_t N, uint32_t G, uint32_t g,
                                               bool valid, float2* __restrict__ winner, int* __restrict__ perm,
                                               uint32_t t, float* __restrict__ bt_w, float* __restrict__ bt_s,
-                                              float* __restrict__ part, int log_w, float* stage_w) {
+                                              float* __restrict__ part, int log_w, float* stage_w,
+                                              int layout = ESTH_LAYOUT_SOA) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
@@ -581,7 +689,7 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
     for (int c = 0; c < S; ++c) {
         if (c % PB == 0) {
 #pragma unroll
-            for (int j = 0; j < PB; ++j) esth_load_run<E>(st_in + (size_t)(c + j) * N + base + le, in[j]);
+            for (int j = 0; j < PB; ++j) esth_load_plane_run<E, S>(st_in, layout, N, c + j, base + le, in[j]);
         }
         float* dst = st_out + (size_t)c * N + base;
         float acc = 0.0f;
@@ -621,7 +729,7 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
                                               float2* __restrict__ winner, int* __restrict__ perm,
                                               uint32_t t, float* __restrict__ bt_w,
                                               float* __restrict__ bt_s, float* __restrict__ part,
-                                              int log_w) {
+                                              int log_w, int layout) {
     constexpr int LPS = esth_tile<LOGM>::LPS;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / ESTH_WAVE;
@@ -630,7 +738,7 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
     if (!valid) g = G - 1;
     __shared__ float stage[4][ESTH_WAVE * esth_tile<LOGM>::E];
     esth_sort_sub<LOGM, S>(w, st_in, st_out, N, G, g, valid, winner, perm, t, bt_w, bt_s, part, log_w,
-                           stage[threadIdx.x / ESTH_WAVE]);
+                           stage[threadIdx.x / ESTH_WAVE], layout);
 }
 
 /* ============================================================= exchange */
@@ -759,7 +867,8 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                                                   bool valid, float thr, int systematic, int log_w,
                                                   int* __restrict__ sel, float* cdf_w,
                                                   bool recv = false, esth_xch xch = esth_xch{},
-                                                  int carry = 0, float* w_mut = nullptr) {
+                                                  int carry = 0, float* w_mut = nullptr,
+                                                  int layout = ESTH_LAYOUT_SOA) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS, LOGE = TL::LOGE, LOGL = TL::LOGL;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
@@ -868,9 +977,9 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
         for (int c = 0; c < PB; ++c) {
             float v[E];
             esth_permute_lds<M, E>(in[c], le, cdf, pick, v);
-            if (go) esth_store_run<E>(st_main + (size_t)(c0 + c) * N + base + le, v);
+            if (go) esth_store_plane_run<E, S>(st_main, layout, N, c0 + c, base + le, v);
             /* carried weights: a skipped sub-filter keeps its sorted, exchanged particles */
-            else if (carry && valid) esth_store_run<E>(st_main + (size_t)(c0 + c) * N + base + le, in[c]);
+            else if (carry && valid) esth_store_plane_run<E, S>(st_main, layout, N, c0 + c, base + le, in[c]);
         }
     }
     if (carry && go) {   /* resampled: equal weights again */
@@ -891,7 +1000,7 @@ __global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
                                                   const float* __restrict__ w,
                                                   const float* __restrict__ rnd, uint32_t N, uint32_t G,
                                                   float thr, int systematic, int log_w,
-                                                  int* __restrict__ sel) {
+                                                  int* __restrict__ sel, int layout) {
     using TL = esth_tile<LOGM>;
     __shared__ float cdf_s[4][ESTH_WAVE * TL::E];
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
@@ -900,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_resample(float* __restrict__ st_main,
     const bool valid = g < G;
     if (!valid) g = G - 1;   /* keep the lane busy; its writes are dropped */
     esth_resample_sub<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, sel,
-                               cdf_s[threadIdx.x / ESTH_WAVE]);
+                               cdf_s[threadIdx.x / ESTH_WAVE], false, esth_xch{}, 0, nullptr, layout);
 }
 
 /* The step's argmax + posterior (work-group 0) beside the scan resampling
@@ -919,7 +1028,8 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
                                                          const float* __restrict__ part,
                                                          const float* __restrict__ best_src, size_t plane_stride,
                                                          uint32_t g_stride, float2* __restrict__ result,
-                                                         double* __restrict__ est, esth_xch xch, int carry) {
+                                                         double* __restrict__ est, esth_xch xch, int carry,
+                                                         int layout) {
     using TL = esth_tile<LOGM>;
     __shared__ float cdf_s[4][ESTH_WAVE * TL::E];
     __shared__ float2 sc[128];
@@ -935,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
     const bool valid = g < G;
     if (!valid) g = G - 1;
     esth_resample_sub<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, nullptr,
-                               cdf_s[threadIdx.x / ESTH_WAVE], true, xch, carry, const_cast<float*>(w));
+                               cdf_s[threadIdx.x / ESTH_WAVE], true, xch, carry, const_cast<float*>(w), layout);
 }
 
 /* =========================================================== Vose alias
@@ -987,7 +1097,8 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
                                                const float* w, const float* __restrict__ rnd,
                                                uint32_t N, uint32_t M, uint32_t g, float thr, int adaptive,
                                                int log_w, int* __restrict__ sel, float* __restrict__ prob_out,
-                                               uint32_t* __restrict__ alias_out, int carry = 0) {
+                                               uint32_t* __restrict__ alias_out, int carry = 0,
+                                               int layout = ESTH_LAYOUT_SOA) {
     __shared__ float prob[ESTH_MAX_NUM_PARTICLES];
     __shared__ uint32_t alias[ESTH_MAX_NUM_PARTICLES];
     __shared__ uint32_t sl[ESTH_MAX_NUM_PARTICLES];
@@ -1003,7 +1114,7 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
             if (sel) sel[idx] = -1;
 #pragma unroll
             for (int c = 0; c < S; ++c)
-                if (carry) st_main[(size_t)c * N + idx] = st_tmp[(size_t)c * N + idx];
+                if (carry) st_main[esth_state_index(layout, S, N, c, idx)] = st_tmp[(size_t)c * N + idx];
             return;
         }
     }
@@ -1035,7 +1146,7 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
             if (sel) sel[idx] = -1;
 #pragma unroll
             for (int c = 0; c < S; ++c)
-                if (carry) st_main[(size_t)c * N + idx] = st_tmp[(size_t)c * N + idx];
+                if (carry) st_main[esth_state_index(layout, S, N, c, idx)] = st_tmp[(size_t)c * N + idx];
             return;
         }
     }
@@ -1134,7 +1245,7 @@ __device__ __forceinline__ void esth_alias_sub(float* __restrict__ st_main, cons
     const uint32_t pick = (r1 < prob[col]) ? col : alias[col];
     const size_t sb = (size_t)g * M;
 #pragma unroll
-    for (int c = 0; c < S; ++c) st_main[(size_t)c * N + idx] = st_tmp[(size_t)c * N + sb + pick];
+    for (int c = 0; c < S; ++c) st_main[esth_state_index(layout, S, N, c, idx)] = st_tmp[(size_t)c * N + sb + pick];
     if (sel) sel[idx] = (int)pick;
     /* every item has read its weight (before the first barrier) */
     if (carry) const_cast<float*>(w)[idx] = log_w ? 0.0f : 1.0f;
@@ -1147,9 +1258,9 @@ __global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
                                                 const float* __restrict__ rnd, uint32_t N, uint32_t M,
                                                 float thr, int adaptive, int log_w,
                                                 int* __restrict__ sel, float* __restrict__ prob_out,
-                                                uint32_t* __restrict__ alias_out, int carry) {
+                                                uint32_t* __restrict__ alias_out, int carry, int layout) {
     esth_alias_sub<S>(st_main, st_tmp, w, rnd, N, M, blockIdx.x, thr, adaptive, log_w, sel, prob_out, alias_out,
-                      carry);
+                      carry, layout);
 }
 
 

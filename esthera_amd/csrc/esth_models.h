@@ -22,6 +22,18 @@
 #include "esth_math.h"
 #include "esthera.h"
 
+/* float offset of state variable c of particle i among N particles of S
+ * variables in an ESTH_LAYOUT_* layout */
+ESTH_HD size_t esth_state_index(int layout, int S, size_t N, int c, size_t i) {
+    if (layout == ESTH_LAYOUT_AOS) return i * (size_t)S + (size_t)c;
+    if (layout == ESTH_LAYOUT_PACKED) {
+        const int q = c & ~3;
+        const int w = S - q < 4 ? S - q : 4;
+        return (size_t)q * N + i * (size_t)w + (size_t)(c - q);
+    }
+    return (size_t)c * N + i;
+}
+
 /* ------------------------------------------------------------ robotarm5
  * src/models/robotarm/opencl/pf_model.cl:52-131, pf_model.h:26-32.
  * State planes: 0..4 joint angles, 5 x, 6 y, 7 vx, 8 vy.

@@ -349,6 +349,8 @@ extern "C" int esth_server_open(esth_server* s, const esth_filter_desc* desc, co
     if (desc->rng != ESTH_RNG_MTGP_PER_FILTER)
         return esth_fail(ESTH_ERR_INVALID, "server open: tracks use ESTH_RNG_MTGP_PER_FILTER");
     if (desc->num_particles < 64) return esth_fail(ESTH_ERR_UNSUPPORTED, "server open: sub-filters need m >= 64");
+    if (desc->state_layout != ESTH_LAYOUT_SOA)
+        return esth_fail(ESTH_ERR_UNSUPPORTED, "server open: tracks keep their state in ESTH_LAYOUT_SOA");
     size_t psize = 0;
     ESTH_TRY(esth_params_size(desc->model, &psize));
     if (psize && (!params || params_size < psize))

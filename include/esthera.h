@@ -164,6 +164,26 @@ int esth_model_build_frame(int model, const void* params, const void* control,
 #define ESTH_RNG_MTGP_COMPAT     0
 #define ESTH_RNG_MTGP_PER_FILTER 1
 
+/* Device layout of a filter's particle state (S floats per particle, N
+ * particles), the three the reference ships as model variants:
+ *   SOA    : plane c of particle i at c*N + i (pf_model_soa.cuh: one array
+ *            per state variable); the default.
+ *   AOS    : i*S + c (pf_model.cuh: an array of state structs).
+ *   PACKED : planes in groups of four, each group interleaved per particle
+ *            (pf_model_soa_packed.cuh: float4 angles1 / float angles2 /
+ *            float4 pos): group q = planes 4q..4q+w-1, w = min(4, S-4q),
+ *            at 4q*N + i*w + (c - 4q).  Groups start at plane 0, so the
+ *            robot arm's 9 planes pack as 4 + 4 + 1 (the reference's arm
+ *            packs 4 + 1 + 4: angles 0-3, angle 4, x y vx vy).
+ * The layout changes where the kernels read and write, never a result: a
+ * filter's particles, weights and estimates are bit-identical in all three.
+ * The host boundary (esth_filter_init, esth_filter_read_particles) stays
+ * SOA; esth_state_convert turns buffers the caller holds in one layout into
+ * another. */
+#define ESTH_LAYOUT_SOA    0
+#define ESTH_LAYOUT_AOS    1
+#define ESTH_LAYOUT_PACKED 2
+
 typedef struct {
     int      model;                /* ESTH_MODEL_* */
     uint32_t num_particles;        /* m: particles per sub-filter, power of two in [2, 1024] */
@@ -185,6 +205,8 @@ typedef struct {
                                       (the TODO of resampling-valias.cl:302-303).  0: the
                                       reference (weights restart every step; a skipped sub-filter
                                       keeps its unsorted pre-exchange state, pf_generic.cl:403-404) */
+    int      state_layout;         /* ESTH_LAYOUT_* of the device-resident state (single filters;
+                                      batches and servers keep SOA) */
 } esth_filter_desc;
 
 /* Fill a descriptor with the reference's defaults (pf_generic.c:1072-1084):
@@ -240,6 +262,9 @@ int esth_filter_set_timing(esth_filter* f, int on);
  * measured stream close to the untimed one. */
 int esth_filter_set_timing_every(esth_filter* f, int every);
 int esth_filter_stage_times(esth_filter* f, float us[6]);
+/* Host-side copy of `n` particles of `state_dim` floats from layout `from`
+ * to layout `to` (ESTH_LAYOUT_*); src and dst must not overlap. */
+int esth_state_convert(int state_dim, size_t n, int from, const float* src, int to, float* dst);
 /* Checkpoint / resume: particles, weights, RNG status and step counter. */
 int esth_filter_snapshot_size(esth_filter* f, size_t* bytes);
 int esth_filter_snapshot(esth_filter* f, void* buf, size_t bytes);
@@ -339,6 +364,12 @@ int esth_batch_info(esth_batch* b, uint64_t* particles, int* streams, uint32_t* 
 int esth_autotune(esth_engine* e, const esth_filter_desc* base, const void* params, size_t params_size,
                   const void* control, const void* measurement, float dt, float us_per_mparticle[5]);
 int esth_autotune_pick(const float us_per_mparticle[5], uint32_t particles, uint32_t min_m, uint32_t* m_out);
+/* Measures, on this device, the device time per step in microseconds of a
+ * filter shaped as `base` (its m, G, model and settings) with each state
+ * layout SOA, AOS, PACKED; *best receives the fastest layout. */
+int esth_autotune_layout(esth_engine* e, const esth_filter_desc* base, const void* params, size_t params_size,
+                         const void* control, const void* measurement, float dt, float us_per_step[3],
+                         int* best);
 
 /* ------------------------------------------------------------ serving
  * A request loop with continuous batching.  Tracks (filters) are opened,

@@ -44,6 +44,48 @@ def test_models_and_defaults():
     assert d.resampling_threshold == 1.0 and d.topology == ea.TOPO_RING
 
 
+def test_camera_model_registered():
+    L = ea.lib()
+    assert L.esth_model_lookup(b"robotarm5_camera") == ea.MODEL_ROBOTARM5_CAMERA
+    S, V, ps, cs, ms = C.c_int(), C.c_int(), C.c_size_t(), C.c_size_t(), C.c_size_t()
+    ea.check(L.esth_model_info(ea.MODEL_ROBOTARM5_CAMERA, C.byref(S), C.byref(V), C.byref(ps), C.byref(cs),
+                               C.byref(ms)))
+    assert (S.value, V.value, ps.value) == (9, 9, C.sizeof(ea.CameraParams))
+    assert (cs.value, ms.value) == (C.sizeof(ea.RobotArm5Control), C.sizeof(ea.RobotArm5Measurement))
+    p = ea.camera_params_default()   # pf_model_soa_packed_code.cuh:33-43
+    np.testing.assert_allclose(list(p.fc) + list(p.cc), [408.65831, 409.69059, 166.32611, 108.93539], rtol=1e-7)
+    np.testing.assert_allclose(list(p.kc), [-0.21561, 1.60704, -0.01291, 0.00007], rtol=1e-6)
+    ctrl, meas, truth = ea.robotarm5_camera_scenario(5, 3, 0.1)
+    fr = ea.build_frame(ea.MODEL_ROBOTARM5_CAMERA, p, ctrl[0], meas[0], 0.1)
+    assert np.array_equal(fr.view(np.uint32),
+                          O.build_frame(ea.MODEL_ROBOTARM5_CAMERA, p, ctrl[0], meas[0], 0.1).view(np.uint32))
+    assert fr[11] == np.float32(meas[0].x) and fr[13] == np.float32(p.fc[0]) and fr[20] == np.float32(p.kc[3])
+    with pytest.raises(ea.EsthError):   # the camera model needs its intrinsics
+        ea.build_frame(ea.MODEL_ROBOTARM5_CAMERA, None, ctrl[0], meas[0], 0.1)
+
+
+@pytest.mark.parametrize("S", [1, 2, 3, 5, 9, 12])
+def test_state_layouts(S):
+    """SOA planes, AOS structs and PACKED groups of four as the header defines
+    them, built here with numpy, and the converter between every pair."""
+    n = 37
+    soa = np.arange(S * n, dtype=np.float32).reshape(S, n) + 0.5
+    aos = soa.T.copy().reshape(-1)
+    groups = []
+    for q in range(0, S, 4):
+        groups.append(soa[q:q + 4].T.reshape(-1))   # group q: particle-major, its w planes interleaved
+    packed = np.concatenate(groups)
+    by = {ea.LAYOUT_SOA: soa.reshape(-1), ea.LAYOUT_AOS: aos, ea.LAYOUT_PACKED: packed}
+    for a, src in by.items():
+        for b, want in by.items():
+            got = ea.state_convert(src, S, n, a, b)
+            assert np.array_equal(got, want), (a, b)
+    if S == 9:   # groups of planes 0-3, 4-7, 8 (the reference's arm packs 0-3, 4, 5-8)
+        assert packed[4 * n] == soa[4, 0] and packed[4 * n + 3] == soa[7, 0] and packed[8 * n] == soa[8, 0]
+    with pytest.raises(ea.EsthError):
+        ea.state_convert(soa, S, n, ea.LAYOUT_SOA, 7)
+
+
 def test_frames_match_oracle():
     p, ctrl, meas, truth = ea.unicycle_scenario(42, 32, 50.0, 2)
     assert np.array_equal(ea.build_frame(1, p, ctrl[1], meas[1], 0.1).view(np.uint32),
