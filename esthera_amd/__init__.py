@@ -172,6 +172,8 @@ _SIGS = {
     "esth_trace_load_robotarm": ([C.c_char_p, C.c_int, _P, _P, _P, _P, _P], C.c_int),
     "esth_robotarm5_estimate_error": ([_P, _P], C.c_float),
     "esth_trace_load_unicycle10": ([C.c_char_p, C.c_int, _P, _P, _P, _P, _P], C.c_int),
+    "esth_trace_write_robotarm": ([C.c_char_p, C.c_int, _P, _P, _P, _P], C.c_int),
+    "esth_trace_write_unicycle10": ([C.c_char_p, C.c_int, _P, _P, _P, _P], C.c_int),
     "esth_unicycle10_filter": ([C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p, C.c_uint32, C.c_int,
                                 _P, _P], C.c_int),
     "esth_unicycle10_params_default": ([_P], None),
@@ -678,6 +680,28 @@ def load_robotarm_trace(path, max_rows=100000):
     check(lib().esth_trace_load_robotarm(path.encode(), max_rows, meas, ctrl, ptr(dt), ptr(truth), C.byref(rows)))
     n = rows.value
     return meas[:n], ctrl[:n], dt[:n], truth[: n * 9].reshape(n, 9)
+
+
+def write_robotarm_trace(path, meas, ctrl, dt, truth):
+    """Rows in the robot-arm trace format (truth: rows x 9, vx and vy not written)."""
+    n = len(meas)
+    m = (RobotArm5Measurement * n)(*meas)
+    c = (RobotArm5Control * n)(*ctrl)
+    d = np.ascontiguousarray(dt, dtype=np.float32)
+    t = np.ascontiguousarray(truth, dtype=np.float32).reshape(n, 9)
+    check(lib().esth_trace_write_robotarm(path.encode(), n, m, c, ptr(d), ptr(t)))
+
+
+def write_unicycle10_trace(path, meas, ctrl, dt, truth):
+    n = len(meas)
+    m = (Unicycle10Measurement * n)(*meas)
+    c = (UnicycleControl * n)(*ctrl)
+    d = np.ascontiguousarray(dt, dtype=np.float32)
+    t = np.ascontiguousarray(truth, dtype=np.float32).reshape(n, 3)
+    check(lib().esth_trace_write_unicycle10(path.encode(), n, m, c, ptr(d), ptr(t)))
+
+
+CLI_PATH = os.path.join(HERE, "esthera_pf")
 
 
 def load_unicycle10_trace(path, max_rows=100000):

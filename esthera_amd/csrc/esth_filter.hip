@@ -565,6 +565,28 @@ extern "C" int esth_trace_load_robotarm(const char* path, int max_rows, esth_rob
     return ESTH_OK;
 }
 
+static void write_floats(FILE* fh, const float* v, int n) {
+    for (int k = 0; k < n; ++k) fprintf(fh, k ? " %.9g" : "%.9g", (double)v[k]);
+    fputc('\n', fh);
+}
+
+extern "C" int esth_trace_write_robotarm(const char* path, int rows, const esth_robotarm5_measurement* meas,
+                                         const esth_robotarm5_control* ctrl, const float* dt, const float* truth) {
+    if (!path || rows < 0 || (rows > 0 && (!meas || !ctrl || !dt || !truth))) return esth_fail(ESTH_ERR_INVALID, "null");
+    FILE* fh = fopen(path, "w");
+    if (!fh) return esth_fail(ESTH_ERR_IO, "could not open %s", path);
+    for (int n = 0; n < rows; ++n) {
+        const float* tr = truth + (size_t)n * 9;
+        float v[20];
+        v[0] = meas[n].x; v[1] = meas[n].y;
+        for (int i = 0; i < 5; ++i) { v[2 + i] = meas[n].angles[i]; v[7 + i] = ctrl[n].angles[i]; v[15 + i] = tr[i]; }
+        v[12] = dt[n]; v[13] = tr[5]; v[14] = tr[6];
+        write_floats(fh, v, 20);
+    }
+    if (fclose(fh)) return esth_fail(ESTH_ERR_IO, "could not write %s", path);
+    return ESTH_OK;
+}
+
 /* estimate_error, pf_model_host.h:56-69 */
 extern "C" float esth_robotarm5_estimate_error(const float* est, const float* act) {
     float error = 0.0f;
@@ -618,6 +640,22 @@ extern "C" int esth_particle_filter(uint32_t m, uint32_t G, uint32_t t, float th
     if (rc != ESTH_OK) return rc;
     if (mean_error) *mean_error = rows ? err_sum / rows : 0.0;
     if (steps) *steps = rows;
+    return ESTH_OK;
+}
+
+extern "C" int esth_trace_write_unicycle10(const char* path, int rows, const esth_unicycle10_measurement* meas,
+                                           const esth_unicycle_control* ctrl, const float* dt, const float* truth) {
+    if (!path || rows < 0 || (rows > 0 && (!meas || !ctrl || !dt || !truth))) return esth_fail(ESTH_ERR_INVALID, "null");
+    FILE* fh = fopen(path, "w");
+    if (!fh) return esth_fail(ESTH_ERR_IO, "could not open %s", path);
+    for (int n = 0; n < rows; ++n) {
+        float v[16];
+        for (int i = 0; i < 10; ++i) v[i] = meas[n].range[i];
+        v[10] = ctrl[n].v; v[11] = ctrl[n].omega; v[12] = dt[n];
+        for (int i = 0; i < 3; ++i) v[13 + i] = truth[(size_t)n * 3 + i];
+        write_floats(fh, v, 16);
+    }
+    if (fclose(fh)) return esth_fail(ESTH_ERR_IO, "could not write %s", path);
     return ESTH_OK;
 }
 

@@ -291,6 +291,12 @@ int esth_particle_filter(uint32_t m, uint32_t G, uint32_t t, float thr, const ch
  * truth receives 9 floats per row (a0..a4, x, y, vx=0, vy=0). */
 int esth_trace_load_robotarm(const char* path, int max_rows, esth_robotarm5_measurement* meas,
                              esth_robotarm5_control* ctrl, float* dt, float* truth, int* rows);
+/* Write `rows` rows in the robot-arm format that esth_trace_load_robotarm
+ * reads (20 columns; truth 9 floats per row, of which vx, vy are not
+ * written).  Floats are printed with 9 significant digits, so a trace
+ * written here reads back bit-identically. */
+int esth_trace_write_robotarm(const char* path, int rows, const esth_robotarm5_measurement* meas,
+                              const esth_robotarm5_control* ctrl, const float* dt, const float* truth);
 /* estimate_error (pf_model_host.h:56-69) on robotarm5 states (9 floats). */
 float esth_robotarm5_estimate_error(const float* estimate, const float* actual);
 
@@ -299,6 +305,10 @@ float esth_robotarm5_estimate_error(const float* estimate, const float* actual);
  * dt, truth x, y, theta.  truth receives 3 floats per row. */
 int esth_trace_load_unicycle10(const char* path, int max_rows, esth_unicycle10_measurement* meas,
                                esth_unicycle_control* ctrl, float* dt, float* truth, int* rows);
+/* Write `rows` rows in the reference unicycle format (16 columns, 9
+ * significant digits; truth 3 floats per row). */
+int esth_trace_write_unicycle10(const char* path, int rows, const esth_unicycle10_measurement* meas,
+                                const esth_unicycle_control* ctrl, const float* dt, const float* truth);
 /* The reference unicycle program's loop (particle_filter.cu:729-1157) on the
  * engine: particles start at (20, 40, -pi/2) (:655-663); writes the mean over
  * steps of the distance between the best particle and the truth (:1044-1057). */

@@ -193,3 +193,62 @@ def test_autotune_pick_rules():
 
 def test_batch_autotune_decl_parses():
     assert "esth_batch_create" in header_functions() and "esth_autotune" in header_functions()
+
+
+def _struct_bits(rows):
+    return b"".join(bytes(r) for r in rows)
+
+
+def test_trace_writers_roundtrip(tmp_path):
+    """Traces written in the reference formats read back bit-identically: the
+    reference's own robot-arm and unicycle inputs, and a seeded camera trace."""
+    arm = os.path.join(REPO, "tests", "golden", "robotarm_input5.txt")
+    meas, ctrl, dt, truth = ea.load_robotarm_trace(arm)
+    out = str(tmp_path / "arm.txt")
+    ea.write_robotarm_trace(out, meas, ctrl, dt, truth)
+    m2, c2, d2, t2 = ea.load_robotarm_trace(out)
+    assert _struct_bits(m2) == _struct_bits(meas) and _struct_bits(c2) == _struct_bits(ctrl)
+    assert np.array_equal(d2.view(np.uint32), dt.view(np.uint32))
+    assert np.array_equal(t2.view(np.uint32), truth.view(np.uint32))
+    # the reference's file itself: same numbers, row for row
+    ref = np.loadtxt(arm, dtype=np.float64).astype(np.float32)
+    mine = np.loadtxt(out, dtype=np.float64).astype(np.float32)
+    assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32))
+    uni = os.path.join(REPO, "tests", "golden", "unicycle_input.txt")
+    um, uc, ud, ut = ea.load_unicycle10_trace(uni)
+    out = str(tmp_path / "uni.txt")
+    ea.write_unicycle10_trace(out, um, uc, ud, ut)
+    um2, uc2, ud2, ut2 = ea.load_unicycle10_trace(out)
+    assert _struct_bits(um2) == _struct_bits(um) and _struct_bits(uc2) == _struct_bits(uc)
+    assert np.array_equal(ut2.view(np.uint32), ut.view(np.uint32))
+    cc, cm, ct = ea.robotarm5_camera_scenario(8, 25, 0.1)
+    out = str(tmp_path / "cam.txt")
+    tr = ct[1:].copy()
+    ea.write_robotarm_trace(out, cm, cc, np.full(25, 0.1, np.float32), tr)
+    m3, c3, d3, t3 = ea.load_robotarm_trace(out)
+    assert _struct_bits(m3) == _struct_bits(cm) and len(m3) == 25
+    assert np.array_equal(t3[:, :7], tr[:, :7]) and not t3[:, 7:].any()
+
+
+def _cli(*args):
+    import subprocess
+    return subprocess.run([ea.CLI_PATH, *args], capture_output=True, text=True, timeout=60)
+
+
+def test_cli_arguments_without_gpu(tmp_path):
+    """The command line's argument handling (pf_generic.c:1103-1138) up to the
+    point where it would open the device."""
+    r = _cli("-h")
+    assert r.returncode != 0 and "Usage:" in r.stderr and "input_file" in r.stderr
+    r = _cli("-m", "64")
+    assert r.returncode != 0 and "no input file given" in r.stderr
+    r = _cli("-c", "x")
+    assert r.returncode != 0 and "no CPU device" in r.stderr
+    r = _cli("-d", "1:0", "x")
+    assert r.returncode != 0 and "-d" in r.stderr
+    r = _cli("-M", "dh6", "x")
+    assert r.returncode != 0 and "trace-driven" in r.stderr
+    r = _cli("-m", "0", "x")
+    assert r.returncode != 0 and "Usage:" in r.stderr
+    r = _cli(str(tmp_path / "missing.txt"))
+    assert r.returncode != 0 and "could not" in r.stderr
