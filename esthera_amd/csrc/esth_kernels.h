@@ -642,7 +642,7 @@ __device__ __forceinline__ void esth_store_plane_run(float* __restrict__ st, int
     for (int r = 0; r < E; ++r) st[b + (size_t)r * stride] = v[r];
 }
 
-template <int LOGM, int S>
+template <int LOGM, int S, bool HOIST = false>
 __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float* __restrict__ st_in,
                                               float* __restrict__ st_out, uint32_t N, uint32_t G, uint32_t g,
                                               bool valid, float2* __restrict__ winner, int* __restrict__ perm,
@@ -658,6 +658,18 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
     float key[E];
     int idx[E];
     esth_load_run<E>(w + base + le, key);
+    /* the particles' planes do not depend on the order: with HOIST, when
+     * they fit in registers, their loads are issued here and land while the
+     * network runs (a gain for many small sub-filters; on a few large ones
+     * the early burst delays the other waves' keys, so there they follow the
+     * network) */
+    constexpr int PB = (S * E <= 64) ? S : 1;
+    constexpr bool EARLY = HOIST && PB == S;
+    float in[PB][E];
+    if constexpr (EARLY) {
+#pragma unroll
+        for (int j = 0; j < S; ++j) esth_load_plane_run<E, S>(st_in, layout, N, j, base + le, in[j]);
+    }
 #pragma unroll
     for (int r = 0; r < E; ++r) idx[r] = le + r;
     esth_bitonic<LOGM>(key, idx, le);
@@ -681,13 +693,11 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
         }
     }
     /* gather (particle_set of block_sort, pf_generic.cl:281-283) and partial
-     * sums; each lane's E consecutive outputs leave as vector stores; the
-     * planes' loads are issued together, PB planes at a time */
-    constexpr int PB = (S * E <= 64) ? S : 1;
-    float in[PB][E];
+     * sums; each lane's E consecutive outputs leave as vector stores; unless
+     * loaded early, the planes' loads are issued together, PB at a time */
 #pragma unroll
     for (int c = 0; c < S; ++c) {
-        if (c % PB == 0) {
+        if (!EARLY && c % PB == 0) {
 #pragma unroll
             for (int j = 0; j < PB; ++j) esth_load_plane_run<E, S>(st_in, layout, N, c + j, base + le, in[j]);
         }
@@ -859,7 +869,7 @@ struct esth_xch {
 /* one sub-filter per wave (or 64/m per wave); every wave of the work-group
  * must call it (one barrier inside); cdf_w: this wave's 64*E floats of LDS;
  * recv: apply the exchange xch on the way in */
-template <int LOGM, int S>
+template <int LOGM, int S, bool HOIST = false>
 __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                                                   const float* __restrict__ st_tmp,
                                                   const float* w,   /* xch.w_out may write it */
@@ -893,6 +903,21 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
     }
     float x[E];
     esth_load_run<E>(w + base + le, x);
+    /* the selection's uniforms and, with HOIST when they fit in registers,
+     * the sorted particles' planes do not depend on the scan: their loads
+     * are issued here and land while it runs */
+    constexpr int PB = (S * E <= 64) ? S : 1;
+    constexpr bool EARLY = HOIST && PB == S;
+    float inh[PB][E];
+    if constexpr (EARLY) {
+#pragma unroll
+        for (int c = 0; c < S; ++c) esth_load_run<E>(st_tmp + (size_t)c * N + base + le, inh[c]);
+    }
+    float uu[E];
+    if (!systematic) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) uu[r] = rnd[base + le + r];
+    }
     if (recv) {
 #pragma unroll
         for (int r = 0; r < E; ++r)
@@ -900,6 +925,14 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                 x[r] = xch.bt_w[rsrc[r]];
                 if (valid) xch.w_out[base + le + r] = x[r];
             }
+        if constexpr (EARLY) {
+            const size_t Gt = (size_t)xch.G * xch.t;
+#pragma unroll
+            for (int c = 0; c < S; ++c)
+#pragma unroll
+                for (int r = 0; r < E; ++r)
+                    if (rsrc[r] >= 0) inh[c][r] = xch.bt_s[(size_t)c * Gt + (size_t)rsrc[r]];
+        }
     }
     if (log_w) {
         float mx = x[0];
@@ -942,7 +975,7 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
     for (int r = 0; r < E; ++r) {
         float s;
         if (systematic) s = ((float)(le + r) + u0) * step;
-        else s = (rnd[base + le + r] - 1.0f) * total;
+        else s = (uu[r] - 1.0f) * total;
         /* binary_search, pf_generic.cl:182-201: on a power-of-two range each
          * round halves it exactly, so its do-while runs LOGM rounds; unrolled,
          * the E searches of a lane interleave */
@@ -957,15 +990,21 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
         pick[r] = mn & (M - 1);
     }
     /* gather the selected particles through the cdf's LDS region, plane by
-     * plane (every lane takes part; only resampled sub-filters store); the
-     * planes' loads are issued together, PB planes at a time */
-    constexpr int PB = (S * E <= 64) ? S : 1;
+     * plane (every lane takes part; only resampled sub-filters store); unless
+     * loaded early, the planes' loads are issued together, PB at a time */
 #pragma unroll
     for (int c0 = 0; c0 < S; c0 += PB) {
         float in[PB][E];
+        if constexpr (EARLY) {
 #pragma unroll
-        for (int c = 0; c < PB; ++c) esth_load_run<E>(st_tmp + (size_t)(c0 + c) * N + base + le, in[c]);
-        if (recv) {
+            for (int c = 0; c < PB; ++c)
+#pragma unroll
+                for (int r = 0; r < E; ++r) in[c][r] = inh[c][r];
+        } else {
+#pragma unroll
+            for (int c = 0; c < PB; ++c) esth_load_run<E>(st_tmp + (size_t)(c0 + c) * N + base + le, in[c]);
+        }
+        if (!EARLY && recv) {
             const size_t Gt = (size_t)xch.G * xch.t;
 #pragma unroll
             for (int c = 0; c < PB; ++c)
@@ -1300,7 +1339,7 @@ __global__ __launch_bounds__(256) void kb_sort(const esth_bfilter* __restrict__ 
     const uint2 e = sub[wv];
     const esth_bfilter& f = F[e.x];
     __shared__ float stage[4][ESTH_WAVE * esth_tile<LOGM>::E];
-    esth_sort_sub<LOGM, S>(f.w, f.state, f.tmp, f.N, f.G, e.y, true, (float2*)f.winner, nullptr, f.t, f.bt_w,
+    esth_sort_sub<LOGM, S, true>(f.w, f.state, f.tmp, f.N, f.G, e.y, true, (float2*)f.winner, nullptr, f.t, f.bt_w,
                            f.bt_s, f.part, log_w, stage[threadIdx.x / ESTH_WAVE]);
 }
 
@@ -1392,7 +1431,7 @@ __global__ __launch_bounds__(256) void kb_resample(const esth_bfilter* __restric
     const uint2 e = sub[valid ? wv : nsub - 1];
     const esth_bfilter& f = F[e.x];
     const esth_xch x{f.bt_w, f.bt_s, f.top, f.w, f.G, f.t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, f.dimx, f.dimy};
-    esth_resample_sub<LOGM, S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, valid, thr, systematic, log_w,
+    esth_resample_sub<LOGM, S, true>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, valid, thr, systematic, log_w,
                                nullptr, cdf_s[threadIdx.x / ESTH_WAVE], topo != ESTH_TOPO_NONE, x,
                                (int)(f.flags & 1u), f.w);
 }
