@@ -394,20 +394,52 @@ int esth_launch_exchange(float* st, float* w, int S, uint32_t N, uint32_t m, uin
     return ESTH_OK;
 }
 
-int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w, const float* rnd,
-                         uint32_t N, uint32_t m, uint32_t G, float thr, int adaptive, int log_w, int* sel, int carry,
-                         int layout, hipStream_t s) {
-    if (resampler == ESTH_RESAMPLE_ALIAS) {
+template <int LOGM, int S>
+static void launch_alias_w_t(float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
+                             uint32_t G, float thr, int adaptive, int log_w, int* sel, float* prob_out,
+                             uint32_t* alias_out, int carry, int layout, hipStream_t s) {
+    if constexpr (LOGM >= 6) {
+        constexpr int WPB = esth_alias_wpb<LOGM>::value;
+        hipLaunchKernelGGL((k_alias_w<LOGM, S>), dim3((G + WPB - 1) / WPB), dim3(WPB * ESTH_WAVE), 0, s, st_main, st_tmp,
+                           w, rnd, N, G, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout);
+    }
+}
+
+/* Vose alias: one wave per sub-filter for m >= 64, one work-group of m below */
+static int launch_alias(int S, float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
+                        uint32_t m, uint32_t G, float thr, int adaptive, int log_w, int* sel, float* prob_out,
+                        uint32_t* alias_out, int carry, int layout, hipStream_t s) {
+    if (m >= ESTH_WAVE) {
+        const int lm = log2u(m);
+#define ESTH_AW_ARGS st_main, st_tmp, w, rnd, N, G, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout, s
         switch (S) {
-        case 1: hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry, layout); break;
-        case ESTH_UNI_S: hipLaunchKernelGGL(k_alias<ESTH_UNI_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry, layout); break;
-        case ESTH_RA5_S: hipLaunchKernelGGL(k_alias<ESTH_RA5_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry, layout); break;
-        case ESTH_DH6_S: hipLaunchKernelGGL(k_alias<ESTH_DH6_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, (float*)nullptr, (uint32_t*)nullptr, carry, layout); break;
+        case 1: ESTH_DISPATCH_LOGM(launch_alias_w_t, 1, ESTH_AW_ARGS); break;
+        case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_alias_w_t, ESTH_UNI_S, ESTH_AW_ARGS); break;
+        case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_alias_w_t, ESTH_RA5_S, ESTH_AW_ARGS); break;
+        case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_alias_w_t, ESTH_DH6_S, ESTH_AW_ARGS); break;
         default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
         }
+#undef ESTH_AW_ARGS
         ESTH_HIP(hipGetLastError());
         return ESTH_OK;
     }
+    switch (S) {
+    case 1: hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout); break;
+    case ESTH_UNI_S: hipLaunchKernelGGL(k_alias<ESTH_UNI_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout); break;
+    case ESTH_RA5_S: hipLaunchKernelGGL(k_alias<ESTH_RA5_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout); break;
+    case ESTH_DH6_S: hipLaunchKernelGGL(k_alias<ESTH_DH6_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout); break;
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+    }
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
+int esth_launch_resample(int resampler, int S, float* st_main, const float* st_tmp, const float* w, const float* rnd,
+                         uint32_t N, uint32_t m, uint32_t G, float thr, int adaptive, int log_w, int* sel, int carry,
+                         int layout, hipStream_t s) {
+    if (resampler == ESTH_RESAMPLE_ALIAS)
+        return launch_alias(S, st_main, st_tmp, w, rnd, N, m, G, thr, adaptive, log_w, sel, nullptr, nullptr, carry,
+                            layout, s);
     const int lm = log2u(m);
     const int sys = resampler == ESTH_RESAMPLE_SYSTEMATIC;
     switch (S) {
@@ -597,10 +629,8 @@ extern "C" int esth_stage_resample(int resampler, float* dst, const float* src, 
 extern "C" int esth_stage_alias_table(const float* d_w, uint32_t m, uint32_t G, int weights, float* d_prob, uint32_t* d_alias) {
     if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES) return esth_fail(ESTH_ERR_INVALID, "m=%u", m);
     if (!d_prob || !d_alias) return esth_fail(ESTH_ERR_INVALID, "null output");
-    hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, 0, (float*)nullptr, (const float*)nullptr, d_w,
-                       (const float*)nullptr, m * G, m, 1.0f, 0, weights == ESTH_WEIGHTS_LOG, (int*)nullptr, d_prob, d_alias, 0,
-                       ESTH_LAYOUT_SOA);
-    ESTH_HIP(hipGetLastError());
+    ESTH_TRY(launch_alias(1, nullptr, nullptr, d_w, nullptr, m * G, m, G, 1.0f, 0, weights == ESTH_WEIGHTS_LOG, nullptr,
+                          d_prob, d_alias, 0, ESTH_LAYOUT_SOA, 0));
     ESTH_HIP(hipDeviceSynchronize());
     return ESTH_OK;
 }

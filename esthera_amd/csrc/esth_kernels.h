@@ -1303,6 +1303,253 @@ __global__ __launch_bounds__(1024) void k_alias(float* __restrict__ st_main,
 }
 
 
+/* ==================================================== Vose alias, one wave
+ * The same resampling for M >= 64 with one wave per sub-filter (E = M/64
+ * items per lane, item i = lane E + r): the work-group algorithm above with
+ * wave-level ordering in place of its barriers.  The sums follow
+ * sum_reduce4's tree, the worklists the same canonical order (ascending item
+ * index) and the pairing rounds the same reads-then-writes steps, so the
+ * tables are identical; but no waves idle at barriers while the pairing
+ * tail runs on one of them, and a sub-filter costs one wave, not M/64. */
+__device__ __forceinline__ int esth_wave_excl_scan_i(int x, int lane, int* total) {
+    int v = x;
+#pragma unroll
+    for (int d = 1; d < ESTH_WAVE; d <<= 1) {
+        const int o = __builtin_amdgcn_ds_bpermute((lane - d) << 2, v);
+        v += lane >= d ? o : 0;
+    }
+    *total = __builtin_amdgcn_readlane(v, ESTH_WAVE - 1);
+    return v - x;
+}
+
+/* sum_reduce4 (resampling-valias.cl:85-104) over arr[0..M) by one wave */
+template <int M>
+__device__ __forceinline__ float esth_wave_sum_reduce4(float* arr, int lane) {
+    int offset = M;
+#pragma unroll
+    for (int it = 0; it < 6 && offset > 2; ++it) {
+        offset >>= 2;
+        for (int l = lane; l < offset; l += ESTH_WAVE) {
+            const float rhs = (arr[l + offset] + arr[l + offset + offset]) + arr[l + offset + offset + offset];
+            arr[l] = arr[l] + rhs;
+        }
+        esth_wave_lds_fence();
+    }
+    if (offset > 1) {
+        if (lane == 0) arr[0] = arr[0] + arr[1];
+        esth_wave_lds_fence();
+    }
+    return arr[0];
+}
+
+template <int LOGM, int S>
+__device__ __forceinline__ void esth_alias_wave(float* __restrict__ st_main, const float* __restrict__ st_tmp,
+                                                const float* w, const float* __restrict__ rnd, uint32_t N,
+                                                uint32_t g, float thr, int adaptive, int log_w,
+                                                int* __restrict__ sel, float* __restrict__ prob_out,
+                                                uint32_t* __restrict__ alias_out, int carry, int layout,
+                                                float* lds) {
+    static_assert(LOGM >= 6, "one wave per sub-filter needs M >= 64");
+    constexpr int M = 1 << LOGM, E = M / ESTH_WAVE;
+    constexpr int J = (M / 2 + ESTH_WAVE - 1) / ESTH_WAVE;   /* a round pairs at most M/2 */
+    const int lane = threadIdx.x & (ESTH_WAVE - 1);
+    const int le = lane * E;
+    const size_t base = (size_t)g * M;
+    float* prob = lds;
+    uint32_t* alias = reinterpret_cast<uint32_t*>(lds + M);
+    float* scratch = lds + 2 * M;                           /* sums, then the worklists */
+    uint32_t* sl = reinterpret_cast<uint32_t*>(lds + 2 * M);
+    const bool table_only = prob_out != nullptr;
+    bool skip = !table_only && !adaptive && (rnd[N + g] - 1.0f) > thr;
+    float p[E];
+    if (!skip) {
+        esth_load_run<E>(w + base + le, p);
+        if (log_w) {
+            float mx = p[0];
+#pragma unroll
+            for (int r = 1; r < E; ++r) mx = p[r] > mx ? p[r] : mx;
+            mx = esth_bfly_max<1, ESTH_WAVE>(mx);
+#pragma unroll
+            for (int r = 0; r < E; ++r) p[r] = esth_expf(p[r] - esth_max_shift(mx));
+        }
+#pragma unroll
+        for (int r = 0; r < E; ++r) scratch[le + r] = p[r];
+        esth_wave_lds_fence();
+        const float sum = esth_wave_sum_reduce4<M>(scratch, lane);
+#pragma unroll
+        for (int r = 0; r < E; ++r) p[r] = sum > 0.0f ? p[r] / sum : 1.0f / (float)M;
+        esth_wave_lds_fence();   /* every lane has read the sum */
+        if (adaptive && !table_only) {
+#pragma unroll
+            for (int r = 0; r < E; ++r) scratch[le + r] = p[r] * p[r];
+            esth_wave_lds_fence();
+            const float recip_ess = esth_wave_sum_reduce4<M>(scratch, lane);
+            skip = !(recip_ess > thr);
+            esth_wave_lds_fence();
+        }
+    }
+    if (skip) {
+        if (sel) {
+#pragma unroll
+            for (int r = 0; r < E; ++r) sel[base + le + r] = -1;
+        }
+        if (carry) {
+#pragma unroll
+            for (int c = 0; c < S; ++c) {
+                float v[E];
+                esth_load_run<E>(st_tmp + (size_t)c * N + base + le, v);
+                esth_store_plane_run<E, S>(st_main, layout, N, c, base + le, v);
+            }
+        }
+        return;
+    }
+    /* vose_alias_init, resampling-valias.cl:111-195 */
+    bool small[E];
+    int cs = 0;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const float pl = p[r] * (float)M;
+        prob[le + r] = pl;
+        alias[le + r] = (uint32_t)(le + r);
+        small[r] = pl <= 1.0f;
+        cs += small[r] ? 1 : 0;
+    }
+    int nsmall_i;
+    int rk = esth_wave_excl_scan_i(cs, lane, &nsmall_i);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const uint32_t item = (uint32_t)(le + r);
+        const uint32_t slot = small[r] ? (uint32_t)rk : ((uint32_t)M - 1u - (item - (uint32_t)rk));
+        sl[slot] = item;
+        rk += small[r] ? 1 : 0;
+    }
+    esth_wave_lds_fence();
+    uint32_t small_begin = 0, small_end = (uint32_t)nsmall_i;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (ESTH_WAVE - lane));
+    for (;;) {
+        const uint32_t a = small_end - small_begin, b = (uint32_t)M - small_end;
+        const uint32_t n = a < b ? a : b;
+        if (n == 0) break;
+        if (n <= (uint32_t)ESTH_WAVE) {   /* the common short round: one pair per lane */
+            const uint32_t k = (uint32_t)lane;
+            const bool act = k < n;
+            uint32_t li = 0;
+            float pL = 0.0f;
+            if (act) {
+                const uint32_t si = sl[small_begin + k];
+                li = sl[small_end + k];
+                const float pS = prob[si];
+                alias[si] = li;
+                pL = (prob[li] + pS) - 1.0f;
+            }
+            esth_wave_lds_fence();
+            const bool bs = act && pL <= 1.0f;
+            const uint64_t bal = __ballot(bs);
+            const uint32_t rs = (uint32_t)__popcll(bal & below);
+            if (act) {
+                const uint32_t s2 = bs ? (small_end + rs) : (small_end - 1u + n - (k - rs));
+                sl[s2] = li;
+                prob[li] = pL;
+            }
+            esth_wave_lds_fence();
+            small_end += (uint32_t)__popcll(bal);
+            small_begin += n;
+            continue;
+        }
+        uint32_t li[J];
+        float pL[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const uint32_t k = (uint32_t)(j * ESTH_WAVE + lane);
+            li[j] = 0;
+            pL[j] = 0.0f;
+            if (k < n) {
+                const uint32_t si = sl[small_begin + k];
+                li[j] = sl[small_end + k];
+                const float pS = prob[si];
+                alias[si] = li[j];
+                pL[j] = (prob[li[j]] + pS) - 1.0f;
+            }
+        }
+        esth_wave_lds_fence();   /* the round's reads before its writes */
+        uint32_t nbs = 0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if ((uint32_t)(j * ESTH_WAVE) >= n) break;
+            const uint32_t k = (uint32_t)(j * ESTH_WAVE + lane);
+            const bool act = k < n;
+            const bool bs = act && pL[j] <= 1.0f;
+            const uint64_t bal = __ballot(bs);
+            const uint32_t rs = nbs + (uint32_t)__popcll(bal & below);
+            if (act) {
+                const uint32_t s2 = bs ? (small_end + rs) : (small_end - 1u + n - (k - rs));
+                sl[s2] = li[j];
+                prob[li[j]] = pL[j];
+            }
+            nbs += (uint32_t)__popcll(bal);
+        }
+        esth_wave_lds_fence();
+        small_end += nbs;
+        small_begin += n;
+    }
+    if (table_only) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            prob_out[base + le + r] = prob[le + r];
+            alias_out[base + le + r] = alias[le + r];
+        }
+        return;
+    }
+    /* vose_alias_generate_next (:198-202) on d_random2[idx] - 1 (:288-295) */
+    float u[2 * E];
+    esth_load_run<2 * E>(rnd + 2 * (base + le), u);
+    int pick[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const float r0 = u[2 * r] - 1.0f, r1 = u[2 * r + 1] - 1.0f;
+        const uint32_t col = (uint32_t)floorf(r0 * (float)M);
+        pick[r] = (int)((r1 < prob[col]) ? col : alias[col]);
+    }
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+        float v[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = st_tmp[(size_t)c * N + base + (size_t)pick[r]];
+        esth_store_plane_run<E, S>(st_main, layout, N, c, base + le, v);
+    }
+    if (sel) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) sel[base + le + r] = pick[r];
+    }
+    if (carry) {   /* every lane has read its weights (before the sums) */
+        float w0[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) w0[r] = log_w ? 0.0f : 1.0f;
+        esth_store_run<E>(const_cast<float*>(w) + base + le, w0);
+    }
+}
+
+/* waves per work-group of the one-wave alias kernels (LDS: 12 M bytes a wave) */
+template <int LOGM>
+struct esth_alias_wpb {
+    static constexpr int value = LOGM >= 10 ? 2 : 4;
+};
+
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void k_alias_w(float* __restrict__ st_main, const float* __restrict__ st_tmp,
+                                                 const float* w, const float* __restrict__ rnd, uint32_t N,
+                                                 uint32_t G, float thr, int adaptive, int log_w,
+                                                 int* __restrict__ sel, float* __restrict__ prob_out,
+                                                 uint32_t* __restrict__ alias_out, int carry, int layout) {
+    constexpr int WPB = esth_alias_wpb<LOGM>::value;
+    __shared__ float lds[WPB][3 << LOGM];
+    const uint32_t wv = threadIdx.x / ESTH_WAVE;
+    const uint32_t g = blockIdx.x * WPB + wv;
+    if (g >= G) return;   /* a whole wave: no work-group barrier follows */
+    esth_alias_wave<LOGM, S>(st_main, st_tmp, w, rnd, N, g, thr, adaptive, log_w, sel, prob_out, alias_out, carry,
+                             layout, lds[wv]);
+}
+
 /* ============================================================== batches
  * Kernels over many filters: a device table of esth_bfilter and tables of
  * (filter, sub-filter) pairs, one per sub-filter size class.  Each wave (or

@@ -336,11 +336,13 @@ def test_alias_table_bit_exact_and_invariant(m, G, engine):
         np.testing.assert_allclose(mass, m * p, atol=2e-5 * m)
 
 
-@pytest.mark.parametrize("shape", ["peaked", "two-heavy", "flat"])
-@pytest.mark.parametrize("m", [128, 1024])
+@pytest.mark.parametrize("shape", ["peaked", "two-heavy", "flat", "dead"])
+@pytest.mark.parametrize("m", [32, 64, 128, 1024])
 def test_alias_table_pairing_regimes(shape, m, engine):
     """Weights whose pairing runs many short rounds (one or two heavy
-    particles), and equal weights (no round at all), in log mode."""
+    particles), equal weights (no round at all) and all weights 0 (log -inf:
+    the uniform table), in log mode; m = 32 runs the work-group kernel, the
+    others one wave per sub-filter."""
     rng = np.random.default_rng(m + len(shape))
     G = 6
     if shape == "peaked":
@@ -349,8 +351,10 @@ def test_alias_table_pairing_regimes(shape, m, engine):
         lw = rng.normal(size=m * G).astype(np.float32)
         lw[::m] += 9.0
         lw[m // 2::m] += 8.0
-    else:
+    elif shape == "flat":
         lw = np.full(m * G, -3.25, np.float32)
+    else:
+        lw = np.full(m * G, -np.inf, np.float32)
     dw, dp, da = DB.from_array(lw), DB(m * G * 4), DB(m * G * 4)
     ea.check(ea.lib().esth_stage_alias_table(dw.ptr, m, G, ea.WEIGHTS_LOG, dp.ptr, da.ptr))
     prob, alias = dp.to_array(np.float32, m * G), da.to_array(np.uint32, m * G)
