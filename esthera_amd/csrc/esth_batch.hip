@@ -367,6 +367,12 @@ template <int LOGM, int S>
 static void launch_kb_sort(esth_batch* b, const esth_bclass& c, int log_w) {
     hipLaunchKernelGGL((kb_sort<LOGM, S>), dim3((c.n + 3) / 4), dim3(256), 0, b->stream, b->d_F, c.d_sub, c.n, log_w);
 }
+template <int LOGM, int S>
+static void launch_kb_alias(esth_batch* b, const esth_bclass& c, const float* R, int log_w) {
+    constexpr int WPB = esth_alias_wpb<LOGM>::value;
+    hipLaunchKernelGGL((kb_alias_w<LOGM, S>), dim3((c.n + WPB - 1) / WPB), dim3(WPB * ESTH_WAVE), 0, b->stream, b->d_F,
+                       c.d_sub, c.n, R, b->d.resampling_threshold, b->d.adaptive, log_w);
+}
 /* argmax_blocks: the first launch after the sorts also runs every filter's
  * argmax (one work-group each, ahead of the resampling work-groups) */
 template <int LOGM, int S>
@@ -422,9 +428,7 @@ static int batch_kernels_s(esth_batch* b, int p) {
             hipLaunchKernelGGL(kb_exchange, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, b->d_F, b->d_sub_all,
                                nsub, S, nslots, b->d.topology);
         }
-        for (const esth_bclass& c : b->cls)
-            hipLaunchKernelGGL(kb_alias<S>, dim3(c.n), dim3(1u << c.logm), 0, s, b->d_F, c.d_sub, R,
-                               b->d.resampling_threshold, b->d.adaptive, log_w);
+        for (const esth_bclass& c : b->cls) ESTH_BCLASS(launch_kb_alias, S, b, c, R, log_w);
     } else {
         /* the exchange's receiving side runs inside the resampling */
         hipLaunchKernelGGL(kb_argmax<S>, dim3(b->n), dim3(256), 0, s, b->d_F, log_w);

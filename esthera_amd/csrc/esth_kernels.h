@@ -1683,14 +1683,22 @@ __global__ __launch_bounds__(256) void kb_resample(const esth_bfilter* __restric
                                (int)(f.flags & 1u), f.w);
 }
 
-template <int S>
-__global__ __launch_bounds__(1024) void kb_alias(const esth_bfilter* __restrict__ F, const uint2* __restrict__ sub,
-                                                 const float* __restrict__ R, float thr, int adaptive, int log_w) {
-    const uint2 e = sub[blockIdx.x];
+/* one wave per (filter, sub-filter) entry of a size class (m >= 64) */
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void kb_alias_w(const esth_bfilter* __restrict__ F, const uint2* __restrict__ sub,
+                                                  uint32_t nsub, const float* __restrict__ R, float thr, int adaptive,
+                                                  int log_w) {
+    constexpr int WPB = esth_alias_wpb<LOGM>::value;
+    __shared__ float lds[WPB][3 << LOGM];
+    const uint32_t wv = threadIdx.x / ESTH_WAVE;
+    const uint32_t i = blockIdx.x * WPB + wv;
+    if (i >= nsub) return;
+    const uint2 e = sub[i];
     const esth_bfilter& f = F[e.x];
-    esth_alias_sub<S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, f.m, e.y, thr, adaptive, log_w, nullptr,
-                      nullptr, nullptr, (int)(f.flags & 1u));
+    esth_alias_wave<LOGM, S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, thr, adaptive, log_w, nullptr, nullptr,
+                             nullptr, (int)(f.flags & 1u), ESTH_LAYOUT_SOA, lds[wv]);
 }
+
 
 /* ============================================================== serving
  * Start-up of tracks admitted in a wave: the state planes set to the initial

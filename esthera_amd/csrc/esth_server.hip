@@ -581,6 +581,14 @@ static void srv_resample(const esth_bfilter* F, const uint2* sub, uint32_t n, co
     default: return esth_fail(ESTH_ERR_INVALID, "m"); \
     }
 
+template <int LOGM, int S>
+static void srv_alias(const esth_bfilter* F, const uint2* sub, uint32_t n, const float* R, float thr, int adaptive,
+                      int log_w, hipStream_t st) {
+    constexpr int WPB = esth_alias_wpb<LOGM>::value;
+    hipLaunchKernelGGL((kb_alias_w<LOGM, S>), dim3((n + WPB - 1) / WPB), dim3(WPB * ESTH_WAVE), 0, st, F, sub, n, R, thr,
+                       adaptive, log_w);
+}
+
 template <int S>
 static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wave_group& g, const esth_bfilter* F,
                          const uint2* sub_all, const uint2* sub_cls, const uint2* chunks, const float* R,
@@ -623,8 +631,8 @@ static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wa
         }
         for (int lm = 6; lm <= 10; ++lm)
             if (g.ncls[lm])
-                hipLaunchKernelGGL(kb_alias<S>, dim3((unsigned)g.ncls[lm]), dim3(1u << lm), 0, st, Fg,
-                                   sub_cls + g.cls0[lm], R, k.thr, k.adaptive, log_w);
+                ESTH_SRV_LOGM(srv_alias, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], R, k.thr, k.adaptive,
+                              log_w, st);
         return ESTH_OK;
     }
     /* exchange inside the resampling, argmax in the first resampling launch */
