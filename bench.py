@@ -345,7 +345,7 @@ def main():
         asecs, astage, aerr = run_plain(ea, eng, ea.RESAMPLE_ALIAS, max(1, args.steps // 2), args.warmup, timing=2)
         out["alias"] = {"value": N * max(1, args.steps // 2) / asecs, "unit": "particles/s",
                         "ms_per_step": asecs * 1e3 / max(1, args.steps // 2),
-                        "stages_us": stage_dict(astage, fused=False)}
+                        "stages_us": stage_dict(astage, fused=True)}
     if not args.no_layouts:
         out["layouts"] = run_layouts(ea, eng)
     if not args.no_batched:

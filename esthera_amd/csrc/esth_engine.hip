@@ -495,6 +495,47 @@ int esth_launch_resample_argmax(int resampler, int S, float* st_main, const floa
     return ESTH_OK;
 }
 
+template <int LOGM, int S>
+static void launch_alias_argmax_t(float* st_main, const float* st_tmp, const float* w, const float* rnd, uint32_t N,
+                                  uint32_t G, float thr, int adaptive, int log_w, const float2* winner,
+                                  const float* part, const float* best_src, size_t plane_stride, uint32_t g_stride,
+                                  float2* result, double* est, const esth_xch& xch, int carry, int layout,
+                                  hipStream_t s) {
+    if constexpr (LOGM >= 6) {
+        constexpr int WPB = esth_alias_wpb<LOGM>::value;
+        hipLaunchKernelGGL((k_alias_argmax_w<LOGM, S>), dim3(1 + (G + WPB - 1) / WPB), dim3(WPB * ESTH_WAVE), 0, s,
+                           st_main, st_tmp, w, rnd, N, G, thr, adaptive, log_w, winner, part, best_src, plane_stride,
+                           g_stride, result, est, xch, carry, layout);
+    }
+}
+
+int esth_launch_alias_argmax(int S, float* st_main, const float* st_tmp, float* w, const float* rnd, uint32_t N,
+                             uint32_t m, uint32_t G, float thr, int adaptive, int log_w, const float2* winner,
+                             const float* part, const float* best_src, size_t plane_stride, uint32_t g_stride,
+                             float2* result, double* est, uint32_t t, int topo, int dimx, int dimy, const float* bt_w,
+                             const float* bt_s, uint32_t* top, int carry, int layout, hipStream_t s) {
+    if (m < ESTH_WAVE) return esth_fail(ESTH_ERR_UNSUPPORTED, "fused alias: m >= 64");
+    const int lm = log2u(m);
+    esth_xch xch{};
+    if (t > 0 && topo != ESTH_TOPO_NONE) {
+        if (topo == ESTH_TOPO_ALL2ALL) {   /* the global top t the receivers read */
+            hipLaunchKernelGGL(k_topt, dim3(1), dim3(1024), 0, s, bt_w, G * t, t, top);
+            ESTH_HIP(hipGetLastError());
+        }
+        xch = {bt_w, bt_s, top, w, G, t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, dimx, dimy};
+    }
+#define ESTH_AA_ARGS st_main, st_tmp, w, rnd, N, G, thr, adaptive, log_w, winner, part, best_src, plane_stride, g_stride, result, est, xch, carry, layout, s
+    switch (S) {
+    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_alias_argmax_t, ESTH_UNI_S, ESTH_AA_ARGS); break;
+    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_alias_argmax_t, ESTH_RA5_S, ESTH_AA_ARGS); break;
+    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_alias_argmax_t, ESTH_DH6_S, ESTH_AA_ARGS); break;
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+    }
+#undef ESTH_AA_ARGS
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
 /* ======================================================= device helpers */
 extern "C" int esth_dev_alloc(size_t bytes, void** p) {
     if (!p) return esth_fail(ESTH_ERR_INVALID, "null out");

@@ -290,14 +290,15 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
                               f->bt_s, f->part, log_w, d.state_layout, s));
     if (all) ESTH_HIP(hipEventRecord(ev[3], s));
     /* reduce (pf_generic.c:604-615), result kept on the device.  With a scan
-     * resampler it rides in the resampling launch (below); its best particle
-     * is rank 0 of the winning sub-filter, read from the sort's compact best-t
-     * copy, which the exchange does not touch. */
-    const bool fused = d.resampler != ESTH_RESAMPLE_ALIAS;
+     * resampler, or the one-wave alias (m >= 64), it rides in the resampling
+     * launch (below); its best particle is rank 0 of the winning sub-filter,
+     * read from the sort's compact best-t copy, which the exchange does not
+     * touch. */
+    const bool fused = d.resampler != ESTH_RESAMPLE_ALIAS || f->m >= 64;   /* one wave per sub-filter */
     if (!fused) ESTH_TRY(esth_launch_argmax(f->S, f->winner, f->G, log_w, f->part, f->tmp, N, f->m, f->result, f->est, s));
     if (all) ESTH_HIP(hipEventRecord(ev[4], s));
-    /* exchange (pf_generic.c:702-715): with a scan resampler its receiving
-     * side is applied by the resampling launch as it reads each sub-filter */
+    /* exchange (pf_generic.c:702-715): in the fused launches its receiving
+     * side is applied by the resampling as it reads each sub-filter */
     if (!fused)
         ESTH_TRY(esth_launch_exchange(f->tmp, f->w, f->S, N, f->m, f->G, f->t, d.topology, f->dimx, f->dimy, f->bt_w,
                                       f->bt_s, f->top, s));
@@ -305,12 +306,20 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
     /* resampling (pf_generic.c:732-757), uniforms at rand_offset = N*V */
     if (fused) {
         const bool bt = f->t > 0 && d.topology != ESTH_TOPO_NONE;
-        ESTH_TRY(esth_launch_resample_argmax(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m,
-                                             f->G, d.resampling_threshold, log_w, f->winner, f->part,
-                                             bt ? f->bt_s : f->tmp, bt ? (size_t)f->G * f->t : (size_t)N,
-                                             bt ? f->t : f->m, f->result, f->est, f->t, d.topology, f->dimx,
-                                             f->dimy, f->bt_w, f->bt_s, f->top, d.carry_weights, d.state_layout,
-                                             s));
+        if (d.resampler == ESTH_RESAMPLE_ALIAS)
+            ESTH_TRY(esth_launch_alias_argmax(f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m, f->G,
+                                              d.resampling_threshold, d.adaptive, log_w, f->winner, f->part,
+                                              bt ? f->bt_s : f->tmp, bt ? (size_t)f->G * f->t : (size_t)N,
+                                              bt ? f->t : f->m, f->result, f->est, f->t, d.topology, f->dimx,
+                                              f->dimy, f->bt_w, f->bt_s, f->top, d.carry_weights, d.state_layout,
+                                              s));
+        else
+            ESTH_TRY(esth_launch_resample_argmax(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N,
+                                                 f->m, f->G, d.resampling_threshold, log_w, f->winner, f->part,
+                                                 bt ? f->bt_s : f->tmp, bt ? (size_t)f->G * f->t : (size_t)N,
+                                                 bt ? f->t : f->m, f->result, f->est, f->t, d.topology, f->dimx,
+                                                 f->dimy, f->bt_w, f->bt_s, f->top, d.carry_weights,
+                                                 d.state_layout, s));
     } else {
         ESTH_TRY(esth_launch_resample(d.resampler, f->S, f->state, f->tmp, f->w, R + (size_t)N * f->V, N, f->m, f->G,
                                       d.resampling_threshold, d.adaptive, log_w, nullptr, d.carry_weights,

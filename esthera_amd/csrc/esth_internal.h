@@ -73,6 +73,12 @@ int esth_launch_resample_argmax(int resampler, int S, float* st_main, const floa
                                 uint32_t g_stride, float2* result, double* est, uint32_t t, int topo, int dimx,
                                 int dimy, const float* bt_w, const float* bt_s, uint32_t* top, int carry,
                                 int layout, hipStream_t s);
+/* the same with the one-wave Vose alias resampling (m >= 64) */
+int esth_launch_alias_argmax(int S, float* st_main, const float* st_tmp, float* w, const float* rnd, uint32_t N,
+                             uint32_t m, uint32_t G, float thr, int adaptive, int log_w, const float2* winner,
+                             const float* part, const float* best_src, size_t plane_stride, uint32_t g_stride,
+                             float2* result, double* est, uint32_t t, int topo, int dimx, int dimy, const float* bt_w,
+                             const float* bt_s, uint32_t* top, int carry, int layout, hipStream_t s);
 /* the global top t of the compact best-t buffer (all-to-all exchange) */
 int esth_launch_topt(const float* bt_w, uint32_t n, uint32_t t, uint32_t* top, hipStream_t s);
 

@@ -1348,7 +1348,7 @@ __device__ __forceinline__ void esth_alias_wave(float* __restrict__ st_main, con
                                                 uint32_t g, float thr, int adaptive, int log_w,
                                                 int* __restrict__ sel, float* __restrict__ prob_out,
                                                 uint32_t* __restrict__ alias_out, int carry, int layout,
-                                                float* lds) {
+                                                float* lds, bool recv = false, esth_xch xch = esth_xch{}) {
     static_assert(LOGM >= 6, "one wave per sub-filter needs M >= 64");
     constexpr int M = 1 << LOGM, E = M / ESTH_WAVE;
     constexpr int J = (M / 2 + ESTH_WAVE - 1) / ESTH_WAVE;   /* a round pairs at most M/2 */
@@ -1359,11 +1359,36 @@ __device__ __forceinline__ void esth_alias_wave(float* __restrict__ st_main, con
     uint32_t* alias = reinterpret_cast<uint32_t*>(lds + M);
     float* scratch = lds + 2 * M;                           /* sums, then the worklists */
     uint32_t* sl = reinterpret_cast<uint32_t*>(lds + 2 * M);
+    uint32_t* rtab = reinterpret_cast<uint32_t*>(lds + 3 * M);   /* received slots' sources */
     const bool table_only = prob_out != nullptr;
+    /* the exchange's receiving side (as in esth_resample_sub): slot M-1-j
+     * holds rank k of neighbour q, j = q t + k, read from the compact best-t
+     * copy (its index into it tabulated once, rtab[j]); its weight is written
+     * back to w */
+    recv = recv && xch.t > 0;
+    const uint32_t nrecv = recv ? xch.nq * xch.t : 0u;
+    const size_t Gt = (size_t)xch.G * xch.t;
+    if (recv) {
+        for (uint32_t j = (uint32_t)lane; j < nrecv; j += ESTH_WAVE) {
+            const uint32_t q = j / xch.t, k = j - q * xch.t;
+            rtab[j] = (uint32_t)esth_recv_src(g, q, k, xch.G, xch.t, xch.topo, xch.dimx, xch.dimy, xch.top);
+        }
+        esth_wave_lds_fence();
+    }
     bool skip = !table_only && !adaptive && (rnd[N + g] - 1.0f) > thr;
     float p[E];
+    esth_load_run<E>(w + base + le, p);
+    if (recv) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t pos = (uint32_t)(le + r);
+            if (pos >= (uint32_t)M - nrecv) {
+                p[r] = xch.bt_w[rtab[(uint32_t)M - 1u - pos]];
+                xch.w_out[base + pos] = p[r];
+            }
+        }
+    }
     if (!skip) {
-        esth_load_run<E>(w + base + le, p);
         if (log_w) {
             float mx = p[0];
 #pragma unroll
@@ -1398,6 +1423,13 @@ __device__ __forceinline__ void esth_alias_wave(float* __restrict__ st_main, con
             for (int c = 0; c < S; ++c) {
                 float v[E];
                 esth_load_run<E>(st_tmp + (size_t)c * N + base + le, v);
+                if (recv) {
+#pragma unroll
+                    for (int r = 0; r < E; ++r) {
+                        const uint32_t pos = (uint32_t)(le + r);
+                        if (pos >= (uint32_t)M - nrecv) v[r] = xch.bt_s[(size_t)c * Gt + rtab[(uint32_t)M - 1u - pos]];
+                    }
+                }
                 esth_store_plane_run<E, S>(st_main, layout, N, c, base + le, v);
             }
         }
@@ -1510,11 +1542,19 @@ __device__ __forceinline__ void esth_alias_wave(float* __restrict__ st_main, con
         const uint32_t col = (uint32_t)floorf(r0 * (float)M);
         pick[r] = (int)((r1 < prob[col]) ? col : alias[col]);
     }
+    size_t src[E];   /* where each pick's particle is: the sorted copy or a received slot */
+    bool rx[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        rx[r] = (uint32_t)pick[r] >= (uint32_t)M - nrecv;
+        src[r] = rx[r] ? (size_t)rtab[(uint32_t)M - 1u - (uint32_t)pick[r]] : base + (size_t)pick[r];
+    }
 #pragma unroll
     for (int c = 0; c < S; ++c) {
         float v[E];
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = st_tmp[(size_t)c * N + base + (size_t)pick[r]];
+        for (int r = 0; r < E; ++r)
+            v[r] = rx[r] ? xch.bt_s[(size_t)c * Gt + src[r]] : st_tmp[(size_t)c * N + src[r]];
         esth_store_plane_run<E, S>(st_main, layout, N, c, base + le, v);
     }
     if (sel) {
@@ -1529,7 +1569,7 @@ __device__ __forceinline__ void esth_alias_wave(float* __restrict__ st_main, con
     }
 }
 
-/* waves per work-group of the one-wave alias kernels (LDS: 12 M bytes a wave) */
+/* waves per work-group of the one-wave alias kernels (LDS: 16 M bytes a wave) */
 template <int LOGM>
 struct esth_alias_wpb {
     static constexpr int value = LOGM >= 10 ? 2 : 4;
@@ -1542,12 +1582,44 @@ __global__ __launch_bounds__(256) void k_alias_w(float* __restrict__ st_main, co
                                                  int* __restrict__ sel, float* __restrict__ prob_out,
                                                  uint32_t* __restrict__ alias_out, int carry, int layout) {
     constexpr int WPB = esth_alias_wpb<LOGM>::value;
-    __shared__ float lds[WPB][3 << LOGM];
+    __shared__ float lds[WPB][4 << LOGM];
     const uint32_t wv = threadIdx.x / ESTH_WAVE;
     const uint32_t g = blockIdx.x * WPB + wv;
     if (g >= G) return;   /* a whole wave: no work-group barrier follows */
     esth_alias_wave<LOGM, S>(st_main, st_tmp, w, rnd, N, g, thr, adaptive, log_w, sel, prob_out, alias_out, carry,
                              layout, lds[wv]);
+}
+
+/* The step's argmax + posterior (work-group 0) and the exchange's receiving
+ * side beside the one-wave alias resampling, as k_resample_argmax does for
+ * the scan resamplers: the argmax reads the winners, partials and best-t
+ * copy, the resampling the weights, uniforms, sorted state and best-t copy;
+ * neither writes what the other reads. */
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void k_alias_argmax_w(float* __restrict__ st_main, const float* __restrict__ st_tmp,
+                                                        const float* w, const float* __restrict__ rnd, uint32_t N,
+                                                        uint32_t G, float thr, int adaptive, int log_w,
+                                                        const float2* __restrict__ winner,
+                                                        const float* __restrict__ part,
+                                                        const float* __restrict__ best_src, size_t plane_stride,
+                                                        uint32_t g_stride, float2* __restrict__ result,
+                                                        double* __restrict__ est, esth_xch xch, int carry,
+                                                        int layout) {
+    constexpr int WPB = esth_alias_wpb<LOGM>::value;
+    __shared__ float lds[WPB][4 << LOGM];
+    __shared__ float2 sc[128];
+    __shared__ float smax[WPB];
+    __shared__ double sd[WPB * (2 + S)];
+    if (blockIdx.x == 0) {
+        esth_argmax_block<S, WPB * ESTH_WAVE>(winner, G, log_w, part, best_src, plane_stride, g_stride, result, est, sc,
+                                              smax, sd);
+        return;
+    }
+    const uint32_t wv = threadIdx.x / ESTH_WAVE;
+    const uint32_t g = (blockIdx.x - 1) * WPB + wv;
+    if (g >= G) return;
+    esth_alias_wave<LOGM, S>(st_main, st_tmp, w, rnd, N, g, thr, adaptive, log_w, nullptr, nullptr, nullptr, carry,
+                             layout, lds[wv], true, xch);
 }
 
 /* ============================================================== batches
@@ -1689,7 +1761,7 @@ __global__ __launch_bounds__(256) void kb_alias_w(const esth_bfilter* __restrict
                                                   uint32_t nsub, const float* __restrict__ R, float thr, int adaptive,
                                                   int log_w) {
     constexpr int WPB = esth_alias_wpb<LOGM>::value;
-    __shared__ float lds[WPB][3 << LOGM];
+    __shared__ float lds[WPB][4 << LOGM];
     const uint32_t wv = threadIdx.x / ESTH_WAVE;
     const uint32_t i = blockIdx.x * WPB + wv;
     if (i >= nsub) return;
