@@ -273,6 +273,26 @@ def traffic_from_profiles():
         return json.load(fh).get("bytes_per_launch")
 
 
+# VALU issue ceiling of MI355X: 1024 SIMDs (256 CUs x 4), each issuing at most
+# two VALU wave-instructions per quad-cycle, at the 2.4 GHz peak engine clock
+VALU_PEAK_WAVE_INST_S = 1024 * 2.4e9 / 2
+
+
+def valu_from_profiles(kernel_us):
+    """VALU issue rate of k_sample_weight: the wave-instructions per launch
+    counted by rocprofv3 (profiles/sample_weight_unicycle2m_valu.md) over the
+    launch time measured here, against the dual-issue ceiling."""
+    p = os.path.join(REPO, "profiles", "sample_weight_unicycle2m_valu.json")
+    if not os.path.exists(p) or not kernel_us:
+        return None
+    with open(p) as fh:
+        ins = json.load(fh)["valu_wave_insts_per_launch"]
+    achieved = ins / (kernel_us * 1e-6)
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_WAVE_INST_S / 1e12,
+            "unit": "T wave-instructions/s", "frac": achieved / VALU_PEAK_WAVE_INST_S,
+            "wave_insts_per_launch": ins}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,7 +355,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
                      "kernel": "k_sample_weight<unicycle>", "kernel_us": sw_us,
-                     "bytes_per_launch": SW_BYTES_PER_PARTICLE * N},
+                     "bytes_per_launch": SW_BYTES_PER_PARTICLE * N,
+                     "valu": valu_from_profiles(sw_us)},
         "stages_us": stage_dict(stage_all, fused=True),
         "position_error_last_step": err,
     }
