@@ -46,10 +46,10 @@ def stage_dict(us, fused):
     return d
 
 
-def run_plain(ea, eng, resampler, steps, warmup, graph=False, timing=1, every=1):
+def run_plain(ea, eng, resampler, steps, warmup, graph=False, timing=1, every=1, landmarks=L, box=50.0):
     desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=G, t=T, topology=ea.TOPO_TORUS, resampler=resampler,
                           weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
-    params, ctrl, meas, truth = ea.unicycle_scenario(42, L, 50.0, warmup + steps)
+    params, ctrl, meas, truth = ea.unicycle_scenario(42, landmarks, box, warmup + steps)
     f = ea.Filter(eng, desc, params)
     import numpy as np
     f.init_uniform(np.zeros(3, np.float32))
@@ -96,6 +96,30 @@ def run_layouts(ea, eng):
     us, best = ea.autotune_layout(eng, d, None, c, m, 0.0589)
     out["robotarm_2m"] = {"us_per_step": dict(zip(names, map(float, us))), "best": names[best]}
     return out
+
+
+def run_accuracy(ea, eng, steps=500):
+    """Position RMSE against the seeded ground truth: the small unicycle
+    workload (16 x 256 particles, 4 landmarks at (+-10, +-10), 4 x 4 torus,
+    t = 1, systematic resampling, seed 1234) over 500 steps, posterior mean
+    and best particle."""
+    import numpy as np
+    d = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=16, t=1, topology=ea.TOPO_TORUS,
+                       resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, seed=1234)
+    p, ctrl, meas, truth = ea.unicycle_scenario(1234, 4, 0.0, steps)
+    f = ea.Filter(eng, d, p)
+    f.init_uniform(np.zeros(3, np.float32))
+    em, eb = [], []
+    for k in range(steps):
+        f.step(ctrl[k], meas[k], 0.1)
+        est = f.estimate()
+        em.append(np.hypot(est.mean[0] - truth[k, 0], est.mean[1] - truth[k, 1]))
+        eb.append(np.hypot(est.best[0] - truth[k, 0], est.best[1] - truth[k, 1]))
+    f.close()
+    return {"workload": "unicycle 16 x 256, 4 landmarks, 4x4 torus t=1, systematic, seed 1234, 500 steps",
+            "rmse_posterior_mean": float(np.sqrt(np.mean(np.square(em)))),
+            "rmse_best_particle": float(np.sqrt(np.mean(np.square(eb)))),
+            "max_error_posterior_mean": float(np.max(em))}
 
 
 def cpu_baseline(budget_s=15.0, max_steps=200):
@@ -360,6 +384,15 @@ def main():
         "stages_us": stage_dict(stage_all, fused=True),
         "position_error_last_step": err,
     }
+    # the same kernel where the likelihood is light: 4 landmarks at (+-10, +-10)
+    # (the small unicycle workload's set) on the same 2M particles; sample+weight
+    # is then HBM-bound and its HBM fraction measures the kernel's memory path
+    _, stage4, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 200), args.warmup, every=10,
+                             landmarks=4, box=0.0)
+    a4 = SW_BYTES_PER_PARTICLE * N / (stage4[1] * 1e-6) / 1e9
+    out["roofline"]["light_likelihood"] = {"workload": "same 2097152 particles, 4 landmarks", "kernel_us": stage4[1],
+                                           "achieved": a4, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                           "frac": a4 / HBM_PEAK_GBS}
     gsecs, _, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, graph=True)
     out["graph_replay"] = {"value": N * args.steps / gsecs, "ms_per_step": gsecs * 1e3 / args.steps}
     if not args.no_alias:
@@ -367,6 +400,7 @@ def main():
         out["alias"] = {"value": N * max(1, args.steps // 2) / asecs, "unit": "particles/s",
                         "ms_per_step": asecs * 1e3 / max(1, args.steps // 2),
                         "stages_us": stage_dict(astage, fused=True)}
+    out["accuracy"] = run_accuracy(ea, eng)
     if not args.no_layouts:
         out["layouts"] = run_layouts(ea, eng)
     if not args.no_batched:

@@ -209,3 +209,38 @@ def test_prefetch_on_off_identical(graph, engine):
     sb, wb = b.particles()
     assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
     assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+
+
+def test_small_unicycle_500_steps_rmse(engine):
+    """The small unicycle workload end to end: 16 x 256 particles, 4 landmarks
+    at (+-10, +-10), 4 x 4 torus with one particle per neighbour, systematic
+    resampling every step, seed 1234, 500 steps against the seeded ground
+    truth.  Particles stay bit-identical to the oracle, posterior means within
+    1e-4 of the FP64 twin at every step, and the position RMSE of the
+    posterior mean is the oracle's and small."""
+    import ctypes
+    d = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=16, t=1, topology=ea.TOPO_TORUS,
+                       resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, seed=1234)
+    p, ctrl, meas, truth = ea.unicycle_scenario(1234, 4, 0.0, 500)
+    f = ea.Filter(engine, d, p)
+    o = O.Filter(d, p, ctypes.sizeof(p))
+    f.init_uniform(np.zeros(3, np.float32))
+    o.init_uniform(np.zeros(3, np.float32))
+    eg, eo, worst = [], [], 0.0
+    for k in range(500):
+        f.step(ctrl[k], meas[k], 0.1)
+        o.step(ctrl[k], meas[k], 0.1)
+        est = f.estimate()
+        m32, m64, best = o.estimate(3)
+        mean = np.array(est.mean[:3])
+        eg.append(np.hypot(mean[0] - truth[k, 0], mean[1] - truth[k, 1]))
+        eo.append(np.hypot(m32[0] - truth[k, 0], m32[1] - truth[k, 1]))
+        worst = max(worst, float(np.max(np.abs(mean - m64) / np.maximum(np.abs(m64), 1.0))))
+    gs, gw = f.particles()
+    os_, ow = o.read(3)
+    assert np.array_equal(gs.view(np.uint32), os_.view(np.uint32))
+    rmse_g, rmse_o = float(np.sqrt(np.mean(np.square(eg)))), float(np.sqrt(np.mean(np.square(eo))))
+    assert worst < 1e-4
+    assert abs(rmse_g - rmse_o) < 1e-6 * max(rmse_o, 1.0)
+    assert rmse_g < 0.2, rmse_g
+    f.close()
