@@ -218,8 +218,6 @@ __device__ __forceinline__ void esth_sample_pair_regs(float (&s0)[esth_model_tra
     }
 }
 
-/* carry: multiply the particles' carried weights by the likelihoods (add the
- * log-likelihoods in log mode) instead of overwriting them */
 /* particles p and p+1 (p even) of a state in layout L: SOA one float2 per
  * plane; AOS the pair's 2S consecutive floats as float2s; PACKED, per group
  * of w planes, the pair's 2w consecutive floats (float4s where w is 2 or 4:

@@ -189,6 +189,38 @@ def test_two_million_particle_properties(engine):
     assert np.hypot(est.mean[0] - truth[2, 0], est.mean[1] - truth[2, 1]) < 0.5
 
 
+def test_maximum_size_properties(engine):
+    """The largest filter the descriptor admits: 65536 sub-filters of 1024
+    particles (67,108,864 particles, a 256 x 256 torus exchanging 4 per
+    neighbour): two steps run and leave finite particles, weights sorted per
+    sub-filter up to the received slots, received slots holding the senders'
+    best weights, and a posterior mean near the truth; one more sub-filter or
+    a larger one is refused."""
+    m, G, t = 1024, 65536, 4
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=m, G=G, t=t, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG,
+                          resampler=ea.RESAMPLE_SYSTEMATIC, seed=3)
+    params, ctrl, meas, truth = ea.unicycle_scenario(3, 8, 20.0, 2)
+    f = ea.Filter(engine, desc, params)
+    f.init_uniform(np.zeros(3, np.float32))
+    for k in range(2):
+        f.step(ctrl[k], meas[k], 0.1)
+    est = f.estimate()
+    st, w = f.particles()
+    assert np.isfinite(st).all()
+    w = w.reshape(G, m)
+    head = w[:, : m - 4 * t]
+    assert np.all(head[:, :-1] >= head[:, 1:])
+    # every received weight is one of the best t of some sub-filter
+    best = np.sort(head[:, :t].reshape(-1))
+    recv = w[:, m - 4 * t:].reshape(-1)
+    assert np.isin(recv, best).all()
+    assert np.hypot(est.mean[0] - truth[1, 0], est.mean[1] - truth[1, 1]) < 0.5
+    f.close()
+    for bad in (dict(m=2048, G=16), dict(m=1024, G=65537)):
+        with pytest.raises(ea.EsthError):
+            ea.Filter(engine, ea.filter_desc(ea.MODEL_UNICYCLE, **bad), params)
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_prefetch_on_off_identical(graph, engine):
     """Generating step n+1's MTGP words on a second stream during step n
