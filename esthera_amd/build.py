@@ -7,8 +7,8 @@ Steps:
      (the 200 MTGP32-11213 parameter sets as a C initializer list)
   2. hipcc --offload-arch=gfx950 on every csrc/*.hip, linked into
      esthera_amd/libesthera.so
-  3. g++ on csrc/esth_pf_main.cpp -> esthera_amd/esthera_pf (the command
-     line, host code over the C ABI)
+  3. gcc on csrc/esth_pf_main.c -> esthera_amd/esthera_pf (the command
+     line, C host code over the C ABI)
   4. make -C oracle (test infrastructure), and make -C oracle ref when the
      reference tree is present (never needed on a GPU box).
 """
@@ -98,10 +98,10 @@ CLI = os.path.join(HERE, "esthera_pf")
 
 def build_cli():
     """esthera_pf: the reference's particle_filter command line over the C ABI
-    (host C++ only, linked against libesthera.so next to it)."""
+    (plain C, as the reference's driver; linked against libesthera.so next to it)."""
     tmp = CLI + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(REPO, 'include')}",
-           os.path.join(CSRC, "esth_pf_main.cpp"), f"-L{HERE}", "-lesthera", "-Wl,-rpath,$ORIGIN",
+    cmd = ["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", f"-I{os.path.join(REPO, 'include')}",
+           os.path.join(CSRC, "esth_pf_main.c"), f"-L{HERE}", "-lesthera", "-lm", "-Wl,-rpath,$ORIGIN",
            "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -21,15 +21,19 @@
  * the best particle), or with -T "ex ey ew ee ax ay" then one line per step
  * (print_estimate, cuda/pf_model_soa_packed_code.cuh:237-240).
  */
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <string>
+#define _POSIX_C_SOURCE 200809L
+#include <math.h>
+#include <stdbool.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
 #include <unistd.h>
-#include <vector>
 
 #include "esthera.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 
 static void usage(const char* prog) {
     fprintf(stderr,
@@ -39,7 +43,7 @@ static void usage(const char* prog) {
 }
 
 static bool parse_range(const char* s, int* a, int* b) {
-    char* end = nullptr;
+    char* end = NULL;
     *a = (int)strtol(s, &end, 10);
     if (end == s) return false;
     if (*end == ':') {
@@ -53,7 +57,7 @@ static bool parse_range(const char* s, int* a, int* b) {
 }
 
 static bool parse_frange(const char* s, double* a, double* b) {
-    char* end = nullptr;
+    char* end = NULL;
     *a = strtod(s, &end);
     if (end == s) return false;
     if (*end == ':') {
@@ -67,13 +71,13 @@ static bool parse_frange(const char* s, double* a, double* b) {
 }
 
 struct trace {
-    int model = ESTH_MODEL_ROBOTARM5;
-    int rows = 0;
-    std::vector<esth_robotarm5_measurement> ra_meas;
-    std::vector<esth_robotarm5_control> ra_ctrl;
-    std::vector<esth_unicycle10_measurement> u_meas;
-    std::vector<esth_unicycle_control> u_ctrl;
-    std::vector<float> dt, truth;   /* 9 (arm) or 3 (unicycle) floats per row */
+    int model;
+    int rows;
+    esth_robotarm5_measurement* ra_meas;
+    esth_robotarm5_control* ra_ctrl;
+    esth_unicycle10_measurement* u_meas;
+    esth_unicycle_control* u_ctrl;
+    float *dt, *truth;   /* 9 (arm) or 3 (unicycle) floats per row */
 };
 
 static int count_rows(const char* path) {
@@ -89,34 +93,44 @@ static int count_rows(const char* path) {
     return n;
 }
 
-static int load_trace(const char* path, trace* tr) {
+static int load_trace(const char* path, struct trace* tr) {
     const int cap = count_rows(path);
     if (cap < 0) {
         fprintf(stderr, "could not open: %s\n", path);
         return ESTH_ERR_IO;
     }
-    tr->dt.resize(cap > 0 ? cap : 1);
+    const size_t n = cap > 0 ? (size_t)cap : 1;
+    tr->dt = calloc(n, sizeof(float));
     if (tr->model == ESTH_MODEL_UNICYCLE10) {
-        tr->u_meas.resize(cap > 0 ? cap : 1);
-        tr->u_ctrl.resize(cap > 0 ? cap : 1);
-        tr->truth.resize((size_t)(cap > 0 ? cap : 1) * 3);
-        return esth_trace_load_unicycle10(path, cap, tr->u_meas.data(), tr->u_ctrl.data(), tr->dt.data(),
-                                          tr->truth.data(), &tr->rows);
+        tr->u_meas = calloc(n, sizeof(*tr->u_meas));
+        tr->u_ctrl = calloc(n, sizeof(*tr->u_ctrl));
+        tr->truth = calloc(n * 3, sizeof(float));
+        if (!tr->dt || !tr->u_meas || !tr->u_ctrl || !tr->truth) return ESTH_ERR_NOMEM;
+        return esth_trace_load_unicycle10(path, cap, tr->u_meas, tr->u_ctrl, tr->dt, tr->truth, &tr->rows);
     }
-    tr->ra_meas.resize(cap > 0 ? cap : 1);
-    tr->ra_ctrl.resize(cap > 0 ? cap : 1);
-    tr->truth.resize((size_t)(cap > 0 ? cap : 1) * 9);
-    return esth_trace_load_robotarm(path, cap, tr->ra_meas.data(), tr->ra_ctrl.data(), tr->dt.data(),
-                                    tr->truth.data(), &tr->rows);
+    tr->ra_meas = calloc(n, sizeof(*tr->ra_meas));
+    tr->ra_ctrl = calloc(n, sizeof(*tr->ra_ctrl));
+    tr->truth = calloc(n * 9, sizeof(float));
+    if (!tr->dt || !tr->ra_meas || !tr->ra_ctrl || !tr->truth) return ESTH_ERR_NOMEM;
+    return esth_trace_load_robotarm(path, cap, tr->ra_meas, tr->ra_ctrl, tr->dt, tr->truth, &tr->rows);
+}
+
+static void free_trace(struct trace* tr) {
+    free(tr->ra_meas);
+    free(tr->ra_ctrl);
+    free(tr->u_meas);
+    free(tr->u_ctrl);
+    free(tr->dt);
+    free(tr->truth);
 }
 
 /* one run of one configuration; returns the mean error over the trace */
-static int run(esth_engine* e, const esth_filter_desc& d, const void* params, size_t psize, const trace& tr,
+static int run(esth_engine* e, const esth_filter_desc* d, const void* params, size_t psize, const struct trace* tr,
                bool per_step, double* mean_error) {
-    esth_filter* f = nullptr;
-    int rc = esth_filter_create(e, &d, params, psize, &f);
+    esth_filter* f = NULL;
+    int rc = esth_filter_create(e, d, params, psize, &f);
     if (rc != ESTH_OK) return rc;
-    if (tr.model == ESTH_MODEL_UNICYCLE10) {
+    if (tr->model == ESTH_MODEL_UNICYCLE10) {
         const float init[3] = {20.0f, 40.0f, (float)(-M_PI / 2)};   /* particle_filter.cu:712-721 */
         rc = esth_filter_init_uniform(f, init);
     } else {
@@ -125,22 +139,22 @@ static int run(esth_engine* e, const esth_filter_desc& d, const void* params, si
         rc = esth_filter_init_uniform(f, init);
     }
     double err_sum = 0.0;
-    for (int i = 0; rc == ESTH_OK && i < tr.rows; ++i) {
-        if (tr.model == ESTH_MODEL_UNICYCLE10)
-            rc = esth_filter_step(f, &tr.u_ctrl[i], &tr.u_meas[i], tr.dt[i]);
+    for (int i = 0; rc == ESTH_OK && i < tr->rows; ++i) {
+        if (tr->model == ESTH_MODEL_UNICYCLE10)
+            rc = esth_filter_step(f, &tr->u_ctrl[i], &tr->u_meas[i], tr->dt[i]);
         else
-            rc = esth_filter_step(f, &tr.ra_ctrl[i], &tr.ra_meas[i], tr.dt[i]);
+            rc = esth_filter_step(f, &tr->ra_ctrl[i], &tr->ra_meas[i], tr->dt[i]);
         esth_estimate est;
         if (rc == ESTH_OK) rc = esth_filter_estimate(f, &est);
         if (rc != ESTH_OK) break;
         double err;
         const float* act;
-        if (tr.model == ESTH_MODEL_UNICYCLE10) {
-            act = &tr.truth[(size_t)i * 3];
+        if (tr->model == ESTH_MODEL_UNICYCLE10) {
+            act = &tr->truth[(size_t)i * 3];
             const double ex = (double)est.best[0] - act[0], ey = (double)est.best[1] - act[1];
-            err = std::sqrt(ex * ex + ey * ey);
+            err = sqrt(ex * ex + ey * ey);
         } else {
-            act = &tr.truth[(size_t)i * 9];
+            act = &tr->truth[(size_t)i * 9];
             float best[9];
             for (int c = 0; c < 9; ++c) best[c] = est.best[c];
             best[7] = 0.0f;   /* the reference reads back x, y and the angles only (pf_generic.c:649-663) */
@@ -149,12 +163,12 @@ static int run(esth_engine* e, const esth_filter_desc& d, const void* params, si
         }
         err_sum += err;
         if (per_step) {
-            const int ix = tr.model == ESTH_MODEL_UNICYCLE10 ? 0 : 5;
+            const int ix = tr->model == ESTH_MODEL_UNICYCLE10 ? 0 : 5;
             printf("%f %f %f %f %f %f\n", est.best[ix], est.best[ix + 1], est.best_weight, err, act[ix], act[ix + 1]);
         }
     }
     esth_filter_destroy(f);
-    if (rc == ESTH_OK) *mean_error = tr.rows ? err_sum / tr.rows : 0.0;
+    if (rc == ESTH_OK) *mean_error = tr->rows ? err_sum / tr->rows : 0.0;
     return rc;
 }
 
@@ -164,7 +178,9 @@ int main(int argc, char* argv[]) {
     bool r_given = false, verbose = false, per_step = false, adaptive = false;
     int resampler = ESTH_RESAMPLE_MULTINOMIAL;
     uint32_t seed = 1234;
-    trace tr;
+    struct trace tr;
+    memset(&tr, 0, sizeof(tr));
+    tr.model = ESTH_MODEL_ROBOTARM5;
     int opt;
     while ((opt = getopt(argc, argv, "m:N:t:r:l:d:cvM:R:As:Th")) != -1) {
         switch (opt) {
@@ -203,7 +219,7 @@ int main(int argc, char* argv[]) {
             else { fprintf(stderr, "-R: multinomial, systematic or alias\n"); return EXIT_FAILURE; }
             break;
         case 'A': adaptive = true; break;
-        case 's': seed = (uint32_t)strtoul(optarg, nullptr, 10); break;
+        case 's': seed = (uint32_t)strtoul(optarg, NULL, 10); break;
         case 'T': per_step = true; break;
         default: usage(argv[0]); return EXIT_FAILURE;
         }
@@ -223,11 +239,13 @@ int main(int argc, char* argv[]) {
     }
     if (load_trace(argv[optind], &tr) != ESTH_OK) {
         fprintf(stderr, "could not read trace %s: %s\n", argv[optind], esth_last_error());
+        free_trace(&tr);
         return EXIT_FAILURE;
     }
-    esth_engine* e = nullptr;
+    esth_engine* e = NULL;
     if (esth_engine_create(device, 0, &e) != ESTH_OK) {
         fprintf(stderr, "device %d: %s\n", device, esth_last_error());
+        free_trace(&tr);
         return EXIT_FAILURE;
     }
     if (verbose) {
@@ -237,7 +255,7 @@ int main(int argc, char* argv[]) {
     }
     esth_unicycle10_params u10;
     esth_camera_params cam;
-    const void* params = nullptr;
+    const void* params = NULL;
     size_t psize = 0;
     if (tr.model == ESTH_MODEL_UNICYCLE10) {
         esth_unicycle10_params_default(&u10);
@@ -266,7 +284,7 @@ int main(int argc, char* argv[]) {
                         d.adaptive = adaptive ? 1 : 0;
                         d.seed = seed;
                         double err = 0.0;
-                        if (run(e, d, params, psize, tr, per_step, &err) != ESTH_OK) {
+                        if (run(e, &d, params, psize, &tr, per_step, &err) != ESTH_OK) {
                             fprintf(stderr, "m=%d N=%d t=%d r=%f: %s\n", m, G, t, r, esth_last_error());
                             status = EXIT_FAILURE;
                             break;
@@ -275,5 +293,6 @@ int main(int argc, char* argv[]) {
                         fflush(stdout);
                     }
     esth_engine_destroy(e);
+    free_trace(&tr);
     return status;
 }
