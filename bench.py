@@ -333,7 +333,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+              f"(sharding across GPUs is not part of it)", file=sys.stderr)
         sys.exit(2)
 
     import esthera_amd as ea

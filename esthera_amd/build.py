@@ -10,7 +10,7 @@ Steps:
   3. gcc on csrc/esth_pf_main.c -> esthera_amd/esthera_pf (the command
      line, C host code over the C ABI)
   4. make -C oracle (test infrastructure), and make -C oracle ref when the
-     reference tree is present (never needed on a GPU box).
+     reference tree is present (not needed where only the built library runs).
 """
 import concurrent.futures as cf
 import os
