@@ -335,9 +335,9 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
 static int capture_graph(esth_filter* f, int slot, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     ESTH_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+    /* the kernels only: the frame reaches slot `slot` by a copy ahead of each
+     * replay, on the RNG stream (as in eager steps) */
     const float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
-    ESTH_HIP(hipMemcpyAsync((void*)dfr, f->h_frames + (size_t)slot * ESTH_FRAME_FLOATS, sizeof(float) * ESTH_FRAME_FLOATS,
-                            hipMemcpyHostToDevice, f->stream));
     int rc = enqueue_step(f, dfr, nullptr);
     hipError_t e2 = hipStreamEndCapture(f->stream, &g);
     if (rc != ESTH_OK) { if (g) hipGraphDestroy(g); return rc; }
@@ -378,7 +378,7 @@ extern "C" int esth_filter_step(esth_filter* f, const void* control, const void*
     float* dfr = f->frames + (size_t)slot * ESTH_FRAME_FLOATS;
     if (ev && f->timing >= 2) ESTH_HIP(hipEventRecord(ev[0], f->stream));
     hipStream_t copy_stream = f->stream;
-    ESTH_TRY(rng_before_step(f, dfr, hf, !f->use_graph, &copy_stream));
+    ESTH_TRY(rng_before_step(f, dfr, hf, true, &copy_stream));
     if (f->use_graph) {
         ESTH_HIP(hipGraphLaunch(f->gexec[slot], f->stream));
     } else {
@@ -386,8 +386,8 @@ extern "C" int esth_filter_step(esth_filter* f, const void* control, const void*
         f->slot = (f->slot + 1) % ESTH_FRAME_RING;
     }
     ESTH_TRY(rng_after_step(f));
-    if (f->use_graph) {
-        ESTH_HIP(hipEventRecord(f->frame_ev[slot], f->stream));
+    if (f->use_graph) {   /* the host slot is free again once its copy has run */
+        ESTH_HIP(hipEventRecord(f->frame_ev[slot], copy_stream));
         f->frame_used[slot] = true;
     } else if (slot == ESTH_FRAME_RING - 1) {
         ESTH_HIP(hipEventRecord(f->ring_ev, copy_stream));
