@@ -13,14 +13,14 @@ pytestmark = pytest.mark.gpu
 
 
 def make_filters(nf, seed, sizes, topology=ea.TOPO_RING, t=1, resampler=ea.RESAMPLE_SYSTEMATIC, threshold=1.0,
-                 carry=0):
+                 carry=0, adaptive=0):
     rng = np.random.default_rng(seed)
     descs, params, scen = [], [], []
     for i in range(nf):
         m, G = sizes[i % len(sizes)]
         d = ea.filter_desc(ea.MODEL_UNICYCLE, m=m, G=G, t=t, topology=topology, resampler=resampler,
                            threshold=threshold, weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER,
-                           seed=int(rng.integers(1, 1 << 30)), carry=carry)
+                           seed=int(rng.integers(1, 1 << 30)), carry=carry, adaptive=adaptive)
         p, ctrl, meas, truth = ea.unicycle_scenario(seed * 1000 + i, 8, 20.0, 8)
         descs.append(d)
         params.append(p)
@@ -42,6 +42,9 @@ CASES = [
     ("ring-alias", dict(sizes=[(64, 4), (256, 3), (1024, 2)], resampler=ea.RESAMPLE_ALIAS)),
     ("all2all", dict(sizes=[(128, 8), (256, 5)], topology=ea.TOPO_ALL2ALL, t=2)),
     ("ring-carry-gate", dict(sizes=[(64, 3), (256, 4), (128, 2)], threshold=0.5, carry=1)),
+    ("torus-alias-adaptive-carry", dict(sizes=[(256, 16), (1024, 4), (64, 64)], topology=ea.TOPO_TORUS, t=2,
+                                        resampler=ea.RESAMPLE_ALIAS, threshold=0.02, adaptive=1, carry=1)),
+    ("all2all-alias", dict(sizes=[(128, 8), (512, 5)], topology=ea.TOPO_ALL2ALL, t=2, resampler=ea.RESAMPLE_ALIAS)),
 ]
 
 

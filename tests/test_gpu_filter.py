@@ -92,6 +92,11 @@ CASES = [
     ("camera-torus-carry-gate", dict(model=ea.MODEL_ROBOTARM5_CAMERA, m=512, G=16, t=4, topology=ea.TOPO_TORUS,
                                      resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, carry=1,
                                      threshold=0.5)),
+    ("unicycle-alias-all2all", dict(model=ea.MODEL_UNICYCLE, m=128, G=32, t=2, topology=ea.TOPO_ALL2ALL,
+                                    resampler=ea.RESAMPLE_ALIAS, weights=ea.WEIGHTS_LOG)),
+    ("dh6-alias-torus-m1024", dict(model=ea.MODEL_DH6, m=1024, G=8, t=4, topology=ea.TOPO_TORUS,
+                                   resampler=ea.RESAMPLE_ALIAS, weights=ea.WEIGHTS_LOG, threshold=0.7, carry=1)),
+    ("robotarm-alias-m32", dict(model=ea.MODEL_ROBOTARM5, m=32, G=16, t=2, resampler=ea.RESAMPLE_ALIAS)),
     ("unicycle-small-m", dict(model=ea.MODEL_UNICYCLE, m=16, G=12, t=2, weights=ea.WEIGHTS_LOG,
                               rng=ea.RNG_MTGP_PER_FILTER)),
 ]
