@@ -204,7 +204,7 @@ __device__ __forceinline__ void esth_sample_pair_regs(float (&s0)[esth_model_tra
         esth_unicycle_propagate(s1, z1, frame, &c1, &n1);
         const int L = (int)frame[8];
         float r0 = 0.0f, b0 = 0.0f, r1 = 0.0f, b1 = 0.0f;
-#pragma unroll 2
+#pragma unroll 4
         for (int j = 0; j < L; ++j) {
             const float* lm = frame + ESTH_UNI_HDR + 5 * j;
             esth_unicycle_landmark(lm, s0[0], s0[1], c0, n0, &r0, &b0);
