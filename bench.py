@@ -28,10 +28,15 @@ sys.path.insert(0, REPO)
 
 METRIC = "particles/s per step @2M unicycle (% HBM roofline); batched p50/p99 step latency"
 HBM_PEAK_GBS = 8000.0
-# sample+weight algorithmic bytes per particle: state read 12 + uniforms read 12
-# + state write 12 + log-weight write 4
-SW_BYTES_PER_PARTICLE = 40
 M, G, T, L = 512, 4096, 4, 32
+S = 3
+# the dominant launch, sample+weight and block sort fused (k_sample_sort):
+# algorithmic bytes per particle: state read 12 + uniforms read 12 + sorted
+# log-weight write 4 + sorted state write 12; per sub-filter: the gate's
+# uniform 4, winner 8, posterior partials 4 (2 + S), compact best t 4 t (1 + S)
+SS_BYTES_PER_PARTICLE = 40
+SS_BYTES_PER_SUB = 4 + 8 + 4 * (2 + S) + 4 * T * (1 + S)
+SS_BYTES_PER_LAUNCH = SS_BYTES_PER_PARTICLE * M * G + SS_BYTES_PER_SUB * G
 STAGES = ["rng_wait", "sample_weight", "sort", "argmax", "exchange_or_topt", "resample"]
 
 
@@ -46,13 +51,16 @@ def stage_dict(us, fused):
     return d
 
 
-def run_plain(ea, eng, resampler, steps, warmup, graph=False, timing=1, every=1, landmarks=L, box=50.0):
+def run_plain(ea, eng, resampler, steps, warmup, graph=False, timing=1, every=1, landmarks=L, box=50.0,
+              fusion=True):
     desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=G, t=T, topology=ea.TOPO_TORUS, resampler=resampler,
                           weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
     params, ctrl, meas, truth = ea.unicycle_scenario(42, landmarks, box, warmup + steps)
     f = ea.Filter(eng, desc, params)
     import numpy as np
     f.init_uniform(np.zeros(3, np.float32))
+    if f.set_fusion(fusion) != fusion:
+        raise RuntimeError("the fused sample+sort launch is expected at m = 512 for the unicycle")
     if graph:
         f.use_graph(True)
     for k in range(warmup):
@@ -290,7 +298,7 @@ def run_served(ea, eng, rates=(1000.0, 5000.0, 20000.0, 50000.0), duration=2.0, 
 
 
 def traffic_from_profiles():
-    p = os.path.join(REPO, "profiles", "sample_weight_unicycle2m_traffic.json")
+    p = os.path.join(REPO, "profiles", "sample_sort_unicycle2m_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as fh:
@@ -303,10 +311,10 @@ VALU_PEAK_WAVE_INST_S = 1024 * 2.4e9 / 2
 
 
 def valu_from_profiles(kernel_us):
-    """VALU issue rate of k_sample_weight: the wave-instructions per launch
-    counted by rocprofv3 (profiles/sample_weight_unicycle2m_valu.md) over the
+    """VALU issue rate of k_sample_sort: the wave-instructions per launch
+    counted by rocprofv3 (profiles/sample_sort_unicycle2m_valu.md) over the
     launch time measured here, against the dual-issue ceiling."""
-    p = os.path.join(REPO, "profiles", "sample_weight_unicycle2m_valu.json")
+    p = os.path.join(REPO, "profiles", "sample_sort_unicycle2m_valu.json")
     if not os.path.exists(p) or not kernel_us:
         return None
     with open(p) as fh:
@@ -325,6 +333,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-alias", action="store_true", help="skip the Vose alias variant")
     ap.add_argument("--graph-only", action="store_true", help="only the HIP-graph replay run (profiling)")
+    ap.add_argument("--main-only", action="store_true",
+                    help="only the timed headline pass and its roofline (profiling: one workload per kernel)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 10k-filter batched workload")
     ap.add_argument("--no-served", action="store_true", help="skip the served request-stream workload")
     ap.add_argument("--no-layouts", action="store_true", help="skip the state-layout comparison")
@@ -351,14 +361,25 @@ def main():
         gsecs, _, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, graph=True)
         print(json.dumps({"graph_ms_per_step": gsecs * 1e3 / args.steps}), flush=True)
         return
-    # timed region: eager steps; HIP events bracket the sample+weight kernel on
-    # every 10th step (each event costs the stream a few microseconds)
+    # timed region: eager steps; HIP events bracket the sample+weight+sort
+    # launch on every 10th step (each event costs the stream a few microseconds)
     secs, stage, err = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, every=10)
     value = N * args.steps / secs
-    # separate short pass with events around every stage (adds idle gaps; breakdown only)
+    if args.main_only:
+        sw_us = stage[1]
+        a = SS_BYTES_PER_LAUNCH / (sw_us * 1e-6) / 1e9
+        print(json.dumps({"ms_per_step": secs * 1e3 / args.steps, "value": value, "kernel_us": sw_us,
+                          "achieved_gbs": a}), flush=True)
+        eng.close()
+        return
+    # separate short pass with events around every stage (adds idle gaps;
+    # breakdown only; per-stage timing runs sample+weight and sort as two launches)
     _, stage_all, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 100), args.warmup, timing=2)
+    # the same step with sample+weight and sort as two launches
+    usecs, ustage, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 300), args.warmup, every=10,
+                                 fusion=False)
     sw_us = stage[1]
-    achieved = SW_BYTES_PER_PARTICLE * N / (sw_us * 1e-6) / 1e9
+    achieved = SS_BYTES_PER_LAUNCH / (sw_us * 1e-6) / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -378,10 +399,12 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "k_sample_weight<unicycle>", "kernel_us": sw_us,
-                     "bytes_per_launch": SW_BYTES_PER_PARTICLE * N,
+                     "kernel": "k_sample_sort<unicycle, SOA, m=512> (sample+weight and block sort)",
+                     "kernel_us": sw_us, "bytes_per_launch": SS_BYTES_PER_LAUNCH,
                      "valu": valu_from_profiles(sw_us)},
         "stages_us": stage_dict(stage_all, fused=True),
+        "two_launches": {"ms_per_step": usecs * 1e3 / min(args.steps, 300), "sample_weight_us": ustage[1],
+                         "note": "sample+weight and sort as separate launches (k_sample_weight, k_sort)"},
         "position_error_last_step": err,
     }
     # the same kernel where the likelihood is light: 4 landmarks at (+-10, +-10)
@@ -389,7 +412,7 @@ def main():
     # is then HBM-bound and its HBM fraction measures the kernel's memory path
     _, stage4, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 200), args.warmup, every=10,
                              landmarks=4, box=0.0)
-    a4 = SW_BYTES_PER_PARTICLE * N / (stage4[1] * 1e-6) / 1e9
+    a4 = SS_BYTES_PER_LAUNCH / (stage4[1] * 1e-6) / 1e9
     out["roofline"]["light_likelihood"] = {"workload": "same 2097152 particles, 4 landmarks", "kernel_us": stage4[1],
                                            "achieved": a4, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                            "frac": a4 / HBM_PEAK_GBS}

@@ -167,6 +167,7 @@ _SIGS = {
     "esth_filter_stream": ([_P, _P], C.c_int),
     "esth_filter_use_graph": ([_P, C.c_int], C.c_int),
     "esth_filter_set_prefetch": ([_P, C.c_int], C.c_int),
+    "esth_filter_set_fusion": ([_P, C.c_int, _P], C.c_int),
     "esth_particle_filter": ([C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p, C.c_uint32, C.c_int,
                               _P, _P], C.c_int),
     "esth_trace_load_robotarm": ([C.c_char_p, C.c_int, _P, _P, _P, _P, _P], C.c_int),
@@ -407,6 +408,13 @@ class Filter:
 
     def set_prefetch(self, on=True):
         check(lib().esth_filter_set_prefetch(self.h, 1 if on else 0))
+
+    def set_fusion(self, on=True):
+        """Sample+weight and sort in one launch where it pays (default);
+        returns whether steps use it."""
+        active = C.c_int(0)
+        check(lib().esth_filter_set_fusion(self.h, 1 if on else 0, C.byref(active)))
+        return bool(active.value)
 
     def use_graph(self, on=True):
         check(lib().esth_filter_use_graph(self.h, 1 if on else 0))

@@ -361,6 +361,128 @@ int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_
     return ESTH_OK;
 }
 
+template <int MODEL, int L, int LOGM>
+static void launch_sample_sort_m(float* state, uint32_t N, uint32_t G, const float* R, const float* frame,
+                                 int log_w, float* w, int carry, float* st_out, float2* winner, uint32_t t,
+                                 float* bt_w, float* bt_s, float* part, const float* gate, float keep_thr,
+                                 int keep_all, hipStream_t s) {
+    hipLaunchKernelGGL((k_sample_sort<MODEL, L, LOGM>), dim3(G), dim3(ESTH_WAVE * esth_ss_waves<LOGM>()), 0, s, state,
+                       N, G, R, frame,
+                       log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, keep_thr, keep_all);
+}
+
+template <int MODEL, int L>
+static int launch_sample_sort_l(int lm, float* state, uint32_t N, uint32_t G, const float* R,
+                                const float* frame, int log_w, float* w, int carry, float* st_out, float2* winner,
+                                uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
+                                float keep_thr, int keep_all, hipStream_t s) {
+#define ESTH_SS_ARGS state, N, G, R, frame, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, keep_thr, \
+                     keep_all, s
+    switch (lm) {
+    case 7: launch_sample_sort_m<MODEL, L, 7>(ESTH_SS_ARGS); break;
+    case 8: launch_sample_sort_m<MODEL, L, 8>(ESTH_SS_ARGS); break;
+    case 9: launch_sample_sort_m<MODEL, L, 9>(ESTH_SS_ARGS); break;
+    case 10: launch_sample_sort_m<MODEL, L, 10>(ESTH_SS_ARGS); break;
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "fused sample+sort needs 128 <= m <= 1024");
+    }
+#undef ESTH_SS_ARGS
+    return ESTH_OK;
+}
+
+template <int MODEL>
+static int launch_sample_sort_t(int layout, int lm, float* state, uint32_t N, uint32_t G, const float* R,
+                                const float* frame, int log_w, float* w, int carry, float* st_out, float2* winner,
+                                uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
+                                float keep_thr, int keep_all, hipStream_t s) {
+#define ESTH_SS_ARGS lm, state, N, G, R, frame, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, \
+                     keep_thr, keep_all, s
+    if (layout == ESTH_LAYOUT_AOS) return launch_sample_sort_l<MODEL, ESTH_LAYOUT_AOS>(ESTH_SS_ARGS);
+    if (layout == ESTH_LAYOUT_PACKED) return launch_sample_sort_l<MODEL, ESTH_LAYOUT_PACKED>(ESTH_SS_ARGS);
+    return launch_sample_sort_l<MODEL, ESTH_LAYOUT_SOA>(ESTH_SS_ARGS);
+#undef ESTH_SS_ARGS
+}
+
+bool esth_sample_sort_fits(uint32_t m) { return m >= 128 && m <= 1024 && (m & (m - 1)) == 0; }
+
+/* resident waves per SIMD of a kernel launched with `threads`-thread
+ * work-groups: 512 VGPRs per lane slot (granule 8), at most 8 waves, and
+ * 160 KiB of LDS per CU shared by its work-groups */
+static int waves_per_simd(const void* fn, int threads) {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, fn) != hipSuccess) return 0;
+    const int regs = a.numRegs > 0 ? (a.numRegs + 7) / 8 * 8 : 512;
+    int occ = 512 / regs;
+    if (occ > 8) occ = 8;
+    if (a.sharedSizeBytes > 0) {
+        const int by_lds = (int)((160u * 1024u) / a.sharedSizeBytes) * ((threads + ESTH_WAVE - 1) / ESTH_WAVE) / 4;
+        if (by_lds < occ) occ = by_lds;
+    }
+    return occ;
+}
+
+template <int MODEL, int L>
+static bool sample_sort_pays_l(int lm) {
+    const void* fn = nullptr;
+    switch (lm) {
+    case 7: fn = (const void*)&k_sample_sort<MODEL, L, 7>; break;
+    case 8: fn = (const void*)&k_sample_sort<MODEL, L, 8>; break;
+    case 9: fn = (const void*)&k_sample_sort<MODEL, L, 9>; break;
+    case 10: fn = (const void*)&k_sample_sort<MODEL, L, 10>; break;
+    default: return false;
+    }
+    const int ss_waves[11] = {0, 0, 0, 0, 0, 0, 0, esth_ss_waves<7>(), esth_ss_waves<8>(), esth_ss_waves<9>(),
+                              esth_ss_waves<10>()};
+    const int fused = waves_per_simd(fn, ESTH_WAVE * ss_waves[lm]);
+    return fused > 0 && fused >= waves_per_simd((const void*)&k_sample_weight<MODEL, L>, 256);
+}
+
+template <int MODEL>
+static bool sample_sort_pays_t(int layout, int lm) {
+    if (layout == ESTH_LAYOUT_AOS) return sample_sort_pays_l<MODEL, ESTH_LAYOUT_AOS>(lm);
+    if (layout == ESTH_LAYOUT_PACKED) return sample_sort_pays_l<MODEL, ESTH_LAYOUT_PACKED>(lm);
+    return sample_sort_pays_l<MODEL, ESTH_LAYOUT_SOA>(lm);
+}
+
+/* the fused launch is taken where it keeps as many waves resident as the
+ * sampling alone: the sort's registers (E = m/64 keys, indices and weights
+ * per lane) would otherwise cost the VALU-bound sampling its occupancy */
+bool esth_sample_sort_pays(int model, int layout, uint32_t m) {
+    if (!esth_sample_sort_fits(m)) return false;
+    const int lm = log2u(m);
+    switch (model) {
+    case ESTH_MODEL_ROBOTARM5: return sample_sort_pays_t<ESTH_MODEL_ROBOTARM5>(layout, lm);
+    case ESTH_MODEL_UNICYCLE: return sample_sort_pays_t<ESTH_MODEL_UNICYCLE>(layout, lm);
+    case ESTH_MODEL_DH6: return sample_sort_pays_t<ESTH_MODEL_DH6>(layout, lm);
+    case ESTH_MODEL_UNICYCLE10: return sample_sort_pays_t<ESTH_MODEL_UNICYCLE10>(layout, lm);
+    case ESTH_MODEL_ROBOTARM5_CAMERA: return sample_sort_pays_t<ESTH_MODEL_ROBOTARM5_CAMERA>(layout, lm);
+    default: return false;
+    }
+}
+
+int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uint32_t G, const float* R,
+                            const float* frame, int log_w, float* w, int carry, int layout, float* st_out,
+                            float2* winner, uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
+                            float keep_thr, int keep_all, hipStream_t s) {
+    if (!esth_sample_sort_fits(m) || (uint64_t)m * G != N)
+        return esth_fail(ESTH_ERR_UNSUPPORTED, "fused sample+sort needs 128 <= m <= 1024 and N = m G");
+    const int lm = log2u(m);
+#define ESTH_SS_ARGS layout, lm, state, N, G, R, frame, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, \
+                     keep_thr, keep_all, s
+    int rc;
+    switch (model) {
+    case ESTH_MODEL_ROBOTARM5: rc = launch_sample_sort_t<ESTH_MODEL_ROBOTARM5>(ESTH_SS_ARGS); break;
+    case ESTH_MODEL_UNICYCLE: rc = launch_sample_sort_t<ESTH_MODEL_UNICYCLE>(ESTH_SS_ARGS); break;
+    case ESTH_MODEL_DH6: rc = launch_sample_sort_t<ESTH_MODEL_DH6>(ESTH_SS_ARGS); break;
+    case ESTH_MODEL_UNICYCLE10: rc = launch_sample_sort_t<ESTH_MODEL_UNICYCLE10>(ESTH_SS_ARGS); break;
+    case ESTH_MODEL_ROBOTARM5_CAMERA: rc = launch_sample_sort_t<ESTH_MODEL_ROBOTARM5_CAMERA>(ESTH_SS_ARGS); break;
+    default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
+    }
+#undef ESTH_SS_ARGS
+    if (rc != ESTH_OK) return rc;
+    ESTH_HIP(hipGetLastError());
+    return ESTH_OK;
+}
+
 int esth_launch_argmax(int S, const float2* winner, uint32_t G, int log_w, const float* part, const float* sorted,
                        uint32_t N, uint32_t m, float2* result, double* est, hipStream_t s) {
     switch (S) {

@@ -37,6 +37,7 @@ struct esth_filter {
     uint32_t* status[2] = {nullptr, nullptr};
     bool rng_ready = false;             /* R[steps & 1] already holds the next step's words */
     bool prefetch = true;
+    bool fuse_sample_sort = false;      /* sample+weight and sort in one launch (k_sample_sort) */
     hipEvent_t ev_rng_done[2], ev_consumed[2], ev_fork;
     float2 *winner = nullptr, *result = nullptr;
     double* est = nullptr;
@@ -109,6 +110,7 @@ extern "C" int esth_filter_create(esth_engine* e, const esth_filter_desc* d, con
     f->G = d->num_blocks;
     f->t = (d->topology == ESTH_TOPO_NONE) ? 0 : d->num_transfer;
     f->N = (size_t)f->m * f->G;
+    f->fuse_sample_sort = esth_sample_sort_pays(d->model, d->state_layout, f->m);
     memset(f->params, 0, sizeof(f->params));
     if (need) memcpy(f->params, params, need);
     f->needed = esth_rng_needed(V, d->resampler, d->adaptive, f->m, f->G);
@@ -281,13 +283,31 @@ static int enqueue_step(esth_filter* f, const float* d_frame, hipEvent_t* ev) {
     const int p = (int)(f->steps & 1);
     float* R = f->R[p];
     if (timing) ESTH_HIP(hipEventRecord(ev[1], s));
-    /* sampling_importance with Box-Muller fused (pf_generic.c:521-533) */
-    ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, R, d_frame, log_w, f->w, d.carry_weights,
-                                       d.state_layout, s));
-    if (timing) ESTH_HIP(hipEventRecord(ev[2], s));
-    /* block_sort (pf_generic.c:572-585) */
-    ESTH_TRY(esth_launch_sort(f->S, f->m, f->w, f->state, f->tmp, N, f->G, f->winner, nullptr, f->t, f->bt_w,
-                              f->bt_s, f->part, log_w, d.state_layout, s));
+    if (!all && f->fuse_sample_sort) {
+        /* sampling_importance (pf_generic.c:521-533) and block_sort
+         * (pf_generic.c:572-585) in one launch: the propagated state stays in
+         * LDS between them.  Per-stage timing keeps the two launches below
+         * (their results are bit-identical). */
+        /* a sub-filter the resampling skips keeps its propagated state unless
+         * weights are carried (then it takes the sorted one): the scan and
+         * the plain alias gate on the uniform rnd[N + g] (pf_generic.cl:403),
+         * the adaptive alias on the weights themselves */
+        const bool carry = d.carry_weights != 0;
+        const int keep_all = !carry && d.resampler == ESTH_RESAMPLE_ALIAS && d.adaptive;
+        const float keep_thr = carry ? INFINITY : d.resampling_threshold;
+        ESTH_TRY(esth_launch_sample_sort(d.model, f->m, f->state, N, f->G, R, d_frame, log_w, f->w, d.carry_weights,
+                                         d.state_layout, f->tmp, f->winner, f->t, f->bt_w, f->bt_s, f->part,
+                                         R + (size_t)N * f->V + N, keep_thr, keep_all, s));
+        if (timing) ESTH_HIP(hipEventRecord(ev[2], s));
+    } else {
+        /* sampling_importance with Box-Muller fused (pf_generic.c:521-533) */
+        ESTH_TRY(esth_launch_sample_weight(d.model, f->state, N, R, d_frame, log_w, f->w, d.carry_weights,
+                                           d.state_layout, s));
+        if (timing) ESTH_HIP(hipEventRecord(ev[2], s));
+        /* block_sort (pf_generic.c:572-585) */
+        ESTH_TRY(esth_launch_sort(f->S, f->m, f->w, f->state, f->tmp, N, f->G, f->winner, nullptr, f->t, f->bt_w,
+                                  f->bt_s, f->part, log_w, d.state_layout, s));
+    }
     if (all) ESTH_HIP(hipEventRecord(ev[3], s));
     /* reduce (pf_generic.c:604-615), result kept on the device.  With a scan
      * resampler, or the one-wave alias (m >= 64), it rides in the resampling
@@ -406,6 +426,17 @@ extern "C" int esth_filter_use_graph(esth_filter* f, int enable) {
 }
 
 /* Overlap the next step's MTGP words with the current step (default on). */
+extern "C" int esth_filter_set_fusion(esth_filter* f, int enable, int* active) {
+    if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
+    ESTH_HIP(hipSetDevice(f->e->device));
+    ESTH_HIP(hipStreamSynchronize(f->stream));
+    for (int i = 0; i < 2; ++i)
+        if (f->gexec[i]) { hipGraphExecDestroy(f->gexec[i]); f->gexec[i] = nullptr; }
+    f->fuse_sample_sort = enable != 0 && esth_sample_sort_pays(f->d.model, f->d.state_layout, f->m);
+    if (active) *active = f->fuse_sample_sort ? 1 : 0;
+    return ESTH_OK;
+}
+
 extern "C" int esth_filter_set_prefetch(esth_filter* f, int enable) {
     if (!f) return esth_fail(ESTH_ERR_INVALID, "null filter");
     ESTH_HIP(hipStreamSynchronize(f->stream));

@@ -55,6 +55,16 @@ int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* 
 int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_out, uint32_t N,
                      uint32_t G, float2* winner, int* perm, uint32_t t, float* bt_w, float* bt_s,
                      float* part, int log_w, int layout, hipStream_t s);
+/* sample+weight and block sort in one launch (one work-group per sub-filter,
+ * the unsorted state kept in LDS); for 128 <= m <= 1024.  The propagated
+ * state is stored to `state` only for sub-filters g with keep_all or
+ * gate[g] - 1 > keep_thr (those the resampling leaves untouched). */
+bool esth_sample_sort_fits(uint32_t m);
+bool esth_sample_sort_pays(int model, int layout, uint32_t m);   /* fits, and costs the sampling no occupancy */
+int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uint32_t G, const float* R,
+                            const float* frame, int log_w, float* w, int carry, int layout, float* st_out,
+                            float2* winner, uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
+                            float keep_thr, int keep_all, hipStream_t s);
 int esth_launch_argmax(int S, const float2* winner, uint32_t G, int log_w, const float* part,
                        const float* sorted, uint32_t N, uint32_t m, float2* result, double* est,
                        hipStream_t s);

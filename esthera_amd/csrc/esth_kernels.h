@@ -47,16 +47,37 @@ __device__ __forceinline__ double esth_shfl_xor_d(double v, int mask) {
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+/* the halves of a 32- or 16-lane xor exchange (MASK 32 or 16), from one
+ * v_permlane32_swap / v_permlane16_swap of v with itself: lo[i] is the value
+ * of the lane of the pair {i, i ^ MASK} whose MASK bit is 0, hi[i] that of the
+ * lane whose bit is 1, in both lanes of the pair.  No LDS access. */
+template <int MASK>
+__device__ __forceinline__ void esth_xor_halves(int v, int& lo, int& hi) {
+    static_assert(MASK == 16 || MASK == 32, "permlane swaps pair lanes 16 or 32 apart");
+    if constexpr (MASK == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, true, false);
+        lo = (int)r[0];
+        hi = (int)r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, true, false);
+        lo = (int)r[0];
+        hi = (int)r[1];
+    }
+}
+
 /* xor-lane exchange with a compile-time mask: DPP quad_perm (1, 2), DPP
- * row_ror:8 (8), ds_swizzle SWAP mode (4, 16), ds_bpermute (32). */
+ * row_ror:8 (8), ds_swizzle SWAP mode (4), a permlane swap (16, 32). */
 template <int MASK>
 __device__ __forceinline__ int esth_xor_lane(int v) {
     if constexpr (MASK == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
     else if constexpr (MASK == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
     else if constexpr (MASK == 4) return __builtin_amdgcn_ds_swizzle(v, 0x101F);
     else if constexpr (MASK == 8) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);
-    else if constexpr (MASK == 16) return __builtin_amdgcn_ds_swizzle(v, 0x401F);
-    else return esth_shfl_xor_i(v, MASK);
+    else if constexpr (MASK == 16 || MASK == 32) {
+        int lo, hi;
+        esth_xor_halves<MASK>(v, lo, hi);
+        return ((int)__lane_id() & MASK) ? lo : hi;
+    } else return esth_shfl_xor_i(v, MASK);
 }
 template <int MASK>
 __device__ __forceinline__ float esth_xor_lane_f(float v) {
@@ -437,6 +458,23 @@ __device__ __forceinline__ void esth_bitonic_stage(float (&key)[esth_tile<LOGM>:
                 idx[b] = sw ? ia : ib;
             }
         }
+    } else if constexpr (STRIDE / E == 16 || STRIDE / E == 32) {
+        /* the pair's lower and upper keys straight from the permlane swap; the
+         * lower lane keeps the upper key iff it swaps, the upper lane iff not */
+        constexpr int MASK = STRIDE / E;
+        const bool lower = (laneg & MASK) == 0;
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            int kl_i, ku_i, il, iu;
+            esth_xor_halves<MASK>(__builtin_bit_cast(int, key[r]), kl_i, ku_i);
+            esth_xor_halves<MASK>(idx[r], il, iu);
+            const float kl = __builtin_bit_cast(float, kl_i), ku = __builtin_bit_cast(float, ku_i);
+            const bool d = (SIZE == M) ? false : (((le + r) & SIZE) != 0);
+            const bool sw = (kl > ku) == d;
+            const bool take_u = sw == lower;
+            key[r] = take_u ? ku : kl;
+            idx[r] = take_u ? iu : il;
+        }
     } else {
         constexpr int MASK = STRIDE / E;
         const bool lower = (laneg & MASK) == 0;
@@ -640,29 +678,37 @@ __device__ __forceinline__ void esth_store_plane_run(float* __restrict__ st, int
     for (int r = 0; r < E; ++r) st[b + (size_t)r * stride] = v[r];
 }
 
-template <int LOGM, int S, bool HOIST = false>
+/* FROM_LDS: the sub-filter's weights and planes are already in LDS at `src`
+ * (src[0..M) the weights, src[(1 + c) M ..) plane c, one sub-filter per
+ * wave): the keys are read from there and the gather permutes there, with
+ * no global reads */
+template <int LOGM, int S, bool HOIST = false, bool FROM_LDS = false>
 __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float* __restrict__ st_in,
                                               float* __restrict__ st_out, uint32_t N, uint32_t G, uint32_t g,
                                               bool valid, float2* __restrict__ winner, int* __restrict__ perm,
                                               uint32_t t, float* __restrict__ bt_w, float* __restrict__ bt_s,
                                               float* __restrict__ part, int log_w, float* stage_w,
-                                              int layout = ESTH_LAYOUT_SOA) {
+                                              int layout = ESTH_LAYOUT_SOA, const float* src = nullptr) {
     using TL = esth_tile<LOGM>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS;
+    static_assert(!FROM_LDS || LPS == ESTH_WAVE, "an LDS-resident sub-filter fills one wave");
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
     const int le = (lane % LPS) * E;
-    float* stage = stage_w + (lane - lane % LPS) * E;   /* this sub-filter's M floats */
+    float* stage = FROM_LDS ? nullptr : stage_w + (lane - lane % LPS) * E;   /* this sub-filter's M floats */
     const size_t base = (size_t)g * M;
     float key[E];
     int idx[E];
-    esth_load_run<E>(w + base + le, key);
+    if constexpr (FROM_LDS)
+        esth_load_run<E>(src + le, key);
+    else
+        esth_load_run<E>(w + base + le, key);
     /* the particles' planes do not depend on the order: with HOIST, when
      * they fit in registers, their loads are issued here and land while the
      * network runs (a gain for many small sub-filters; on a few large ones
      * the early burst delays the other waves' keys, so there they follow the
      * network) */
-    constexpr int PB = (S * E <= 64) ? S : 1;
-    constexpr bool EARLY = HOIST && PB == S;
+    constexpr int PB = FROM_LDS ? 1 : (S * E <= 64) ? S : 1;
+    constexpr bool EARLY = HOIST && !FROM_LDS && PB == S;
     float in[PB][E];
     if constexpr (EARLY) {
 #pragma unroll
@@ -695,14 +741,19 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
      * loaded early, the planes' loads are issued together, PB at a time */
 #pragma unroll
     for (int c = 0; c < S; ++c) {
-        if (!EARLY && c % PB == 0) {
-#pragma unroll
-            for (int j = 0; j < PB; ++j) esth_load_plane_run<E, S>(st_in, layout, N, c + j, base + le, in[j]);
-        }
         float* dst = st_out + (size_t)c * N + base;
         float acc = 0.0f;
         float v[E];
-        esth_permute_lds<M, E>(in[c % PB], le, stage, idx, v);
+        if constexpr (FROM_LDS) {
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = src[(1 + c) * M + idx[r]];
+        } else {
+            if (!EARLY && c % PB == 0) {
+#pragma unroll
+                for (int j = 0; j < PB; ++j) esth_load_plane_run<E, S>(st_in, layout, N, c + j, base + le, in[j]);
+            }
+            esth_permute_lds<M, E>(in[c % PB], le, stage, idx, v);
+        }
 #pragma unroll
         for (int r = 0; r < E; ++r) acc = esth_fma(wt[r], v[r], acc);
         if (valid) esth_store_run<E>(dst + le, v);
@@ -747,6 +798,73 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
     __shared__ float stage[4][ESTH_WAVE * esth_tile<LOGM>::E];
     esth_sort_sub<LOGM, S>(w, st_in, st_out, N, G, g, valid, winner, perm, t, bt_w, bt_s, part, log_w,
                            stage[threadIdx.x / ESTH_WAVE], layout);
+}
+
+/* sample+weight and block sort of one sub-filter in one work-group of M/2
+ * threads: each thread samples and weights its pair (as k_sample_weight,
+ * same uniforms, same arithmetic) into the work-group's LDS, then wave 0
+ * sorts the sub-filter there (esth_sort_sub, FROM_LDS) and writes what
+ * k_sort writes.  The propagated unsorted state is read only by the sort,
+ * except in a sub-filter the resampling will skip without carried weights,
+ * which keeps it (pf_generic.cl:403): it is stored only for those, known
+ * here from the gate's uniform (gate[g] - 1 > keep_thr) or, for the
+ * data-dependent alias gate, always (keep_all).  The sort's wave runs beside
+ * other work-groups' sampling instead of after all of them. */
+/* waves per work-group of k_sample_sort (W = min(M / 128, 4)): each thread
+ * samples M / (128 W) pairs.  Four waves beat one or two at m = 512 (90 µs
+ * against 93 and 95 for the 2M-particle unicycle): more sampling waves
+ * resident around each sorting one. */
+template <int LOGM>
+__host__ __device__ constexpr int esth_ss_waves() {
+    return (1 << LOGM) / 128 < 4 ? (1 << LOGM) / 128 : 4;
+}
+
+template <int MODEL, int L, int LOGM>
+__global__ __launch_bounds__(ESTH_WAVE * esth_ss_waves<LOGM>()) void k_sample_sort(float* __restrict__ state, uint32_t N,
+                                                                 uint32_t G, const float* __restrict__ R,
+                                                                 const float* __restrict__ frame, int log_w,
+                                                                 float* __restrict__ w, int carry,
+                                                                 float* __restrict__ st_out,
+                                                                 float2* __restrict__ winner, uint32_t t,
+                                                                 float* __restrict__ bt_w,
+                                                                 float* __restrict__ bt_s,
+                                                                 float* __restrict__ part,
+                                                                 const float* __restrict__ gate, float keep_thr,
+                                                                 int keep_all) {
+    using T = esth_model_traits<MODEL>;
+    constexpr int M = 1 << LOGM, S = T::S;
+    static_assert(LOGM >= 7 && LOGM <= 10, "one work-group per sub-filter of 128..1024 particles");
+    __shared__ __align__(16) float sub[(1 + S) * M];   /* weights, then plane c at (1 + c) M */
+    constexpr int NT = ESTH_WAVE * esth_ss_waves<LOGM>();
+    const uint32_t g = blockIdx.x;
+    const bool keep = keep_all || (gate[g] - 1.0f) > keep_thr;
+#pragma unroll 1
+    for (int q = 2 * (int)threadIdx.x; q < M; q += 2 * NT) {
+        const uint32_t p = g * (uint32_t)M + (uint32_t)q;
+        float s0[S], s1[S], u0[T::V], u1[T::V];
+        esth_load_pair<S, L>(state, N, p, s0, s1);
+#pragma unroll
+        for (int k = 0; k < T::V; ++k) {
+            const float2 u = *reinterpret_cast<const float2*>(R + (size_t)k * N + p);
+            u0[k] = u.x;
+            u1[k] = u.y;
+        }
+        float lw0, lw1;
+        esth_sample_pair_regs<MODEL>(s0, s1, u0, u1, frame, lw0, lw1);
+        if (keep) esth_store_pair<S, L>(state, N, p, s0, s1);
+        float2 wo = log_w ? make_float2(lw0, lw1) : make_float2(esth_expf(lw0), esth_expf(lw1));
+        if (carry) {
+            const float2 wp = *reinterpret_cast<const float2*>(w + p);
+            wo = log_w ? make_float2(wp.x + wo.x, wp.y + wo.y) : make_float2(wp.x * wo.x, wp.y * wo.y);
+        }
+        *reinterpret_cast<float2*>(sub + q) = wo;
+#pragma unroll
+        for (int c = 0; c < S; ++c) *reinterpret_cast<float2*>(sub + (1 + c) * M + q) = make_float2(s0[c], s1[c]);
+    }
+    __syncthreads();
+    if (threadIdx.x >= ESTH_WAVE) return;
+    esth_sort_sub<LOGM, S, false, true>(w, nullptr, st_out, N, G, g, true, winner, nullptr, t, bt_w, bt_s, part,
+                                        log_w, nullptr, ESTH_LAYOUT_SOA, sub);
 }
 
 /* ============================================================= exchange */

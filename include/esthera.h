@@ -253,7 +253,8 @@ int esth_filter_read_particles(esth_filter* f, float* state_soa, float* weights)
 /* Stage timing with hipEvents recorded on the filter's stream.
  * set_timing(f, level) starts a new measurement (eager steps only, up to 4096
  * steps): level 1 brackets only the sample+weight kernel (two events per
- * step), level 2 every stage.  stage_times returns the mean device time per
+ * step; with the fused sample+sort launch, that launch), level 2 every
+ * stage.  stage_times returns the mean device time per
  * step in microseconds of [rng wait, sample_weight, sort, argmax,
  * exchange/top-t, resample]; stages not timed are NaN. */
 int esth_filter_set_timing(esth_filter* f, int on);
@@ -274,6 +275,14 @@ int esth_filter_stream(esth_filter* f, void** hip_stream);
 /* Generate the next step's MTGP32 words on a second stream while the current
  * step runs (default on; results are identical either way). */
 int esth_filter_set_prefetch(esth_filter* f, int enable);
+/* Sample+weight and the block sort as one launch per sub-filter (the
+ * propagated state stays in LDS between them; no counterpart in the
+ * reference, whose two kernels are pf_generic.c:521-533 and :572-585).  On by
+ * default for 128 <= m <= 1024 where the fused kernel keeps as many waves
+ * resident as the sampling alone; enable = 0 keeps two launches.  *active
+ * (may be NULL) receives whether steps use the fused launch.  Results are
+ * bit-identical either way.  Timing level 2 always runs the two launches. */
+int esth_filter_set_fusion(esth_filter* f, int enable, int* active);
 /* Capture one step into a HIP graph and replay it for later steps with the
  * same control/measurement layout (the frame buffer is updated per call). */
 int esth_filter_use_graph(esth_filter* f, int enable);

@@ -248,6 +248,75 @@ def test_prefetch_on_off_identical(graph, engine):
     assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
 
 
+FUSION_CASES = [
+    ("unicycle-512-soa-torus-log", ea.MODEL_UNICYCLE, 512, ea.LAYOUT_SOA,
+     dict(G=16, t=4, topology=ea.TOPO_TORUS, resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG)),
+    ("unicycle-128-aos-ring-gate", ea.MODEL_UNICYCLE, 128, ea.LAYOUT_AOS, dict(G=24, t=1, threshold=0.5)),
+    ("unicycle-256-packed-carry", ea.MODEL_UNICYCLE, 256, ea.LAYOUT_PACKED,
+     dict(G=16, t=2, topology=ea.TOPO_TORUS, threshold=0.5, carry=1, weights=ea.WEIGHTS_LOG)),
+    ("unicycle-1024-soa-alias", ea.MODEL_UNICYCLE, 1024, ea.LAYOUT_SOA,
+     dict(G=8, t=4, topology=ea.TOPO_TORUS, resampler=ea.RESAMPLE_ALIAS, weights=ea.WEIGHTS_LOG)),
+    ("robotarm-128-soa-linear", ea.MODEL_ROBOTARM5, 128, ea.LAYOUT_SOA, dict(G=16, t=1)),
+    ("robotarm-256-packed-alias", ea.MODEL_ROBOTARM5, 256, ea.LAYOUT_PACKED,
+     dict(G=8, t=1, resampler=ea.RESAMPLE_ALIAS, adaptive=1, threshold=0.01)),
+    ("robotarm-512-soa-log", ea.MODEL_ROBOTARM5, 512, ea.LAYOUT_SOA, dict(G=8, t=2, weights=ea.WEIGHTS_LOG)),
+    ("dh6-256-soa-torus", ea.MODEL_DH6, 256, ea.LAYOUT_SOA, dict(G=8, t=1, topology=ea.TOPO_TORUS,
+                                                                 weights=ea.WEIGHTS_LOG)),
+    ("unicycle10-256-soa-all2all", ea.MODEL_UNICYCLE10, 256, ea.LAYOUT_SOA,
+     dict(G=16, t=2, topology=ea.TOPO_ALL2ALL, weights=ea.WEIGHTS_LOG)),
+    ("camera-256-aos-log", ea.MODEL_ROBOTARM5_CAMERA, 256, ea.LAYOUT_AOS, dict(G=8, t=1, weights=ea.WEIGHTS_LOG)),
+]
+
+
+@pytest.mark.parametrize("name,model,m,layout,kw", FUSION_CASES, ids=[c[0] for c in FUSION_CASES])
+def test_fused_sample_sort_identical(name, model, m, layout, kw, engine):
+    """Sample+weight and sort in one launch (state kept in LDS) against the
+    two launches, step by step: particles, sorted weights and estimates bit
+    for bit.  Where the fused launch would cost occupancy it is not taken and
+    both filters run the same launches."""
+    desc = ea.filter_desc(model, m=m, layout=layout, seed=77, **kw)
+    params, ctrl, meas, init, truth, dt = make_case(model, 8)
+    a, b = ea.Filter(engine, desc, params), ea.Filter(engine, desc, params)
+    fused = a.set_fusion(True)
+    assert b.set_fusion(False) is False
+    if model == ea.MODEL_UNICYCLE and m <= 512:
+        assert fused, "the fused launch is expected for the unicycle at m <= 512"
+    print(name, "fused" if fused else "not fused (occupancy)", flush=True)
+    for f in (a, b):
+        f.init_uniform(init)
+    for k in range(8):
+        d = float(dt[k]) if np.ndim(dt) else dt
+        a.step(ctrl[k], meas[k], d)
+        b.step(ctrl[k], meas[k], d)
+        sa, wa = a.particles()
+        sb, wb = b.particles()
+        assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32)), f"state differs at step {k}"
+        assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32)), f"weights differ at step {k}"
+        ea_, eb = a.estimate(), b.estimate()
+        assert ea_.best_weight == eb.best_weight and ea_.best_block == eb.best_block
+        assert np.array_equal(np.array(ea_.mean), np.array(eb.mean))
+
+
+def test_fused_sample_sort_graph(engine):
+    """The fused launch inside a replayed graph equals eager two-launch steps."""
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=512, G=32, t=4, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG,
+                          resampler=ea.RESAMPLE_SYSTEMATIC, seed=9)
+    params, ctrl, meas, init, truth, dt = make_case(ea.MODEL_UNICYCLE, 10)
+    a, b = ea.Filter(engine, desc, params), ea.Filter(engine, desc, params)
+    assert a.set_fusion(False) is False and b.set_fusion(True) is True
+    for f in (a, b):
+        f.init_uniform(init)
+    b.use_graph(True)
+    for k in range(10):
+        a.step(ctrl[k], meas[k], dt)
+        b.step(ctrl[k], meas[k], dt)
+        if k % 3 == 2 or k == 9:
+            sa, wa = a.particles()
+            sb, wb = b.particles()
+            assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+            assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+
+
 def test_small_unicycle_500_steps_rmse(engine):
     """The small unicycle workload end to end: 16 x 256 particles, 4 landmarks
     at (+-10, +-10), 4 x 4 torus with one particle per neighbour, systematic
