@@ -89,7 +89,50 @@ struct esth_bclass {
     int logm = 0;
     uint32_t n = 0;
     uint2* d_sub = nullptr;
+    bool fused = false;   /* sample+weight and sort in one launch (kb_sample_sort) */
 };
+
+/* kb_sample_sort<MODEL, LOGM, S> for size class lm keeps as many waves
+ * resident per SIMD as kb_sample_weight<MODEL> */
+template <int MODEL, int S>
+static bool kb_fuse_pays_m(int lm) {
+    if constexpr (esth_model_traits<MODEL>::S != S) {
+        return false;
+    } else {
+        const void* fn = nullptr;
+        switch (lm) {
+        case 7: fn = (const void*)&kb_sample_sort<MODEL, 7, S>; break;
+        case 8: fn = (const void*)&kb_sample_sort<MODEL, 8, S>; break;
+        case 9: fn = (const void*)&kb_sample_sort<MODEL, 9, S>; break;
+        case 10: fn = (const void*)&kb_sample_sort<MODEL, 10, S>; break;
+        default: return false;
+        }
+        const int waves = lm >= 9 ? 4 : (1 << lm) / 128;
+        const int fused = esth_waves_per_simd(fn, ESTH_WAVE * waves);
+        return fused > 0 && fused >= esth_waves_per_simd((const void*)&kb_sample_weight<MODEL>, 256);
+    }
+}
+
+template <int S>
+static bool kb_fuse_pays_s(int model, int lm) {
+    switch (model) {
+    case ESTH_MODEL_ROBOTARM5: return kb_fuse_pays_m<ESTH_MODEL_ROBOTARM5, S>(lm);
+    case ESTH_MODEL_UNICYCLE: return kb_fuse_pays_m<ESTH_MODEL_UNICYCLE, S>(lm);
+    case ESTH_MODEL_UNICYCLE10: return kb_fuse_pays_m<ESTH_MODEL_UNICYCLE10, S>(lm);
+    case ESTH_MODEL_ROBOTARM5_CAMERA: return kb_fuse_pays_m<ESTH_MODEL_ROBOTARM5_CAMERA, S>(lm);
+    case ESTH_MODEL_DH6: return kb_fuse_pays_m<ESTH_MODEL_DH6, S>(lm);
+    default: return false;
+    }
+}
+
+static bool kb_fuse_pays(int model, int S, int lm) {
+    switch (S) {
+    case ESTH_UNI_S: return kb_fuse_pays_s<ESTH_UNI_S>(model, lm);
+    case ESTH_RA5_S: return kb_fuse_pays_s<ESTH_RA5_S>(model, lm);
+    case ESTH_DH6_S: return kb_fuse_pays_s<ESTH_DH6_S>(model, lm);
+    default: return false;
+    }
+}
 
 struct esth_batch {
     esth_engine* e = nullptr;
@@ -267,8 +310,14 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
     BA(b->frames[1], sizeof(float) * b->frame_stride * n);
     BA(b->d_F, sizeof(esth_bfilter) * n);
     BA(b->d_sub_all, sizeof(uint2) * Gtot);
-    for (int i = 0; i < n; ++i) b->nchunks += (b->hf[i].N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
-    BA(b->d_chunks, sizeof(uint2) * b->nchunks);
+    /* size classes whose sub-filters sample and sort in one launch; the
+     * other filters' particles go through kb_sample_weight's chunks */
+    bool fuse[11] = {};
+    for (int lm = 7; lm <= 10; ++lm) fuse[lm] = kb_fuse_pays(b->d.model, S, lm);
+    auto class_of = [](uint32_t m) { int k = 0; while ((1u << k) < m) ++k; return k; };
+    for (int i = 0; i < n; ++i)
+        if (!fuse[class_of(b->hf[i].m)]) b->nchunks += (b->hf[i].N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
+    if (b->nchunks) BA(b->d_chunks, sizeof(uint2) * b->nchunks);
 #undef BA
     if (rc != ESTH_OK) { batch_free(b); return rc; }
     /* filter table, sub-filter tables, RNG streams */
@@ -309,8 +358,10 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
     std::vector<uint2> chunks;
     chunks.reserve(b->nchunks);
     for (int i = 0; i < n; ++i)
-        for (uint32_t q = 0; q < b->hf[i].N / 2; q += ESTH_WAVE) chunks.push_back(make_uint2((uint32_t)i, q));
-    if (err == hipSuccess) err = hipMemcpy(b->d_chunks, chunks.data(), sizeof(uint2) * chunks.size(), hipMemcpyHostToDevice);
+        if (!fuse[class_of(b->hf[i].m)])
+            for (uint32_t q = 0; q < b->hf[i].N / 2; q += ESTH_WAVE) chunks.push_back(make_uint2((uint32_t)i, q));
+    if (err == hipSuccess && !chunks.empty())
+        err = hipMemcpy(b->d_chunks, chunks.data(), sizeof(uint2) * chunks.size(), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(b->streams, hs.data(), sizeof(esth_mtgp_stream) * hs.size(), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(b->status[0], st.data(), sizeof(uint32_t) * st.size(), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(b->w, 0, sizeof(float) * b->Ntot);
@@ -321,6 +372,7 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
         esth_bclass c;
         c.logm = lm;
         c.n = (uint32_t)by_class[lm].size();
+        c.fused = fuse[lm];
         rc = balloc(b, sizeof(uint2) * c.n, (void**)&c.d_sub);
         if (rc == ESTH_OK && hipMemcpy(c.d_sub, by_class[lm].data(), sizeof(uint2) * c.n, hipMemcpyHostToDevice) != hipSuccess)
             rc = esth_fail(ESTH_ERR_DEVICE, "batch class upload");
@@ -382,6 +434,43 @@ static void launch_kb_resample(esth_batch* b, const esth_bclass& c, const float*
                        log_w, b->d.topology, argmax_blocks);
 }
 
+/* sample+weight and sort of a fused size class (m >= 128) */
+template <int MODEL, int S>
+static int launch_kb_sample_sort_m(esth_batch* b, const esth_bclass& c, const float* R, const float* frames,
+                                   int log_w) {
+    if constexpr (esth_model_traits<MODEL>::S != S) {
+        return esth_fail(ESTH_ERR_INVALID, "state dimension");
+    } else {
+        const float thr = b->d.resampling_threshold;
+        const int keep_all = b->d.resampler == ESTH_RESAMPLE_ALIAS && b->d.adaptive;
+#define ESTH_KBSS(LM)                                                                                              \
+    hipLaunchKernelGGL((kb_sample_sort<MODEL, LM, S>), dim3(c.n), dim3(ESTH_WAVE * esth_ss_waves<LM>()), 0, b->stream, \
+                       b->d_F, c.d_sub, R, frames, log_w, thr, keep_all)
+        switch (c.logm) {
+        case 7: ESTH_KBSS(7); break;
+        case 8: ESTH_KBSS(8); break;
+        case 9: ESTH_KBSS(9); break;
+        case 10: ESTH_KBSS(10); break;
+        default: return esth_fail(ESTH_ERR_INVALID, "fused class m");
+        }
+#undef ESTH_KBSS
+        return ESTH_OK;
+    }
+}
+
+template <int S>
+static int launch_kb_sample_sort(esth_batch* b, const esth_bclass& c, const float* R, const float* frames,
+                                 int log_w) {
+    switch (b->d.model) {
+    case ESTH_MODEL_ROBOTARM5: return launch_kb_sample_sort_m<ESTH_MODEL_ROBOTARM5, S>(b, c, R, frames, log_w);
+    case ESTH_MODEL_UNICYCLE: return launch_kb_sample_sort_m<ESTH_MODEL_UNICYCLE, S>(b, c, R, frames, log_w);
+    case ESTH_MODEL_UNICYCLE10: return launch_kb_sample_sort_m<ESTH_MODEL_UNICYCLE10, S>(b, c, R, frames, log_w);
+    case ESTH_MODEL_ROBOTARM5_CAMERA:
+        return launch_kb_sample_sort_m<ESTH_MODEL_ROBOTARM5_CAMERA, S>(b, c, R, frames, log_w);
+    default: return launch_kb_sample_sort_m<ESTH_MODEL_DH6, S>(b, c, R, frames, log_w);
+    }
+}
+
 #define ESTH_BCLASS(FN, S, ...)                        \
     switch (c.logm) {                                  \
     case 6: FN<6, S>(__VA_ARGS__); break;              \
@@ -400,7 +489,7 @@ static int batch_kernels_s(esth_batch* b, int p) {
     hipStream_t s = b->stream;
     const uint32_t nsub = b->nsub;
     const int grid_w = (int)(((uint64_t)b->nchunks * ESTH_WAVE + 255) / 256);
-    switch (b->d.model) {
+    if (b->nchunks) switch (b->d.model) {
     case ESTH_MODEL_ROBOTARM5:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
         break;
@@ -417,7 +506,12 @@ static int batch_kernels_s(esth_batch* b, int p) {
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
         break;
     }
-    for (const esth_bclass& c : b->cls) ESTH_BCLASS(launch_kb_sort, S, b, c, log_w);
+    for (const esth_bclass& c : b->cls) {
+        if (c.fused)
+            ESTH_TRY(launch_kb_sample_sort<S>(b, c, R, frames, log_w));
+        else
+            ESTH_BCLASS(launch_kb_sort, S, b, c, log_w);
+    }
     const uint32_t t = b->d.topology == ESTH_TOPO_NONE ? 0 : b->d.num_transfer;
     if (t > 0 && b->d.topology == ESTH_TOPO_ALL2ALL) hipLaunchKernelGGL(kb_topt, dim3(b->n), dim3(1024), 0, s, b->d_F);
     if (b->d.resampler == ESTH_RESAMPLE_ALIAS) {

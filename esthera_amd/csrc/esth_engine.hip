@@ -406,10 +406,12 @@ bool esth_sample_sort_fits(uint32_t m) { return m >= 128 && m <= 1024 && (m & (m
 
 /* resident waves per SIMD of a kernel launched with `threads`-thread
  * work-groups: 512 VGPRs per lane slot (granule 8), at most 8 waves, and
- * 160 KiB of LDS per CU shared by its work-groups */
-static int waves_per_simd(const void* fn, int threads) {
+ * 160 KiB of LDS per CU shared by its work-groups; 0 for a kernel that spills
+ * to scratch memory */
+int esth_waves_per_simd(const void* fn, int threads) {
     hipFuncAttributes a;
     if (hipFuncGetAttributes(&a, fn) != hipSuccess) return 0;
+    if (a.localSizeBytes > 0) return 0;
     const int regs = a.numRegs > 0 ? (a.numRegs + 7) / 8 * 8 : 512;
     int occ = 512 / regs;
     if (occ > 8) occ = 8;
@@ -432,8 +434,8 @@ static bool sample_sort_pays_l(int lm) {
     }
     const int ss_waves[11] = {0, 0, 0, 0, 0, 0, 0, esth_ss_waves<7>(), esth_ss_waves<8>(), esth_ss_waves<9>(),
                               esth_ss_waves<10>()};
-    const int fused = waves_per_simd(fn, ESTH_WAVE * ss_waves[lm]);
-    return fused > 0 && fused >= waves_per_simd((const void*)&k_sample_weight<MODEL, L>, 256);
+    const int fused = esth_waves_per_simd(fn, ESTH_WAVE * ss_waves[lm]);
+    return fused > 0 && fused >= esth_waves_per_simd((const void*)&k_sample_weight<MODEL, L>, 256);
 }
 
 template <int MODEL>

@@ -59,6 +59,7 @@ int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_
  * the unsorted state kept in LDS); for 128 <= m <= 1024.  The propagated
  * state is stored to `state` only for sub-filters g with keep_all or
  * gate[g] - 1 > keep_thr (those the resampling leaves untouched). */
+int esth_waves_per_simd(const void* kernel, int threads);   /* resident waves per SIMD (registers, LDS) */
 bool esth_sample_sort_fits(uint32_t m);
 bool esth_sample_sort_pays(int model, int layout, uint32_t m);   /* fits, and costs the sampling no occupancy */
 int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uint32_t G, const float* R,

@@ -1778,6 +1778,58 @@ __global__ __launch_bounds__(256) void kb_sort(const esth_bfilter* __restrict__ 
                            f.bt_s, f.part, log_w, stage[threadIdx.x / ESTH_WAVE]);
 }
 
+/* k_sample_sort over one size class of a batch (m >= 128): work-group b
+ * samples, weights and sorts sub-filter sub[b] = (filter, sub-filter) with
+ * that filter's uniforms, frame and buffers.  keep_all: the resampler's gate
+ * depends on the weights (adaptive alias), so every sub-filter without
+ * carried weights stores its propagated state; otherwise those whose gate
+ * uniform exceeds thr do.  Compiled for 8 waves per SIMD (at most 64
+ * VGPRs): left free, the compiler spends 80 on the unicycle's m = 128 and 256
+ * classes and halves the sampling's occupancy; the instantiations that would
+ * spill instead are not used. */
+template <int MODEL, int LOGM, int S>
+__global__ __launch_bounds__(ESTH_WAVE * esth_ss_waves<LOGM>()) __attribute__((amdgpu_waves_per_eu(8))) void kb_sample_sort(
+    const esth_bfilter* __restrict__ F, const uint2* __restrict__ sub, const float* __restrict__ R,
+    const float* __restrict__ frames, int log_w, float thr, int keep_all) {
+    using T = esth_model_traits<MODEL>;
+    static_assert(T::S == S, "state dimension of the model");
+    constexpr int M = 1 << LOGM;
+    constexpr int NT = ESTH_WAVE * esth_ss_waves<LOGM>();
+    __shared__ __align__(16) float lds[(1 + S) * M];
+    const uint2 e = sub[blockIdx.x];
+    const esth_bfilter& f = F[e.x];
+    const int carry = (int)(f.flags & 1u);
+    const float* Rf = R + f.r_off;
+    const bool keep = !carry && (keep_all || (Rf[f.uoff + f.N + e.y] - 1.0f) > thr);
+#pragma unroll 1
+    for (int q = 2 * (int)threadIdx.x; q < M; q += 2 * NT) {
+        const uint32_t p = e.y * (uint32_t)M + (uint32_t)q;
+        float s0[S], s1[S], u0[T::V], u1[T::V];
+        esth_load_pair<S, ESTH_LAYOUT_SOA>(f.state, f.N, p, s0, s1);
+#pragma unroll
+        for (int k = 0; k < T::V; ++k) {
+            const float2 u = *reinterpret_cast<const float2*>(Rf + (size_t)k * f.N + p);
+            u0[k] = u.x;
+            u1[k] = u.y;
+        }
+        float lw0, lw1;
+        esth_sample_pair_regs<MODEL>(s0, s1, u0, u1, frames + f.f_off, lw0, lw1);
+        if (keep) esth_store_pair<S, ESTH_LAYOUT_SOA>(f.state, f.N, p, s0, s1);
+        float2 wo = log_w ? make_float2(lw0, lw1) : make_float2(esth_expf(lw0), esth_expf(lw1));
+        if (carry) {
+            const float2 wp = *reinterpret_cast<const float2*>(f.w + p);
+            wo = log_w ? make_float2(wp.x + wo.x, wp.y + wo.y) : make_float2(wp.x * wo.x, wp.y * wo.y);
+        }
+        *reinterpret_cast<float2*>(lds + q) = wo;
+#pragma unroll
+        for (int c = 0; c < S; ++c) *reinterpret_cast<float2*>(lds + (1 + c) * M + q) = make_float2(s0[c], s1[c]);
+    }
+    __syncthreads();
+    if (threadIdx.x >= ESTH_WAVE) return;
+    esth_sort_sub<LOGM, S, false, true>(f.w, nullptr, f.tmp, f.N, f.G, e.y, true, (float2*)f.winner, nullptr, f.t,
+                                        f.bt_w, f.bt_s, f.part, log_w, nullptr, ESTH_LAYOUT_SOA, lds);
+}
+
 template <int S>
 __global__ __launch_bounds__(256) void kb_argmax(const esth_bfilter* __restrict__ F, int log_w) {
     __shared__ float2 sc[128];

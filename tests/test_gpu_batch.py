@@ -45,6 +45,10 @@ CASES = [
     ("torus-alias-adaptive-carry", dict(sizes=[(256, 16), (1024, 4), (64, 64)], topology=ea.TOPO_TORUS, t=2,
                                         resampler=ea.RESAMPLE_ALIAS, threshold=0.02, adaptive=1, carry=1)),
     ("all2all-alias", dict(sizes=[(128, 8), (512, 5)], topology=ea.TOPO_ALL2ALL, t=2, resampler=ea.RESAMPLE_ALIAS)),
+    # skipped sub-filters without carried weights keep their propagated state
+    ("ring-gate", dict(sizes=[(128, 4), (256, 3), (64, 2), (512, 2)], threshold=0.5)),
+    ("ring-alias-adaptive", dict(sizes=[(256, 4), (128, 6), (1024, 2)], resampler=ea.RESAMPLE_ALIAS, threshold=0.02,
+                                 adaptive=1)),
 ]
 
 
