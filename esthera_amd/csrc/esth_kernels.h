@@ -682,6 +682,21 @@ __device__ __forceinline__ void esth_store_plane_run(float* __restrict__ st, int
  * (src[0..M) the weights, src[(1 + c) M ..) plane c, one sub-filter per
  * wave): the keys are read from there and the gather permutes there, with
  * no global reads */
+/* v[r] to particle i0 + r * stride of plane c (a strided run: with stride
+ * 64, each r is one coalesced store across the wave) */
+template <int E, int S>
+__device__ __forceinline__ void esth_store_plane_strided(float* __restrict__ st, int layout, size_t N, int c,
+                                                         size_t i0, int stride, const float (&v)[E]) {
+    if (layout == ESTH_LAYOUT_SOA) {
+        float* p = st + (size_t)c * N + i0;
+#pragma unroll
+        for (int r = 0; r < E; ++r) p[(size_t)r * stride] = v[r];
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) st[esth_state_index(layout, S, N, c, i0 + (size_t)r * stride)] = v[r];
+}
+
 template <int LOGM, int S, bool HOIST = false, bool FROM_LDS = false>
 __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float* __restrict__ st_in,
                                               float* __restrict__ st_out, uint32_t N, uint32_t G, uint32_t g,
@@ -1029,10 +1044,14 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
 #pragma unroll
         for (int c = 0; c < S; ++c) esth_load_run<E>(st_tmp + (size_t)c * N + base + le, inh[c]);
     }
+    /* output r of this lane is particle laneg + LPS r of the sub-filter:
+     * lanes take consecutive outputs, so at every level of the binary
+     * searches the wave's probes fall on consecutive (or equal) CDF words,
+     * not E words apart, and the stores of each r are one coalesced run */
     float uu[E];
     if (!systematic) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) uu[r] = rnd[base + le + r];
+        for (int r = 0; r < E; ++r) uu[r] = rnd[base + laneg + LPS * r];
     }
     if (recv) {
 #pragma unroll
@@ -1090,7 +1109,7 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         float s;
-        if (systematic) s = ((float)(le + r) + u0) * step;
+        if (systematic) s = ((float)(laneg + LPS * r) + u0) * step;
         else s = (uu[r] - 1.0f) * total;
         /* binary_search, pf_generic.cl:182-201: on a power-of-two range each
          * round halves it exactly, so its do-while runs LOGM rounds; unrolled,
@@ -1132,7 +1151,7 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
         for (int c = 0; c < PB; ++c) {
             float v[E];
             esth_permute_lds<M, E>(in[c], le, cdf, pick, v);
-            if (go) esth_store_plane_run<E, S>(st_main, layout, N, c0 + c, base + le, v);
+            if (go) esth_store_plane_strided<E, S>(st_main, layout, N, c0 + c, base + laneg, LPS, v);
             /* carried weights: a skipped sub-filter keeps its sorted, exchanged particles */
             else if (carry && valid) esth_store_plane_run<E, S>(st_main, layout, N, c0 + c, base + le, in[c]);
         }
@@ -1145,7 +1164,7 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
     }
     if (sel && valid) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) sel[base + le + r] = go ? pick[r] : -1;
+        for (int r = 0; r < E; ++r) sel[base + laneg + LPS * r] = go ? pick[r] : -1;
     }
 }
 
