@@ -125,13 +125,22 @@ static bool kb_fuse_pays_s(int model, int lm) {
     }
 }
 
-static bool kb_fuse_pays(int model, int S, int lm) {
+/* answers cached per (model, class): 0 unknown, 1 no, 2 yes */
+static std::atomic<int> g_kb_fuse[8][11];
+
+bool esth_kb_fuse_pays(int model, int S, int lm) {
+    if (model < 0 || model >= 8 || lm < 0 || lm > 10) return false;
+    const int c = g_kb_fuse[model][lm].load(std::memory_order_relaxed);
+    if (c) return c == 2;
+    bool yes = false;
     switch (S) {
-    case ESTH_UNI_S: return kb_fuse_pays_s<ESTH_UNI_S>(model, lm);
-    case ESTH_RA5_S: return kb_fuse_pays_s<ESTH_RA5_S>(model, lm);
-    case ESTH_DH6_S: return kb_fuse_pays_s<ESTH_DH6_S>(model, lm);
-    default: return false;
+    case ESTH_UNI_S: yes = kb_fuse_pays_s<ESTH_UNI_S>(model, lm); break;
+    case ESTH_RA5_S: yes = kb_fuse_pays_s<ESTH_RA5_S>(model, lm); break;
+    case ESTH_DH6_S: yes = kb_fuse_pays_s<ESTH_DH6_S>(model, lm); break;
+    default: break;
     }
+    g_kb_fuse[model][lm].store(yes ? 2 : 1, std::memory_order_relaxed);
+    return yes;
 }
 
 struct esth_batch {
@@ -313,7 +322,7 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
     /* size classes whose sub-filters sample and sort in one launch; the
      * other filters' particles go through kb_sample_weight's chunks */
     bool fuse[11] = {};
-    for (int lm = 7; lm <= 10; ++lm) fuse[lm] = kb_fuse_pays(b->d.model, S, lm);
+    for (int lm = 7; lm <= 10; ++lm) fuse[lm] = esth_kb_fuse_pays(b->d.model, S, lm);
     auto class_of = [](uint32_t m) { int k = 0; while ((1u << k) < m) ++k; return k; };
     for (int i = 0; i < n; ++i)
         if (!fuse[class_of(b->hf[i].m)]) b->nchunks += (b->hf[i].N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
@@ -434,19 +443,18 @@ static void launch_kb_resample(esth_batch* b, const esth_bclass& c, const float*
                        log_w, b->d.topology, argmax_blocks);
 }
 
-/* sample+weight and sort of a fused size class (m >= 128) */
+/* sample+weight and sort of a fused size class (m >= 128) of n sub-filters
+ * (filter, sub-filter) in `sub`, filters from F */
 template <int MODEL, int S>
-static int launch_kb_sample_sort_m(esth_batch* b, const esth_bclass& c, const float* R, const float* frames,
-                                   int log_w) {
+static int launch_kbss_m(int lm, const esth_bfilter* F, const uint2* sub, uint32_t n, const float* R,
+                         const float* frames, int log_w, float thr, int keep_all, hipStream_t st) {
     if constexpr (esth_model_traits<MODEL>::S != S) {
         return esth_fail(ESTH_ERR_INVALID, "state dimension");
     } else {
-        const float thr = b->d.resampling_threshold;
-        const int keep_all = b->d.resampler == ESTH_RESAMPLE_ALIAS && b->d.adaptive;
-#define ESTH_KBSS(LM)                                                                                              \
-    hipLaunchKernelGGL((kb_sample_sort<MODEL, LM, S>), dim3(c.n), dim3(ESTH_WAVE * esth_ss_waves<LM>()), 0, b->stream, \
-                       b->d_F, c.d_sub, R, frames, log_w, thr, keep_all)
-        switch (c.logm) {
+#define ESTH_KBSS(LM)                                                                                         \
+    hipLaunchKernelGGL((kb_sample_sort<MODEL, LM, S>), dim3(n), dim3(ESTH_WAVE * esth_ss_waves<LM>()), 0, st, F, \
+                       sub, R, frames, log_w, thr, keep_all)
+        switch (lm) {
         case 7: ESTH_KBSS(7); break;
         case 8: ESTH_KBSS(8); break;
         case 9: ESTH_KBSS(9); break;
@@ -459,15 +467,29 @@ static int launch_kb_sample_sort_m(esth_batch* b, const esth_bclass& c, const fl
 }
 
 template <int S>
-static int launch_kb_sample_sort(esth_batch* b, const esth_bclass& c, const float* R, const float* frames,
-                                 int log_w) {
-    switch (b->d.model) {
-    case ESTH_MODEL_ROBOTARM5: return launch_kb_sample_sort_m<ESTH_MODEL_ROBOTARM5, S>(b, c, R, frames, log_w);
-    case ESTH_MODEL_UNICYCLE: return launch_kb_sample_sort_m<ESTH_MODEL_UNICYCLE, S>(b, c, R, frames, log_w);
-    case ESTH_MODEL_UNICYCLE10: return launch_kb_sample_sort_m<ESTH_MODEL_UNICYCLE10, S>(b, c, R, frames, log_w);
-    case ESTH_MODEL_ROBOTARM5_CAMERA:
-        return launch_kb_sample_sort_m<ESTH_MODEL_ROBOTARM5_CAMERA, S>(b, c, R, frames, log_w);
-    default: return launch_kb_sample_sort_m<ESTH_MODEL_DH6, S>(b, c, R, frames, log_w);
+static int launch_kbss_s(int model, int lm, const esth_bfilter* F, const uint2* sub, uint32_t n, const float* R,
+                         const float* frames, int log_w, float thr, int keep_all, hipStream_t st) {
+#define ESTH_KBSS_ARGS lm, F, sub, n, R, frames, log_w, thr, keep_all, st
+    switch (model) {
+    case ESTH_MODEL_ROBOTARM5: return launch_kbss_m<ESTH_MODEL_ROBOTARM5, S>(ESTH_KBSS_ARGS);
+    case ESTH_MODEL_UNICYCLE: return launch_kbss_m<ESTH_MODEL_UNICYCLE, S>(ESTH_KBSS_ARGS);
+    case ESTH_MODEL_UNICYCLE10: return launch_kbss_m<ESTH_MODEL_UNICYCLE10, S>(ESTH_KBSS_ARGS);
+    case ESTH_MODEL_ROBOTARM5_CAMERA: return launch_kbss_m<ESTH_MODEL_ROBOTARM5_CAMERA, S>(ESTH_KBSS_ARGS);
+    case ESTH_MODEL_DH6: return launch_kbss_m<ESTH_MODEL_DH6, S>(ESTH_KBSS_ARGS);
+    default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
+    }
+#undef ESTH_KBSS_ARGS
+}
+
+int esth_launch_kb_sample_sort(int model, int S, int lm, const esth_bfilter* F, const uint2* sub, uint32_t n,
+                               const float* R, const float* frames, int log_w, float thr, int keep_all,
+                               hipStream_t st) {
+    if (n == 0) return ESTH_OK;
+    switch (S) {
+    case ESTH_UNI_S: return launch_kbss_s<ESTH_UNI_S>(model, lm, F, sub, n, R, frames, log_w, thr, keep_all, st);
+    case ESTH_RA5_S: return launch_kbss_s<ESTH_RA5_S>(model, lm, F, sub, n, R, frames, log_w, thr, keep_all, st);
+    case ESTH_DH6_S: return launch_kbss_s<ESTH_DH6_S>(model, lm, F, sub, n, R, frames, log_w, thr, keep_all, st);
+    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
     }
 }
 
@@ -508,7 +530,9 @@ static int batch_kernels_s(esth_batch* b, int p) {
     }
     for (const esth_bclass& c : b->cls) {
         if (c.fused)
-            ESTH_TRY(launch_kb_sample_sort<S>(b, c, R, frames, log_w));
+            ESTH_TRY(esth_launch_kb_sample_sort(b->d.model, S, c.logm, b->d_F, c.d_sub, c.n, R, frames, log_w,
+                                                b->d.resampling_threshold,
+                                                b->d.resampler == ESTH_RESAMPLE_ALIAS && b->d.adaptive, s));
         else
             ESTH_BCLASS(launch_kb_sort, S, b, c, log_w);
     }

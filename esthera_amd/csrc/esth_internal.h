@@ -60,6 +60,12 @@ int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_
  * state is stored to `state` only for sub-filters g with keep_all or
  * gate[g] - 1 > keep_thr (those the resampling leaves untouched). */
 int esth_waves_per_simd(const void* kernel, int threads);   /* resident waves per SIMD (registers, LDS) */
+/* table-driven (batch, server) sample+weight + sort for size class lm: taken
+ * where kb_sample_sort keeps kb_sample_weight's occupancy (cached answer) */
+bool esth_kb_fuse_pays(int model, int S, int lm);
+int esth_launch_kb_sample_sort(int model, int S, int lm, const esth_bfilter* F, const uint2* sub, uint32_t n,
+                               const float* R, const float* frames, int log_w, float thr, int keep_all,
+                               hipStream_t st);
 bool esth_sample_sort_fits(uint32_t m);
 bool esth_sample_sort_pays(int model, int layout, uint32_t m);   /* fits, and costs the sampling no occupancy */
 int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uint32_t G, const float* R,

@@ -601,7 +601,7 @@ static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wa
     const uint2* ch = chunks + g.chk0;
     const uint32_t nch = (uint32_t)g.nchk;
     const int grid_w = (int)(((uint64_t)nch * ESTH_WAVE + 255) / 256);
-    switch (k.model) {
+    if (nch) switch (k.model) {
     case ESTH_MODEL_ROBOTARM5:
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
         break;
@@ -618,8 +618,14 @@ static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wa
         hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
         break;
     }
-    for (int lm = 6; lm <= 10; ++lm)
-        if (g.ncls[lm]) ESTH_SRV_LOGM(srv_sort, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], log_w, st);
+    for (int lm = 6; lm <= 10; ++lm) {
+        if (!g.ncls[lm]) continue;
+        if (esth_kb_fuse_pays(k.model, S, lm))
+            ESTH_TRY(esth_launch_kb_sample_sort(k.model, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], R, frames,
+                                                log_w, k.thr, k.resampler == ESTH_RESAMPLE_ALIAS && k.adaptive, st));
+        else
+            ESTH_SRV_LOGM(srv_sort, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], log_w, st);
+    }
     if (k.t > 0 && k.topology == ESTH_TOPO_ALL2ALL) hipLaunchKernelGGL(kb_topt, dim3(g.nf), dim3(1024), 0, st, Fg);
     if (k.resampler == ESTH_RESAMPLE_ALIAS) {
         hipLaunchKernelGGL(kb_argmax<S>, dim3(g.nf), dim3(256), 0, st, Fg, log_w);
@@ -691,7 +697,8 @@ static int launch_wave(esth_server* s, bool* launched) {
     for (auto& u : upd) {
         words += u.t->words;
         nsub += u.t->d.num_blocks;
-        nchk += (u.t->bf.N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
+        /* fused size classes sample inside kb_sample_sort, without chunks */
+        if (!esth_kb_fuse_pays(u.t->d.model, u.t->S, u.t->logm)) nchk += (u.t->bf.N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
         nstreams += (uint64_t)u.t->K;
         frames += u.t->frame_floats;
     }
@@ -756,7 +763,8 @@ static int launch_wave(esth_server* s, bool* launched) {
         const uint32_t li = (uint32_t)(i - g.f0);
         for (uint32_t b = 0; b < t->d.num_blocks; ++b) hall[sub_pos + b] = make_uint2(li, b);
         sub_pos += t->d.num_blocks;
-        for (uint32_t q = 0; q < t->bf.N / 2; q += ESTH_WAVE) hchk[chk_pos++] = make_uint2(li, q);
+        if (!esth_kb_fuse_pays(t->d.model, t->S, t->logm))
+            for (uint32_t q = 0; q < t->bf.N / 2; q += ESTH_WAVE) hchk[chk_pos++] = make_uint2(li, q);
         g.nchk = chk_pos - g.chk0;
         g.nf++;
         g.nsub += t->d.num_blocks;

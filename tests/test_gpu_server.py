@@ -58,8 +58,8 @@ def test_server_mix_equals_tracks_alone(engine):
              unicycle_track(14, 1024, 4, L=32), unicycle_track(15, 512, 6, topology=ea.TOPO_RING, t=2,
                                                                resampler=ea.RESAMPLE_ALIAS),
              robotarm_track(16, 256, 3), unicycle_track(17, 128, 9, topology=ea.TOPO_ALL2ALL, t=2),
-             unicycle_track(18, 256, 6, threshold=0.5, carry=1)]
-    nsteps = [9, 14, 7, 11, 6, 12, 8, 10]
+             unicycle_track(18, 256, 6, threshold=0.5, carry=1), unicycle_track(19, 128, 8, threshold=0.5)]
+    nsteps = [9, 14, 7, 11, 6, 12, 8, 10, 9]
     srv = ea.Server(engine, 64, GiB)
     tracks = [None] * len(specs)
     sent = [0] * len(specs)
