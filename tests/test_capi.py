@@ -191,6 +191,17 @@ def test_autotune_pick_rules():
         ea.autotune_pick(us, 100, 64)
 
 
+def test_filter_controls_reject_null_handles():
+    """Filter switches fail with a status (no device call) on a null filter."""
+    import ctypes as C
+    L = ea.lib()
+    active = C.c_int(7)
+    assert L.esth_filter_set_fusion(None, 1, C.byref(active)) != 0
+    assert active.value == 7
+    assert L.esth_filter_set_prefetch(None, 1) != 0
+    assert "null filter" in L.esth_last_error().decode()
+
+
 def test_batch_autotune_decl_parses():
     assert "esth_batch_create" in header_functions() and "esth_autotune" in header_functions()
 
