@@ -362,8 +362,10 @@ def main():
         print(json.dumps({"graph_ms_per_step": gsecs * 1e3 / args.steps}), flush=True)
         return
     # timed region: eager steps; HIP events bracket the sample+weight+sort
-    # launch on every 10th step (each event costs the stream a few microseconds)
-    secs, stage, err = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, every=10)
+    # launch on every 25th step (every 10th below 250 steps; each event costs
+    # the stream a few microseconds)
+    every = 25 if args.steps >= 250 else 10
+    secs, stage, err = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, every=every)
     value = N * args.steps / secs
     if args.main_only:
         sw_us = stage[1]
