@@ -1802,14 +1802,15 @@ __global__ __launch_bounds__(256) void kb_sort(const esth_bfilter* __restrict__ 
  * that filter's uniforms, frame and buffers.  keep_all: the resampler's gate
  * depends on the weights (adaptive alias), so every sub-filter without
  * carried weights stores its propagated state; otherwise those whose gate
- * uniform exceeds thr do.  Compiled for the occupancy of kb_sample_weight:
- * 8 waves per SIMD (at most 64 VGPRs) for the 3-dimensional models, 6 for
- * the others; left free, the compiler spends 80 VGPRs on the unicycle's
- * m = 128 and 256 classes and halves the sampling's occupancy.  The
- * instantiations that would spill are not used. */
+ * uniform exceeds thr do.  The 3-dimensional models are compiled for 8
+ * waves per SIMD (at most 64 VGPRs): left free, the compiler spends 80 VGPRs
+ * on the unicycle's m = 128 and 256 classes and halves the sampling's
+ * occupancy.  The others are left free (a 6-wave target made the arm models
+ * spill).  Instantiations that spill, or keep fewer waves resident than
+ * kb_sample_weight, are not used. */
 template <int MODEL>
 __host__ __device__ constexpr int esth_kbss_waves_per_eu() {
-    return esth_model_traits<MODEL>::S <= 3 ? 8 : 6;
+    return esth_model_traits<MODEL>::S <= 3 ? 8 : 1;
 }
 
 template <int MODEL, int LOGM, int S>
