@@ -85,6 +85,43 @@ def test_batch_equals_filters_alone(name, kw, graph, engine):
     b.close()
 
 
+def test_batch_dh6_filters_equal_alone(engine):
+    """dh6 filters (12-dimensional state) in one batch, fused size classes
+    (m = 128, 256) beside m = 64 and 512, with a gate that skips sub-filters."""
+    sizes = [(128, 4), (256, 2), (64, 6), (512, 2)]
+    nf, steps = 6, 4
+    descs, params, scen, inits = [], [], [], []
+    for i in range(nf):
+        m, G = sizes[i % len(sizes)]
+        descs.append(ea.filter_desc(ea.MODEL_DH6, m=m, G=G, t=1, topology=ea.TOPO_RING,
+                                    resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, threshold=0.5,
+                                    rng=ea.RNG_MTGP_PER_FILTER, seed=300 + i))
+        p, meas, truth = ea.dh6_scenario(70 + i, steps)
+        params.append(p)
+        scen.append(meas)
+        inits.append(truth[0])
+    b = ea.Batch(engine, descs, params)
+    b.init_uniform(np.array(inits, np.float32))
+    alone = []
+    for i in range(nf):
+        f = ea.Filter(engine, descs[i], params[i])
+        f.init_uniform(inits[i])
+        alone.append(f)
+    for k in range(steps):
+        ctrls = (ea.DH6Control * nf)()
+        meas = (ea.DH6Measurement * nf)(*[sc[k] for sc in scen])
+        b.step(ctrls, meas, np.full(nf, 0.01, np.float32))
+        for i, f in enumerate(alone):
+            f.step(ea.DH6Control(), scen[i][k], 0.01)
+    for i, f in enumerate(alone):
+        sb, wb = b.particles(i)
+        sa, wa = f.particles()
+        assert np.array_equal(sb.view(np.uint32), sa.view(np.uint32)), f"filter {i} state differs"
+        assert np.array_equal(wb.view(np.uint32), wa.view(np.uint32)), f"filter {i} weights differ"
+        f.close()
+    b.close()
+
+
 def test_batch_camera_filters_equal_alone(engine):
     """robotarm5_camera filters with their own calibrations in one batch."""
     sizes = [(64, 4), (256, 2), (128, 3)]
