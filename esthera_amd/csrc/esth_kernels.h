@@ -815,16 +815,6 @@ __global__ __launch_bounds__(256) void k_sort(float* __restrict__ w, const float
                            stage[threadIdx.x / ESTH_WAVE], layout);
 }
 
-/* sample+weight and block sort of one sub-filter in one work-group of M/2
- * threads: each thread samples and weights its pair (as k_sample_weight,
- * same uniforms, same arithmetic) into the work-group's LDS, then wave 0
- * sorts the sub-filter there (esth_sort_sub, FROM_LDS) and writes what
- * k_sort writes.  The propagated unsorted state is read only by the sort,
- * except in a sub-filter the resampling will skip without carried weights,
- * which keeps it (pf_generic.cl:403): it is stored only for those, known
- * here from the gate's uniform (gate[g] - 1 > keep_thr) or, for the
- * data-dependent alias gate, always (keep_all).  The sort's wave runs beside
- * other work-groups' sampling instead of after all of them. */
 /* waves per work-group of k_sample_sort (W = min(M / 128, 4)): each thread
  * samples M / (128 W) pairs.  Four waves beat one or two at m = 512 (90 µs
  * against 93 and 95 for the 2M-particle unicycle): more sampling waves
@@ -834,6 +824,17 @@ __host__ __device__ constexpr int esth_ss_waves() {
     return (1 << LOGM) / 128 < 4 ? (1 << LOGM) / 128 : 4;
 }
 
+/* sample+weight and block sort of one sub-filter in one work-group of
+ * esth_ss_waves waves: each thread samples and weights its pairs (as
+ * k_sample_weight, same uniforms, same arithmetic) into the work-group's LDS,
+ * then wave 0
+ * sorts the sub-filter there (esth_sort_sub, FROM_LDS) and writes what
+ * k_sort writes.  The propagated unsorted state is read only by the sort,
+ * except in a sub-filter the resampling will skip without carried weights,
+ * which keeps it (pf_generic.cl:403): it is stored only for those, known
+ * here from the gate's uniform (gate[g] - 1 > keep_thr) or, for the
+ * data-dependent alias gate, always (keep_all).  The sort's wave runs beside
+ * other work-groups' sampling instead of after all of them. */
 template <int MODEL, int L, int LOGM>
 __global__ __launch_bounds__(ESTH_WAVE * esth_ss_waves<LOGM>()) void k_sample_sort(float* __restrict__ state, uint32_t N,
                                                                  uint32_t G, const float* __restrict__ R,
