@@ -1223,6 +1223,185 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
                                cdf_s[threadIdx.x / ESTH_WAVE], true, xch, carry, const_cast<float*>(w), layout);
 }
 
+/* The same resampling of one sub-filter by two waves (m >= 256): wave h
+ * holds particles h M/2 .. (h + 1) M/2 - 1, E = M/128 per lane, lane-major.
+ * Every value is the one-wave version's: the scan tree is the same binary
+ * tree over positions (register levels, cross-lane levels, then the wave
+ * level, whose left node wave 1 adds first, as bit 8 of its positions asks),
+ * the block max is a max, the search and the selections are per output.
+ * Twice the waves per launch cover twice the latency.  cdf: the sub-filter's
+ * M floats of LDS; stage: S M floats (the gather's planes); xl: 4 floats
+ * (the waves' maxima and totals).  Both waves (and every wave of the
+ * work-group) must call it: three work-group barriers inside. */
+template <int LOGM, int S>
+__device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, const float* __restrict__ st_tmp,
+                                                   const float* w, const float* __restrict__ rnd, uint32_t N,
+                                                   uint32_t g, bool valid, float thr, int systematic, int log_w,
+                                                   float* cdf, float* stage, float* xl, int h, esth_xch xch,
+                                                   int carry, float* w_mut, int layout) {
+    constexpr int M = 1 << LOGM, E = M / 128, LOGE = LOGM - 7;
+    static_assert(LOGM >= 8, "two waves of at least two particles per lane");
+    const int lane = threadIdx.x & (ESTH_WAVE - 1);
+    const int le = h * (M / 2) + lane * E;   /* first position of this lane */
+    const size_t base = (size_t)g * M;
+    const bool go = valid && !((rnd[N + g] - 1.0f) > thr);   /* gate, pf_generic.cl:403 */
+    int rsrc[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) rsrc[r] = -1;
+    const bool recv = xch.t > 0;
+    if (recv) {
+        const uint32_t t = xch.t, nrecv = xch.nq * t;
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t pos = (uint32_t)(le + r);
+            if (pos >= (uint32_t)M - nrecv) {
+                const uint32_t j = (uint32_t)M - 1u - pos, q = j / t, k = j - q * t;
+                rsrc[r] = (int)esth_recv_src(g, q, k, xch.G, t, xch.topo, xch.dimx, xch.dimy, xch.top);
+            }
+        }
+    }
+    float x[E];
+    esth_load_run<E>(w + base + le, x);
+    /* outputs: particle h M/2 + lane + 64 r */
+    const int ob = h * (M / 2) + lane;
+    float uu[E];
+    if (!systematic) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) uu[r] = rnd[base + ob + ESTH_WAVE * r];
+    }
+    if (recv) {
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+            if (rsrc[r] >= 0) {
+                x[r] = xch.bt_w[rsrc[r]];
+                if (valid) xch.w_out[base + le + r] = x[r];
+            }
+    }
+    if (log_w) {
+        float mx = x[0];
+#pragma unroll
+        for (int r = 1; r < E; ++r) mx = x[r] > mx ? x[r] : mx;
+        mx = esth_bfly_max<1, ESTH_WAVE>(mx);
+        if (lane == 0) xl[h] = mx;
+        __syncthreads();
+        const float m0 = xl[0], m1 = xl[1];
+        mx = m1 > m0 ? m1 : m0;
+#pragma unroll
+        for (int r = 0; r < E; ++r) x[r] = esth_expf(x[r] - esth_max_shift(mx));
+    } else {
+        __syncthreads();
+    }
+    float lvl[LOGE + 1][E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) lvl[0][r] = x[r];
+#pragma unroll
+    for (int k = 1; k <= LOGE; ++k)
+#pragma unroll
+        for (int j = 0; j < (E >> k); ++j) lvl[k][j] = lvl[k - 1][2 * j] + lvl[k - 1][2 * j + 1];
+    float node = lvl[LOGE][0];
+    float lx[6];
+    esth_scan_levels<0, 6>(node, lx);
+    if (lane == 0) xl[2 + h] = node;   /* the wave's node: [h M/2, (h + 1) M/2) */
+    __syncthreads();
+    const float left = xl[2];
+    const float total = left + xl[3];
+    float acc = 0.0f;
+    if (h) acc = acc + left;
+#pragma unroll
+    for (int k = 5; k >= 0; --k)
+        if ((lane >> k) & 1) acc = acc + lx[k];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        float a = acc;
+#pragma unroll
+        for (int k = LOGE - 1; k >= 0; --k)
+            if ((r >> k) & 1) a = a + lvl[k][(r >> k) - 1];
+        cdf[le + r] = a;
+    }
+    /* the sorted particles (with the received ones), staged for the gather */
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+        float in[E];
+        esth_load_run<E>(st_tmp + (size_t)c * N + base + le, in);
+        if (recv) {
+            const size_t Gt = (size_t)xch.G * xch.t;
+#pragma unroll
+            for (int r = 0; r < E; ++r)
+                if (rsrc[r] >= 0) in[r] = xch.bt_s[(size_t)c * Gt + (size_t)rsrc[r]];
+        }
+        esth_store_run<E>(stage + c * M + le, in);
+        /* carried weights: a skipped sub-filter keeps its sorted, exchanged particles */
+        if (!go && carry && valid) esth_store_plane_run<E, S>(st_main, layout, N, c, base + le, in);
+    }
+    __syncthreads();
+    const float u0 = rnd[base] - 1.0f;
+    const float step = total / (float)M;
+    int pick[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        float sv;
+        if (systematic) sv = ((float)(ob + ESTH_WAVE * r) + u0) * step;
+        else sv = (uu[r] - 1.0f) * total;
+        int mn = 0, mxi = M - 1;
+#pragma unroll
+        for (int it = 0; it < LOGM; ++it) {
+            const int cur = mn + (mxi - mn + 1) / 2;
+            const bool lft = cdf[cur] > sv;
+            mxi = lft ? cur - 1 : mxi;
+            mn = lft ? mn : cur;
+        }
+        pick[r] = mn & (M - 1);
+    }
+    if (go) {
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+            float v[E];
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = stage[c * M + pick[r]];
+            esth_store_plane_strided<E, S>(st_main, layout, N, c, base + ob, ESTH_WAVE, v);
+        }
+    }
+    if (carry && go) {   /* resampled: equal weights again */
+        float w0[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) w0[r] = log_w ? 0.0f : 1.0f;
+        esth_store_run<E>(w_mut + base + le, w0);
+    }
+}
+
+/* k_resample_argmax with two waves per sub-filter (m >= 256): work-group 0
+ * the argmax, each other work-group of four waves two sub-filters */
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void k_resample2_argmax(float* __restrict__ st_main,
+                                                          const float* __restrict__ st_tmp, const float* w,
+                                                          const float* __restrict__ rnd, uint32_t N, uint32_t G,
+                                                          float thr, int systematic, int log_w,
+                                                          const float2* __restrict__ winner,
+                                                          const float* __restrict__ part,
+                                                          const float* __restrict__ best_src, size_t plane_stride,
+                                                          uint32_t g_stride, float2* __restrict__ result,
+                                                          double* __restrict__ est, esth_xch xch, int carry,
+                                                          int layout) {
+    constexpr int M = 1 << LOGM;
+    __shared__ __align__(16) float cdf_s[2][M];
+    __shared__ __align__(16) float stage_s[2][S * M];
+    __shared__ float xl_s[2][4];
+    __shared__ float2 sc[128];
+    __shared__ float smax[4];
+    __shared__ double sd[4 * (2 + S)];
+    if (blockIdx.x == 0) {
+        esth_argmax_block<S, 256>(winner, G, log_w, part, best_src, plane_stride, g_stride, result, est, sc, smax, sd);
+        return;
+    }
+    const int wv = threadIdx.x / ESTH_WAVE;   /* 0..3: sub-filter wv / 2, half wv % 2 */
+    const int j = wv >> 1;
+    uint32_t g = (blockIdx.x - 1) * 2u + (uint32_t)j;
+    const bool valid = g < G;
+    if (!valid) g = G - 1;
+    esth_resample_sub2<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, cdf_s[j], stage_s[j],
+                                xl_s[j], wv & 1, xch, carry, const_cast<float*>(w), layout);
+}
+
 /* =========================================================== Vose alias
  * resampling_valias (resampling-valias.cl:214-304): one work-group of m per
  * sub-filter.  Normalisation sums with sum_reduce4's exact tree (:85-104);
