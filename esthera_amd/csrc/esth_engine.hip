@@ -583,9 +583,10 @@ static void launch_resample_argmax_t(float* st_main, const float* st_tmp, const 
                                      const float* part, const float* best_src, size_t plane_stride, uint32_t g_stride,
                                      float2* result, double* est, const esth_xch& xch, int carry, int layout,
                                      hipStream_t s) {
-    /* two waves per sub-filter where the work-group's two sub-filters'
-     * CDFs and staged planes fit in 48 KB of LDS */
-    if constexpr (LOGM >= 8 && (1 + S) * (1 << LOGM) <= 6144) {
+    /* two waves per sub-filter for m >= 512 where the work-group's two
+     * sub-filters' CDFs and staged planes fit in 48 KB of LDS (at m = 256 in
+     * the batched classes the halves were too short to pay for the barriers) */
+    if constexpr (LOGM >= 9 && (1 + S) * (1 << LOGM) <= 6144) {
         hipLaunchKernelGGL((k_resample2_argmax<LOGM, S>), dim3(1 + (G + 1) / 2), dim3(256), 0, s, st_main, st_tmp, w,
                            rnd, N, G, thr, systematic, log_w, winner, part, best_src, plane_stride, g_stride, result,
                            est, xch, carry, layout);

@@ -76,6 +76,12 @@ CASES = [
     ("robotarm-alias-adaptive", dict(model=ea.MODEL_ROBOTARM5, m=256, G=8, t=1, resampler=ea.RESAMPLE_ALIAS,
                                      adaptive=1, threshold=0.01)),
     ("dh6-torus-log", dict(model=ea.MODEL_DH6, m=512, G=8, t=1, topology=ea.TOPO_TORUS, weights=ea.WEIGHTS_LOG)),
+    # scan resampling with two waves per sub-filter (m >= 512)
+    ("unicycle-m512-systematic-torus-t4", dict(model=ea.MODEL_UNICYCLE, m=512, G=16, t=4, topology=ea.TOPO_TORUS,
+                                                resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG)),
+    ("unicycle-m1024-multinomial-gate", dict(model=ea.MODEL_UNICYCLE, m=1024, G=8, t=2, threshold=0.5,
+                                             weights=ea.WEIGHTS_LOG)),
+    ("robotarm-m512-linear-all2all", dict(model=ea.MODEL_ROBOTARM5, m=512, G=8, t=2, topology=ea.TOPO_ALL2ALL)),
     ("unicycle10-reference-linear", dict(model=ea.MODEL_UNICYCLE10, m=128, G=16, t=1)),
     ("unicycle-carry-gate-log", dict(model=ea.MODEL_UNICYCLE, m=256, G=16, t=1, topology=ea.TOPO_TORUS,
                                      resampler=ea.RESAMPLE_SYSTEMATIC, threshold=0.5, weights=ea.WEIGHTS_LOG,
