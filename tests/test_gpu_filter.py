@@ -200,6 +200,34 @@ def test_two_million_particle_properties(engine):
     assert np.hypot(est.mean[0] - truth[2, 0], est.mean[1] - truth[2, 1]) < 0.5
 
 
+def test_headline_size_bit_exact_vs_oracle(engine):
+    """The 2M-particle headline step itself (64x64 torus of 512, t = 4,
+    systematic, 32 landmarks, MTGP32 seed 42; the fused sample+sort launch and
+    the two-wave resampling) against the oracle, bit for bit, for two steps."""
+    import ctypes
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=512, G=4096, t=4, topology=ea.TOPO_TORUS,
+                          resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, seed=42)
+    params, ctrl, meas, truth = ea.unicycle_scenario(42, 32, 50.0, 2)
+    f = ea.Filter(engine, desc, params)
+    assert f.set_fusion(True)
+    o = O.Filter(desc, params, ctypes.sizeof(params))
+    init = np.zeros(3, np.float32)
+    f.init_uniform(init)
+    o.init_uniform(init)
+    for k in range(2):
+        f.step(ctrl[k], meas[k], 0.1)
+        o.step(ctrl[k], meas[k], 0.1)
+        gs, gw = f.particles()
+        os_, ow = o.read(3)
+        assert np.array_equal(gs.view(np.uint32), os_.view(np.uint32)), f"state differs at step {k}"
+        assert np.array_equal(gw.view(np.uint32), ow.view(np.uint32)), f"weights differ at step {k}"
+    est = f.estimate()
+    m32, m64, best = o.estimate(3)
+    assert est.best_weight == best[0] and est.best_block == int(best[1])
+    np.testing.assert_allclose(np.array(est.mean[:3]), m32, rtol=1e-6, atol=1e-6)
+    f.close()
+
+
 def test_maximum_size_properties(engine):
     """The largest filter the descriptor admits: 65536 sub-filters of 1024
     particles (67,108,864 particles, a 256 x 256 torus exchanging 4 per
