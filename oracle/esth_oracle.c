@@ -693,6 +693,7 @@ struct or_filter {
     float* tmp;               /* S x N */
     float* w;                 /* N */
     float winner_best[2];
+    float best_state[ESTH_MAX_STATE_DIM];
     double* mean32;
     double* mean64;
     double* st64;
@@ -819,6 +820,9 @@ int or_filter_step(or_filter* f, const void* ctrl, const void* meas, float dt) {
     /* reduce, pf_generic.c:604-615 */
     or_argmax(winner, G, d->weights, f->winner_best);
     free(winner);
+    if (f->winner_best[1] >= 0.0f)
+        for (int c = 0; c < f->S; ++c)
+            f->best_state[c] = f->tmp[(size_t)c * N + (size_t)f->winner_best[1] * m];
     /* exchange, pf_generic.c:702-715 */
     if (d->num_transfer > 0 && d->topology != ESTH_TOPO_NONE)
         if (or_exchange(f->tmp, f->w, f->S, m, G, d->num_transfer, d->topology, f->dimx, f->dimy))
@@ -854,6 +858,13 @@ void or_filter_estimate(or_filter* f, double* m32, double* m64, float* best) {
     if (m32) memcpy(m32, f->mean32, sizeof(double) * f->S);
     if (m64) memcpy(m64, f->mean64, sizeof(double) * f->S);
     if (best) { best[0] = f->winner_best[0]; best[1] = f->winner_best[1]; }
+}
+
+/* the best particle of the best sub-filter of the last step: position 0 of
+ * its sorted block, as the argmax found it (pf_generic.c:634-668 reads the
+ * sorted copy before the exchange) */
+void or_filter_best_particle(or_filter* f, float* out) {
+    for (int c = 0; c < f->S; ++c) out[c] = f->best_state[c];
 }
 
 uint64_t or_filter_rng_words(or_filter* f) { return f->words; }

@@ -63,6 +63,8 @@ void or_filter_read(or_filter* f, float* state, float* w);
  * sampled particles (mean32) and from an FP64 propagation of the same prior
  * with the same normals (mean64); best = (w, g) of the argmax. */
 void or_filter_estimate(or_filter* f, double* mean_f32path, double* mean_f64path, float* best);
+/* state of the best particle (sorted block winner_best[1], position 0) */
+void or_filter_best_particle(or_filter* f, float* out);
 uint64_t or_filter_rng_words(or_filter* f);
 void or_filter_set_f64(or_filter* f, int on);
 void or_set_threads(int n);

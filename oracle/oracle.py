@@ -66,6 +66,7 @@ def lib():
         L.or_filter_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_filter_estimate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_filter_rng_words.argtypes = [C.c_void_p]
+        L.or_filter_best_particle.argtypes = [C.c_void_p, _f32p]
         L.or_filter_rng_words.restype = C.c_uint64
         L.or_set_threads.argtypes = [C.c_int]
         L.or_filter_set_f64.argtypes = [C.c_void_p, C.c_int]
@@ -216,6 +217,11 @@ class Filter:
         w = np.zeros(self.N, np.float32)
         lib().or_filter_read(self.h, st.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p))
         return st, w
+
+    def best_particle(self, S):
+        out = np.zeros(S, np.float32)
+        lib().or_filter_best_particle(self.h, out)
+        return out
 
     def estimate(self, S):
         m32 = np.zeros(S, np.float64)

@@ -1,13 +1,36 @@
 """The batched form: many filters of different sizes, landmark sets and seeds
 stepped by one launch sequence per step.  Every filter's particles must be
-bit-identical to the same filter stepped alone (ESTH_RNG_MTGP_PER_FILTER), in
-eager and graph-replay mode, for each resampler and topology."""
+bit-identical to the same filter stepped alone (ESTH_RNG_MTGP_PER_FILTER) and
+to the oracle's run of that filter, in eager and graph-replay mode, for each
+resampler and topology; posterior means within 1e-4 * |m64| + 1e-6 of the
+oracle's FP64 twin."""
 import ctypes as C
 
 import numpy as np
 import pytest
 
 import esthera_amd as ea
+import oracle as O
+
+
+def oracle_run(desc, params, inputs, init, S):
+    """The oracle's run of one filter: (state, weights, m32, m64, best)."""
+    o = O.Filter(desc, params, C.sizeof(params) if params is not None else 0)
+    o.init_uniform(init)
+    for c, m, dt in inputs:
+        o.step(c, m, dt)
+    st, w = o.read(S)
+    m32, m64, best = o.estimate(S)
+    return st, w, m32, m64, best
+
+
+def check_vs_oracle(i, st, w, est, ref, S):
+    ost, ow, m32, m64, best = ref
+    assert np.array_equal(st.view(np.uint32), ost.view(np.uint32)), f"filter {i} state differs from the oracle"
+    assert np.array_equal(w.view(np.uint32), ow.view(np.uint32)), f"filter {i} weights differ from the oracle"
+    assert est.best_weight == best[0] and est.best_block == int(best[1])
+    mean = np.array(est.mean[:S])
+    assert np.all(np.abs(mean - m64) <= 1e-4 * np.abs(m64) + 1e-6), f"filter {i}: {mean} vs FP64 {m64}"
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +102,9 @@ def test_batch_equals_filters_alone(name, kw, graph, engine):
         eb, ea_ = b.estimate(i), f.estimate()
         np.testing.assert_allclose(np.array(eb.mean[:3]), np.array(ea_.mean[:3]), rtol=1e-9, atol=1e-9)
         assert eb.best_weight == ea_.best_weight and eb.best_block == ea_.best_block
+        ref = oracle_run(descs[i], params[i], [(scen[i][0][k], scen[i][1][k], 0.1) for k in range(steps)],
+                         np.zeros(3, np.float32), 3)
+        check_vs_oracle(i, sb, wb, eb, ref, 3)
         f.close()
     t = b.step_times()
     assert len(t) == steps and np.all(t > 0)
@@ -118,6 +144,9 @@ def test_batch_dh6_filters_equal_alone(engine):
         sa, wa = f.particles()
         assert np.array_equal(sb.view(np.uint32), sa.view(np.uint32)), f"filter {i} state differs"
         assert np.array_equal(wb.view(np.uint32), wa.view(np.uint32)), f"filter {i} weights differ"
+        ref = oracle_run(descs[i], params[i], [(ea.DH6Control(), scen[i][k], 0.01) for k in range(steps)],
+                         inits[i], 12)
+        check_vs_oracle(i, sb, wb, b.estimate(i), ref, 12)
         f.close()
     b.close()
 

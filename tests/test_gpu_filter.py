@@ -2,7 +2,11 @@
 device-resident state) against the oracle's restatement of the reference's
 step loop (pf_generic.c:438-808), on the same seed and inputs.  Particle
 states after every step are compared bit for bit; posterior means against the
-FP64 twin within the 1e-4 relative bar."""
+FP64 twin per component within 1e-4 * |m64| + 1e-6 (relative wherever the
+component is not tiny).  The camera model's pixel likelihood (1 px noise at a
+focal length of ~400 px) turns FP32 rounding in the projection into log-weight
+differences of ~1e-3, so its means are held to 1e-4 * |m64| + 1e-4; the
+oracle's own FP32 path shows the same gap to its FP64 twin."""
 import os
 
 import numpy as np
@@ -57,8 +61,8 @@ def run_pair(engine, desc, steps, check_every=1):
             est = f.estimate()
             m32, m64, best = o.estimate(S)
             mean = np.array(est.mean[:S])
-            scale = np.maximum(np.abs(m64), 1.0)
-            worst = max(worst, float(np.max(np.abs(mean - m64) / scale)))
+            atol = 1e-4 if model == ea.MODEL_ROBOTARM5_CAMERA else 1e-6
+            worst = max(worst, float(np.max(np.abs(mean - m64) / (1e-4 * np.abs(m64) + atol))))
             np.testing.assert_allclose(mean, m32, rtol=1e-6, atol=1e-6)
             assert est.best_weight == best[0] and est.best_block == int(best[1])
     return worst
@@ -112,7 +116,7 @@ CASES = [
 def test_filter_steps_bit_exact(name, kw, engine):
     desc = ea.filter_desc(seed=1234, **kw)
     worst = run_pair(engine, desc, 20)
-    assert worst < 1e-4, f"posterior mean vs FP64 oracle: {worst:.3e}"
+    assert worst < 1.0, f"posterior mean vs FP64 oracle: {worst:.3f} of the bar"
 
 
 def test_graph_replay_equals_eager(engine):
@@ -153,9 +157,29 @@ def test_snapshot_restore_resumes_identically(engine):
 
 
 def test_reference_driver_robotarm_input5(engine):
-    err, steps = ea.particle_filter(64, 16, 1, 1.0, os.path.join(HERE, "golden", "robotarm_input5.txt"))
+    """esth_particle_filter(64, 16, 1, 1.0, input5), the drop-in for
+    particle_filter(), against the oracle's run of the same loop
+    (pf_generic.c:634-696,814): the reference's defaults (ring, multinomial,
+    linear weights, seed 1234), particles starting at pf_generic.c:346-361,
+    estimate_error of the best particle each step (pf_model_host.h:56-69),
+    accumulated in double and averaged over the 100 rows."""
+    import ctypes
+    path = os.path.join(HERE, "golden", "robotarm_input5.txt")
+    err, steps = ea.particle_filter(64, 16, 1, 1.0, path)
     assert steps == 100
-    assert np.isfinite(err) and err > 0
+    meas, ctrl, dts, truth = ea.load_robotarm_trace(path)
+    desc = ea.filter_desc(ea.MODEL_ROBOTARM5, m=64, G=16, t=1, topology=ea.TOPO_RING,
+                          resampler=ea.RESAMPLE_MULTINOMIAL, threshold=1.0, seed=1234)
+    o = O.Filter(desc, None, 0)
+    a0 = np.float32(0.75 * np.pi)
+    o.init_uniform(np.array([a0] * 5 + [7.0, 0.0, 0.0, 0.0], np.float32))
+    total = 0.0
+    for k in range(steps):
+        o.step(ctrl[k], meas[k], float(dts[k]))
+        best = o.best_particle(9)
+        best[7:] = 0.0
+        total += ea.lib().esth_robotarm5_estimate_error(ea.ptr(best), ea.ptr(np.ascontiguousarray(truth[k])))
+    assert err == total / steps, (err, total / steps)
 
 
 def test_reference_unicycle_driver(engine):
@@ -375,12 +399,12 @@ def test_small_unicycle_500_steps_rmse(engine):
         mean = np.array(est.mean[:3])
         eg.append(np.hypot(mean[0] - truth[k, 0], mean[1] - truth[k, 1]))
         eo.append(np.hypot(m32[0] - truth[k, 0], m32[1] - truth[k, 1]))
-        worst = max(worst, float(np.max(np.abs(mean - m64) / np.maximum(np.abs(m64), 1.0))))
+        worst = max(worst, float(np.max(np.abs(mean - m64) / (1e-4 * np.abs(m64) + 1e-6))))
     gs, gw = f.particles()
     os_, ow = o.read(3)
     assert np.array_equal(gs.view(np.uint32), os_.view(np.uint32))
     rmse_g, rmse_o = float(np.sqrt(np.mean(np.square(eg)))), float(np.sqrt(np.mean(np.square(eo))))
-    assert worst < 1e-4
+    assert worst < 1.0
     assert abs(rmse_g - rmse_o) < 1e-6 * max(rmse_o, 1.0)
     assert rmse_g < 0.2, rmse_g
     f.close()

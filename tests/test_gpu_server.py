@@ -1,13 +1,17 @@
 """The served form: tracks of different models, sizes and settings opened,
 updated and closed by requests that arrive while earlier waves run.  Every
 result (the estimate of each update) and every track's final particles must be
-bit-identical to the same filter stepped alone with per-request launches."""
+bit-identical to the same filter stepped alone with per-request launches, and
+to the oracle's run of that filter (posterior means within
+1e-4 * |m64| + 1e-6 of its FP64 twin)."""
+import ctypes
 import os
 
 import numpy as np
 import pytest
 
 import esthera_amd as ea
+import oracle as O
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -97,6 +101,22 @@ def test_server_mix_equals_tracks_alone(engine):
         st, w = srv.particles(tracks[i])
         assert np.array_equal(st.view(np.uint32), ref[i][1].view(np.uint32)), f"track {i} state"
         assert np.array_equal(w.view(np.uint32), ref[i][2].view(np.uint32)), f"track {i} weights"
+    by_step = {v: rid for rid, v in expect.items()}
+    for i, (d, p, inputs, init) in enumerate(specs):
+        S = ea.STATE_DIM[d.model]
+        o = O.Filter(d, p, ctypes.sizeof(p) if p is not None else 0)
+        o.init_uniform(init)
+        for k in range(nsteps[i]):
+            o.step(*inputs[k])
+            m32, m64, best = o.estimate(S)
+            e = got[by_step[(i, k)]].estimate
+            assert e.best_weight == best[0] and e.best_block == int(best[1]), f"track {i} step {k}"
+            mean = np.array(e.mean[:S])
+            assert np.all(np.abs(mean - m64) <= 1e-4 * np.abs(m64) + 1e-6), f"track {i} step {k}"
+        ost, ow = o.read(S)
+        st, w = srv.particles(tracks[i])
+        assert np.array_equal(st.view(np.uint32), ost.view(np.uint32)), f"track {i} state vs oracle"
+        assert np.array_equal(w.view(np.uint32), ow.view(np.uint32)), f"track {i} weights vs oracle"
     waves, wave_updates, peak = srv.wave_stats()
     assert wave_updates == len(expect) and waves < len(expect)
     srv.close()
