@@ -18,6 +18,9 @@ LIB_PATH = os.path.join(HERE, "libesthera.so")
 
 # --------------------------------------------------------------- constants
 ESTH_OK = 0
+ESTH_ERR_INVALID = -1
+ESTH_ERR_BUDGET = -4
+ESTH_ERR_UNSUPPORTED = -5
 MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6, MODEL_UNICYCLE10, MODEL_ROBOTARM5_CAMERA = 0, 1, 2, 3, 4
 TOPO_NONE, TOPO_RING, TOPO_TORUS, TOPO_ALL2ALL = 0, 1, 2, 3
 RESAMPLE_MULTINOMIAL, RESAMPLE_SYSTEMATIC, RESAMPLE_ALIAS = 0, 1, 2
@@ -223,6 +226,7 @@ _SIGS = {
     "esth_stage_boxmuller": ([_P, C.c_int], C.c_int),
     "esth_stage_sample_weight": ([C.c_int, _P, C.c_uint32, _P, _P, C.c_int, _P], C.c_int),
     "esth_stage_sort": ([_P, _P, _P, C.c_int, C.c_uint32, C.c_uint32, _P, _P], C.c_int),
+    "esth_stage_sort_noalloc": ([_P, _P, _P, C.c_int, C.c_uint32, C.c_uint32, _P, _P, _P], C.c_int),
     "esth_stage_argmax": ([_P, C.c_uint32, C.c_int, _P], C.c_int),
     "esth_stage_exchange": ([_P, _P, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int],
                             C.c_int),

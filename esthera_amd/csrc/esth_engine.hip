@@ -38,6 +38,8 @@ static const uint32_t g_mtgp_table[ESTH_MTGP_SETS * ESTH_MTGP_PARAM_WORDS] = {
 const uint32_t* esth_param_table() { return g_mtgp_table; }
 extern "C" const uint32_t* esth_mtgp_params(void) { return g_mtgp_table; }
 
+static int esth_noalloc_scratch(float** out);
+
 /* mtgp32_init_state, mtgp32-fast.c:119-137 */
 extern "C" int esth_mtgp_seed_state(int set, uint32_t seed, uint32_t* st) {
     if (set < 0 || set >= ESTH_MTGP_SETS || !st) return esth_fail(ESTH_ERR_INVALID, "bad MTGP set %d", set);
@@ -220,6 +222,9 @@ extern "C" int esth_engine_create(int device, size_t budget, esth_engine** out) 
     if (err != hipSuccess) { delete e; return esth_fail(ESTH_ERR_DEVICE, "hipMalloc params: %s", hipGetErrorString(err)); }
     err = hipMemcpy(e->d_params, g_mtgp_table, sizeof(g_mtgp_table), hipMemcpyHostToDevice);
     if (err != hipSuccess) { hipFree(e->d_params); delete e; return esth_fail(ESTH_ERR_DEVICE, "upload params: %s", hipGetErrorString(err)); }
+    float* sc = nullptr;
+    int rc = esth_noalloc_scratch(&sc);   /* this device's allocation-free scratch address */
+    if (rc != ESTH_OK) { hipFree(e->d_params); delete e; return rc; }
     *out = e;
     return ESTH_OK;
 }
@@ -739,6 +744,60 @@ extern "C" int esth_stage_sort(float* d_w, const float* d_in, float* d_out, int 
     hipFree(part);
     if (win) hipFree(win);
     return rc;
+}
+
+/* ----------------------------------------- allocation-free block sort
+ * esth_stage_sort_noalloc runs esth_stage_sort's kernel with its scratch (the
+ * per-sub-filter partial sums, G (2 + S) floats, and the winners, 2 G floats,
+ * when the caller passes none) in this file-scope device array instead of
+ * hipMalloc'd buffers.  Each GPU holds its own copy of the array, at its own
+ * address, for the life of the process; host code reaches it only through
+ * the address hipGetSymbolAddress returns for the current device, kept per
+ * device in g_scratch_dev (the array's own name in host code is not a device
+ * address).  The sort writes every scratch word it later reads, so nothing
+ * needs clearing between calls.  Calls that overlap on one GPU share the
+ * array and race. */
+#define ESTH_NOALLOC_SCRATCH_FLOATS (1u << 19)   /* 2 MiB per GPU */
+#define ESTH_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) float scratch[ESTH_NOALLOC_SCRATCH_FLOATS];
+static float* g_scratch_dev[ESTH_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+/* the current device's scratch address; looked up once per device (at
+ * engine creation, or at the first allocation-free call on that device) */
+static int esth_noalloc_scratch(float** out) {
+    int dev = -1;
+    ESTH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= ESTH_NOALLOC_MAX_DEVICES) return esth_fail(ESTH_ERR_UNSUPPORTED, "device %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return esth_fail(ESTH_ERR_DEVICE, "scratch address on device %d: %s", dev, hipGetErrorString(err));
+        g_scratch_dev[dev] = (float*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return ESTH_OK;
+}
+
+extern "C" int esth_stage_sort_noalloc(float* d_w, const float* d_in, float* d_out, int S, uint32_t m, uint32_t G,
+                                       float* d_winner, int* d_perm, void* stream) {
+    if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES) return esth_fail(ESTH_ERR_INVALID, "m=%u", m);
+    if (G == 0 || G > ESTH_MAX_NUM_BLOCKS) return esth_fail(ESTH_ERR_INVALID, "G=%u", G);
+    if (S != 1 && S != ESTH_UNI_S && S != ESTH_RA5_S && S != ESTH_DH6_S)
+        return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+    if (!d_w || !d_in || !d_out) return esth_fail(ESTH_ERR_INVALID, "null buffer");
+    const uint64_t need = (uint64_t)G * (2 + S) + (d_winner ? 0 : 2ull * G);
+    if (need > ESTH_NOALLOC_SCRATCH_FLOATS)
+        return esth_fail(ESTH_ERR_BUDGET, "allocation-free sort: G=%u, S=%d needs %llu scratch floats, at most %u",
+                         G, S, (unsigned long long)need, ESTH_NOALLOC_SCRATCH_FLOATS);
+    float* sc = nullptr;
+    ESTH_TRY(esth_noalloc_scratch(&sc));
+    float* part = sc;
+    float2* win = d_winner ? (float2*)d_winner : (float2*)(sc + (size_t)G * (2 + S));
+    return esth_launch_sort(S, m, d_w, d_in, d_out, m * G, G, win, d_perm, 0, nullptr, nullptr, part, 0,
+                            ESTH_LAYOUT_SOA, (hipStream_t)stream);
 }
 
 extern "C" int esth_stage_argmax(const float* d_winner, uint32_t G, int weights, float* d_result) {

@@ -520,6 +520,18 @@ int esth_stage_sample_weight(int model, float* d_state, uint32_t N, const float*
  * if d_perm != NULL, the permutation. */
 int esth_stage_sort(float* d_w, const float* d_state_in, float* d_state_out, int state_dim,
                     uint32_t m, uint32_t G, float* d_winner, int* d_perm);
+/* Allocation-free variant of esth_stage_sort.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same inputs, outputs and results as esth_stage_sort; the kernel's scratch
+ * (per-sub-filter partial sums and, when d_winner is NULL, the winners) lives
+ * in a 2 MiB array that each GPU holds for the life of the process, so it
+ * needs G * (2 + S) floats (+ 2 G without d_winner) <= 524288 and fails with
+ * ESTH_ERR_BUDGET beyond that.  Enqueued on `stream` (a hipStream_t; NULL =
+ * the null stream) without waiting; it calls no allocator.  Calls that
+ * overlap on one GPU share that GPU's array and race. */
+int esth_stage_sort_noalloc(float* d_w, const float* d_state_in, float* d_state_out, int state_dim,
+                            uint32_t m, uint32_t G, float* d_winner, int* d_perm, void* stream);
 /* reduce (reduce.cl:19-55): argmax over G float2 winners -> d_result float2. */
 int esth_stage_argmax(const float* d_winner, uint32_t G, int weights, float* d_result);
 /* exchange of best particles on sorted sub-filters (in place on d_state/d_w). */
