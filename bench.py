@@ -37,6 +37,9 @@ S = 3
 SS_BYTES_PER_PARTICLE = 40
 SS_BYTES_PER_SUB = 4 + 8 + 4 * (2 + S) + 4 * T * (1 + S)
 SS_BYTES_PER_LAUNCH = SS_BYTES_PER_PARTICLE * M * G + SS_BYTES_PER_SUB * G
+# k_sample_weight alone: state read 12 + uniforms read 12 + state write 12 +
+# log-weight write 4 bytes per particle
+SW_BYTES_PER_LAUNCH = 40 * M * G
 STAGES = ["rng_wait", "sample_weight", "sort", "argmax", "exchange_or_topt", "resample"]
 
 
@@ -51,8 +54,35 @@ def stage_dict(us, fused):
     return d
 
 
+def settle(ea, eng, seconds):
+    """Untimed steps of the headline filter (a filter of its own) for about
+    `seconds`: the GPU raises its engine clock over the first tens of
+    milliseconds of load, and a measurement of a few steps taken before that
+    measures the clock ramp (0.129 against 0.113 ms per step at 20 steps)."""
+    if seconds <= 0:
+        return
+    import numpy as np
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=G, t=T, topology=ea.TOPO_TORUS, resampler=ea.RESAMPLE_SYSTEMATIC,
+                          weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
+    params, ctrl, meas, truth = ea.unicycle_scenario(42, L, 50.0, 64)
+    f = ea.Filter(eng, desc, params)
+    f.init_uniform(np.zeros(3, np.float32))
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(32):
+            f.step(ctrl[k % 64], meas[k % 64], 0.1)
+            k += 1
+        f.sync()
+    f.close()
+
+
 def run_plain(ea, eng, resampler, steps, warmup, graph=False, timing=1, every=1, landmarks=L, box=50.0,
               fusion=True):
+    """One pass of the headline filter: `warmup` untimed steps, then `steps`
+    timed ones.  timing 0: no events in the stream (the throughput pass);
+    1: events around the sample+weight(+sort) launch of one step in `every`;
+    2: events around every stage."""
     desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=G, t=T, topology=ea.TOPO_TORUS, resampler=resampler,
                           weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
     params, ctrl, meas, truth = ea.unicycle_scenario(42, landmarks, box, warmup + steps)
@@ -66,14 +96,14 @@ def run_plain(ea, eng, resampler, steps, warmup, graph=False, timing=1, every=1,
     for k in range(warmup):
         f.step(ctrl[k], meas[k], 0.1)
     ea.device_sync()
-    if not graph:
+    if not graph and timing:
         f.set_timing(timing, every)
     t0 = time.perf_counter()
     for k in range(warmup, warmup + steps):
         f.step(ctrl[k], meas[k], 0.1)
     ea.device_sync()
     t1 = time.perf_counter()
-    stage = [float(x) for x in f.stage_times()] if not graph else [float("nan")] * 6
+    stage = [float(x) for x in f.stage_times()] if (timing and not graph) else [float("nan")] * 6
     est = f.estimate()
     k = warmup + steps - 1
     err = float(np.hypot(est.mean[0] - truth[k, 0], est.mean[1] - truth[k, 1]))
@@ -130,35 +160,156 @@ def run_accuracy(ea, eng, steps=500):
             "max_error_posterior_mean": float(np.max(em))}
 
 
-def cpu_baseline(budget_s=15.0, max_steps=200):
+def run_accuracy_headline(ea, eng, steps=1000):
+    """The 2M-particle headline filter over its 1000 steps against the seeded
+    ground truth: RMSE of the posterior mean (position, heading) and of the
+    best particle's position."""
+    import numpy as np
+    d = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=G, t=T, topology=ea.TOPO_TORUS, resampler=ea.RESAMPLE_SYSTEMATIC,
+                       weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
+    p, ctrl, meas, truth = ea.unicycle_scenario(42, L, 50.0, steps)
+    f = ea.Filter(eng, d, p)
+    f.init_uniform(np.zeros(3, np.float32))
+    ep, eh, eb = [], [], []
+    for k in range(steps):
+        f.step(ctrl[k], meas[k], 0.1)
+        e = f.estimate()
+        ep.append(np.hypot(e.mean[0] - truth[k, 0], e.mean[1] - truth[k, 1]))
+        eh.append(np.angle(np.exp(1j * (e.mean[2] - truth[k, 2]))))
+        eb.append(np.hypot(e.best[0] - truth[k, 0], e.best[1] - truth[k, 1]))
+    f.close()
+    return {"workload": f"unicycle {G} x {M}, {L} landmarks, 64x64 torus t={T}, systematic, seed 42, {steps} steps",
+            "rmse_position_posterior_mean": float(np.sqrt(np.mean(np.square(ep)))),
+            "rmse_heading_posterior_mean": float(np.sqrt(np.mean(np.square(eh)))),
+            "rmse_position_best_particle": float(np.sqrt(np.mean(np.square(eb)))),
+            "max_position_error_posterior_mean": float(np.max(ep))}
+
+
+DH_M, DH_G, DH_T = 512, 2048, 1
+# k_sample_weight<dh6>: state read 48 + uniforms read 24 (6 per particle) +
+# state write 48 + log-weight write 4 bytes per particle
+DH_BYTES_PER_PARTICLE = 124
+
+
+def run_dh6(ea, eng, steps, warmup, acc_steps=2000):
+    """The robot-arm configuration: 6-joint DH arm (seed 7), 2048 sub-filters x
+    512 = 1,048,576 particles on the 32 x 64 torus, t = 1, systematic
+    resampling, dt = 0.01.  Throughput (untimed stream), the sample+weight
+    kernel's time from events around it on every step of a separate pass, and
+    the RMSE of the posterior mean against the seeded truth over 2000 steps."""
+    import numpy as np
+    d = ea.filter_desc(ea.MODEL_DH6, m=DH_M, G=DH_G, t=DH_T, topology=ea.TOPO_TORUS,
+                       resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, seed=7)
+    ksteps = min(max(steps, 50), 200)
+    n = max(steps, ksteps, acc_steps) + warmup
+    p, meas, truth = ea.dh6_scenario(7, n)
+    c = ea.DH6Control()
+    N = DH_M * DH_G
+
+    def one(timed_steps, timing):
+        f = ea.Filter(eng, d, p)
+        f.init_uniform(truth[0])
+        for k in range(warmup):
+            f.step(c, meas[k], 0.01)
+        f.sync()
+        if timing:
+            f.set_timing(1, 1)
+        t0 = time.perf_counter()
+        for k in range(warmup, warmup + timed_steps):
+            f.step(c, meas[k], 0.01)
+        f.sync()
+        secs = time.perf_counter() - t0
+        us = float(f.stage_times()[1]) if timing else None
+        f.close()
+        return secs, us
+
+    secs, _ = one(steps, False)
+    _, kus = one(ksteps, True)
+    f = ea.Filter(eng, d, p)
+    f.init_uniform(truth[0])
+    eq, ev = [], []
+    for k in range(acc_steps):
+        f.step(c, meas[k], 0.01)
+        e = f.estimate()
+        mean = np.array(e.mean[:12])
+        eq.append(np.sqrt(np.mean(np.square(mean[:6] - truth[k + 1, :6]))))
+        ev.append(np.sqrt(np.mean(np.square(mean[6:] - truth[k + 1, 6:]))))
+    f.close()
+    a = DH_BYTES_PER_PARTICLE * N / (kus * 1e-6) / 1e9
+    return {"workload": f"dh6 arm (seed 7), {DH_G} sub-filters x {DH_M} = {N} particles, 32x64 torus t={DH_T}, "
+                        f"systematic resampling, dt 0.01",
+            "particles": N, "steps": steps, "ms_per_step": secs * 1e3 / steps, "particles_per_s": N * steps / secs,
+            "roofline": {"bound": "hbm", "kernel": "k_sample_weight<dh6, SOA>", "kernel_us": kus,
+                         "bytes_per_launch": DH_BYTES_PER_PARTICLE * N, "achieved": a, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": a / HBM_PEAK_GBS},
+            "accuracy": {"steps": acc_steps,
+                         "rmse_joint_angle_rad": float(np.sqrt(np.mean(np.square(eq)))),
+                         "rmse_joint_velocity_rad_s": float(np.sqrt(np.mean(np.square(ev))))}}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(warm=3, reps=5, rep_s=3.0):
     """The oracle's FP32 restatement of the reference step (the same MTGP
-    stream, bitonic network, scan and resampler), OpenMP over sub-filters, on
-    a bounded sample of the workload: the full 4096 x 512 filter for as many
-    steps as fit in about `budget_s` seconds."""
+    stream, bitonic network, scan and resampler), OpenMP over sub-filters,
+    built here with -O3 -march=native (make -C oracle baseline), on a bounded
+    sample of the workload: the full 4096 x 512 filter, 3 untimed steps, then
+    5 repetitions of the step count that takes about `rep_s` seconds; the
+    median repetition is reported."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import statistics
+    import subprocess
+    import tempfile
+
     import numpy as np
 
     import esthera_amd as ea
     import oracle
+    out = tempfile.mkdtemp(prefix="esth_cpu_baseline_")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "baseline", f"OUT={out}"],
+                       capture_output=True, text=True)
+    native = os.path.join(out, "liboracle_native.so")
+    build = "gcc -O3 -march=native -fopenmp (oracle/Makefile baseline)"
+    if r.returncode != 0 or not os.path.exists(native):
+        native, build = None, "the committed -O2 oracle build (the native build failed)"
+    olib = oracle.use_library(native) if native else oracle.lib()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    oracle.lib().or_set_threads(threads)
-    Gs = G
-    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=Gs, t=T, topology=ea.TOPO_TORUS,
+    olib.or_set_threads(threads)
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=M, G=G, t=T, topology=ea.TOPO_TORUS,
                           resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_COMPAT, seed=42)
-    params, ctrl, meas, truth = ea.unicycle_scenario(42, L, 50.0, max_steps + 1)
+    nrow = 400
+    params, ctrl, meas, truth = ea.unicycle_scenario(42, L, 50.0, nrow)
     o = oracle.Filter(desc, params, ctypes.sizeof(params))
     o.set_f64(False)
     o.init_uniform(np.zeros(3, np.float32))
-    o.step(ctrl[0], meas[0], 0.1)
-    n = 0
+    k = 0
     t0 = time.perf_counter()
-    while n < max_steps and (time.perf_counter() - t0) < budget_s:
-        o.step(ctrl[n + 1], meas[n + 1], 0.1)
-        n += 1
-    dt = time.perf_counter() - t0
-    return {"value": M * Gs * n / dt, "unit": "particles/s", "cores": threads, "kind": "port",
-            "sample": f"{n} steps (after 1 untimed) of the full 4096 x 512 workload, FP32 oracle "
-                      f"restatement, OpenMP {threads} threads, {dt:.1f} s"}
+    for _ in range(warm):
+        o.step(ctrl[k % nrow], meas[k % nrow], 0.1)
+        k += 1
+    per_step = (time.perf_counter() - t0) / warm
+    n = max(1, int(rep_s / per_step))
+    rates = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            o.step(ctrl[k % nrow], meas[k % nrow], 0.1)
+            k += 1
+        rates.append(M * G * n / (time.perf_counter() - t0))
+    return {"value": statistics.median(rates), "unit": "particles/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "nproc": os.cpu_count(), "build": build,
+            "sample": f"full {G} x {M} workload, {warm} untimed steps, then the median of {reps} repetitions of "
+                      f"{n} steps (rates {min(rates):.3g}..{max(rates):.3g} particles/s), FP32 oracle "
+                      f"restatement, OpenMP {threads} threads"}
 
 
 def run_batched(ea, eng, steps=200, warmup=3, nf=10000, plain_every=20):
@@ -169,28 +320,18 @@ def run_batched(ea, eng, steps=200, warmup=3, nf=10000, plain_every=20):
     step.  Beside it, every `plain_every`-th filter stepped as its own filter
     (per-filter launches), and the two compared filter by filter."""
     import numpy as np
-    rng = np.random.default_rng(99)
-    counts = np.maximum((np.round(np.exp(rng.uniform(np.log(256), np.log(65536), nf)) / 64) * 64).astype(np.int64), 256)
-    NS = 64
-    scen = [ea.unicycle_scenario(1000 + j, 8, 20.0, 8) for j in range(NS)]
+
+    from esthera_amd import workloads
+    scen = workloads.batched_scenarios()
+    NS = len(scen)
     p0, c0, m0, _ = scen[0]
     d0 = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=4, t=1, topology=ea.TOPO_RING, resampler=ea.RESAMPLE_SYSTEMATIC,
                         weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=1)
     us = ea.autotune(eng, d0, p0, c0[0], m0[0], 0.1)
-    descs, params = [], []
-    for i, n in enumerate(counts):
-        m = ea.autotune_pick(us, int(n), 64)
-        if int(n) // m < 2:      # every filter keeps at least two sub-filters to exchange between
-            m //= 2
-        Gi = int(n) // m
-        descs.append(ea.filter_desc(ea.MODEL_UNICYCLE, m=m, G=Gi, t=1, topology=ea.TOPO_RING,
-                                    resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG,
-                                    rng=ea.RNG_MTGP_PER_FILTER, seed=99 * 1000003 + i))
-        params.append(scen[i % NS][0])
+    counts, descs, params, scen = workloads.batched_filters(us, nf=nf, scen=scen)
     b = ea.Batch(eng, descs, params)
     b.init_uniform(np.zeros((nf, 3), np.float32))
-    IN = [((ea.UnicycleControl * nf)(*[scen[i % NS][1][k] for i in range(nf)]),
-           (ea.UnicycleMeasurement * nf)(*[scen[i % NS][2][k] for i in range(nf)])) for k in range(8)]
+    IN = [workloads.batched_inputs(scen, nf, k) for k in range(workloads.BATCH_SCENARIO_STEPS)]
     dts = np.full(nf, 0.1, np.float32)
     for k in range(warmup):
         b.step(IN[k % 8][0], IN[k % 8][1], dts)
@@ -338,6 +479,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 10k-filter batched workload")
     ap.add_argument("--no-served", action="store_true", help="skip the served request-stream workload")
     ap.add_argument("--no-layouts", action="store_true", help="skip the state-layout comparison")
+    ap.add_argument("--no-dh6", action="store_true", help="skip the robot-arm (dh6, 2048 x 512) workload")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy passes")
+    ap.add_argument("--settle", type=float, default=0.3,
+                    help="seconds of untimed headline steps before the warmup (engine clock ramp)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -361,27 +506,30 @@ def main():
         gsecs, _, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, graph=True)
         print(json.dumps({"graph_ms_per_step": gsecs * 1e3 / args.steps}), flush=True)
         return
-    # timed region: eager steps; HIP events bracket the sample+weight+sort
-    # launch on every 25th step (every 10th below 250 steps; each event costs
-    # the stream a few microseconds)
-    every = 25 if args.steps >= 250 else 10
-    secs, stage, err = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, every=every)
+    settle(ea, eng, args.settle)
+    # timed region: eager steps, no events in the stream
+    secs, _, err = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, timing=0)
     value = N * args.steps / secs
+    # the dominant launch's own time: a separate pass of 400 steps with events
+    # around the sample+weight+sort launch of every 10th step (40 launches;
+    # each event pair costs the stream a few microseconds and delays the
+    # launch after it, so the throughput pass above carries none)
+    ksteps, kevery = 400, 10
+    _, kstage, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, ksteps, args.warmup, timing=1, every=kevery)
+    sw_us = float(kstage[1])
+    achieved = SS_BYTES_PER_LAUNCH / (sw_us * 1e-6) / 1e9
     if args.main_only:
-        sw_us = stage[1]
-        a = SS_BYTES_PER_LAUNCH / (sw_us * 1e-6) / 1e9
         print(json.dumps({"ms_per_step": secs * 1e3 / args.steps, "value": value, "kernel_us": sw_us,
-                          "achieved_gbs": a}), flush=True)
+                          "kernel_timed_launches": ksteps // kevery, "achieved_gbs": achieved}), flush=True)
         eng.close()
         return
     # separate short pass with events around every stage (adds idle gaps;
     # breakdown only; per-stage timing runs sample+weight and sort as two launches)
     _, stage_all, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 100), args.warmup, timing=2)
     # the same step with sample+weight and sort as two launches
-    usecs, ustage, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 300), args.warmup, every=10,
-                                 fusion=False)
-    sw_us = stage[1]
-    achieved = SS_BYTES_PER_LAUNCH / (sw_us * 1e-6) / 1e9
+    usecs, _, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 300), args.warmup, timing=0,
+                            fusion=False)
+    _, ustage, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, 100, args.warmup, timing=1, fusion=False)
     out = {
         "metric": METRIC,
         "value": value,
@@ -399,25 +547,33 @@ def main():
                                "64x64 torus t=4, systematic resampling, MTGP32 seed 42",
                    "particles": N, "sub_filters": G, "particles_per_sub_filter": M, "landmarks": L,
                    "parallelism": "single GPU"},
+        "settle_s": args.settle,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "traffic_source": "profiles/sample_sort_unicycle2m_traffic.json (rocprofv3 FETCH_SIZE + "
+                                       "WRITE_SIZE, calibrated by a device copy)",
                      "kernel": "k_sample_sort<unicycle, SOA, m=512> (sample+weight and block sort)",
-                     "kernel_us": sw_us, "bytes_per_launch": SS_BYTES_PER_LAUNCH,
+                     "kernel_us": sw_us, "kernel_timed_launches": ksteps // kevery, "bytes_per_launch": SS_BYTES_PER_LAUNCH,
                      "valu": valu_from_profiles(sw_us)},
         "stages_us": stage_dict(stage_all, fused=True),
-        "two_launches": {"ms_per_step": usecs * 1e3 / min(args.steps, 300), "sample_weight_us": ustage[1],
+        "two_launches": {"ms_per_step": usecs * 1e3 / min(args.steps, 300), "sample_weight_us": float(ustage[1]),
                          "note": "sample+weight and sort as separate launches (k_sample_weight, k_sort)"},
         "position_error_last_step": err,
     }
     # the same kernel where the likelihood is light: 4 landmarks at (+-10, +-10)
     # (the small unicycle workload's set) on the same 2M particles; sample+weight
     # is then HBM-bound and its HBM fraction measures the kernel's memory path
-    _, stage4, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, min(args.steps, 200), args.warmup, every=10,
-                             landmarks=4, box=0.0)
+    _, stage4, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, 100, args.warmup, timing=1, landmarks=4, box=0.0)
+    _, stage4u, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, 100, args.warmup, timing=1, landmarks=4, box=0.0,
+                              fusion=False)
     a4 = SS_BYTES_PER_LAUNCH / (stage4[1] * 1e-6) / 1e9
+    a4u = SW_BYTES_PER_LAUNCH / (stage4u[1] * 1e-6) / 1e9
     out["roofline"]["light_likelihood"] = {"workload": "same 2097152 particles, 4 landmarks", "kernel_us": stage4[1],
                                            "achieved": a4, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                           "frac": a4 / HBM_PEAK_GBS}
+                                           "frac": a4 / HBM_PEAK_GBS,
+                                           "sample_weight_alone": {"kernel": "k_sample_weight<unicycle, SOA>",
+                                                                   "kernel_us": float(stage4u[1]), "achieved": a4u,
+                                                                   "frac": a4u / HBM_PEAK_GBS}}
     gsecs, _, _ = run_plain(ea, eng, ea.RESAMPLE_SYSTEMATIC, args.steps, args.warmup, graph=True)
     out["graph_replay"] = {"value": N * args.steps / gsecs, "ms_per_step": gsecs * 1e3 / args.steps}
     if not args.no_alias:
@@ -425,7 +581,11 @@ def main():
         out["alias"] = {"value": N * max(1, args.steps // 2) / asecs, "unit": "particles/s",
                         "ms_per_step": asecs * 1e3 / max(1, args.steps // 2),
                         "stages_us": stage_dict(astage, fused=True)}
-    out["accuracy"] = run_accuracy(ea, eng)
+    if not args.no_accuracy:
+        out["accuracy"] = run_accuracy(ea, eng)
+        out["accuracy_headline"] = run_accuracy_headline(ea, eng)
+    if not args.no_dh6:
+        out["dh6"] = run_dh6(ea, eng, min(args.steps, 500), args.warmup)
     if not args.no_layouts:
         out["layouts"] = run_layouts(ea, eng)
     if not args.no_batched:

@@ -223,6 +223,7 @@ _SIGS = {
     "esth_mtgp_params": ([], C.POINTER(C.c_uint32)),
     "esth_mtgp_seed_state": ([C.c_int, C.c_uint32, _P], C.c_int),
     "esth_stage_mtgp": ([_P, C.c_int, C.c_int, C.c_int, _P, C.c_int], C.c_int),
+    "esth_stage_mtgp_steps": ([_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int], C.c_int),
     "esth_stage_boxmuller": ([_P, C.c_int], C.c_int),
     "esth_stage_sample_weight": ([C.c_int, _P, C.c_uint32, _P, _P, C.c_int, _P], C.c_int),
     "esth_stage_sort": ([_P, _P, _P, C.c_int, C.c_uint32, C.c_uint32, _P, _P], C.c_int),

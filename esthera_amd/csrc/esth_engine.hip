@@ -254,6 +254,18 @@ int esth_engine_alloc(esth_engine* e, size_t bytes, void** p) {
     return ESTH_OK;
 }
 
+int esth_engine_account(esth_engine* e, int64_t bytes) {
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (bytes > 0 && e->used + (size_t)bytes > e->budget)
+        return esth_fail(ESTH_ERR_BUDGET, "HBM budget: %zu used + %lld requested > %zu", e->used, (long long)bytes,
+                         e->budget);
+    if (bytes >= 0)
+        e->used += (size_t)bytes;
+    else
+        e->used -= (size_t)(-bytes);
+    return ESTH_OK;
+}
+
 void esth_engine_release(esth_engine* e, void* p, size_t bytes) {
     if (!p) return;
     std::lock_guard<std::mutex> lk(e->mu);
@@ -265,47 +277,68 @@ void esth_engine_release(esth_engine* e, void* p, size_t bytes) {
 static int grid_for(uint64_t threads, int block) { return (int)((threads + block - 1) / block); }
 
 int esth_launch_mtgp(const uint32_t* d_params, const uint32_t* d_status_in, uint32_t* d_status_out,
-                     const esth_mtgp_stream* d_streams, int nstreams, void* d_out, int float_mode, hipStream_t s) {
-    if (nstreams <= 0) return ESTH_OK;
+                     const esth_mtgp_stream* d_streams, int nstreams, void* d_out, int float_mode, hipStream_t s,
+                     int nsteps, uint64_t out_step, uint64_t status_step) {
+    if (nstreams <= 0 || nsteps <= 0) return ESTH_OK;
+    if (nsteps > 1 && d_status_in == d_status_out)
+        return esth_fail(ESTH_ERR_INVALID, "multi-step MTGP32 launch needs separate status buffers");
     const uint32_t mask = g_mtgp_table[3];
     hipLaunchKernelGGL(k_mtgp, dim3(nstreams), dim3(ESTH_MTGP_THREADS), 0, s, d_params, mask, d_status_in,
-                       d_status_out, d_streams, (uint32_t*)d_out, float_mode);
+                       d_status_out, d_streams, (uint32_t*)d_out, float_mode, nsteps, out_step, status_step);
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }
 
+template <int MODEL, int L>
+static void launch_sample_weight_l(int grid, float* state, uint32_t N, const float* R, const float* frame,
+                                   const esth_frame_arg* fa, int log_w, float* w, int carry, const esth_mtgp_inline& mi,
+                                   hipStream_t s) {
+    if (fa)
+        hipLaunchKernelGGL((k_sample_weight<MODEL, L, true>), dim3(grid + mi.n), dim3(256), 0, s, state, N, R, nullptr,
+                           *fa, log_w, w, carry, mi);
+    else
+        hipLaunchKernelGGL((k_sample_weight<MODEL, L, false>), dim3(grid + mi.n), dim3(256), 0, s, state, N, R, frame,
+                           esth_frame_arg{}, log_w, w, carry, mi);
+}
+
 template <int MODEL>
 static void launch_sample_weight_t(int layout, int grid, float* state, uint32_t N, const float* R, const float* frame,
-                                   int log_w, float* w, int carry, hipStream_t s) {
+                                   const esth_frame_arg* fa, int log_w, float* w, int carry, const esth_mtgp_inline& mi,
+                                   hipStream_t s) {
     if (layout == ESTH_LAYOUT_AOS)
-        hipLaunchKernelGGL((k_sample_weight<MODEL, ESTH_LAYOUT_AOS>), dim3(grid), dim3(256), 0, s, state, N, R, frame,
-                           log_w, w, carry);
+        launch_sample_weight_l<MODEL, ESTH_LAYOUT_AOS>(grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
     else if (layout == ESTH_LAYOUT_PACKED)
-        hipLaunchKernelGGL((k_sample_weight<MODEL, ESTH_LAYOUT_PACKED>), dim3(grid), dim3(256), 0, s, state, N, R,
-                           frame, log_w, w, carry);
+        launch_sample_weight_l<MODEL, ESTH_LAYOUT_PACKED>(grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
     else
-        hipLaunchKernelGGL((k_sample_weight<MODEL, ESTH_LAYOUT_SOA>), dim3(grid), dim3(256), 0, s, state, N, R, frame,
-                           log_w, w, carry);
+        launch_sample_weight_l<MODEL, ESTH_LAYOUT_SOA>(grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
 }
 
 int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame, int log_w,
-                              float* w, int carry, int layout, hipStream_t s) {
+                              float* w, int carry, int layout, hipStream_t s, const float* host_frame,
+                              const esth_mtgp_inline* mtgp) {
+    const esth_mtgp_inline mi = mtgp ? *mtgp : esth_mtgp_inline{};
+    esth_frame_arg fa_v;
+    const esth_frame_arg* fa = nullptr;
+    if (host_frame) {
+        memcpy(fa_v.v, host_frame, sizeof(fa_v.v));
+        fa = &fa_v;
+    }
     const int grid = grid_for(N / 2, 256);
     switch (model) {
     case ESTH_MODEL_ROBOTARM5:
-        launch_sample_weight_t<ESTH_MODEL_ROBOTARM5>(layout, grid, state, N, R, frame, log_w, w, carry, s);
+        launch_sample_weight_t<ESTH_MODEL_ROBOTARM5>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
         break;
     case ESTH_MODEL_UNICYCLE:
-        launch_sample_weight_t<ESTH_MODEL_UNICYCLE>(layout, grid, state, N, R, frame, log_w, w, carry, s);
+        launch_sample_weight_t<ESTH_MODEL_UNICYCLE>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
         break;
     case ESTH_MODEL_DH6:
-        launch_sample_weight_t<ESTH_MODEL_DH6>(layout, grid, state, N, R, frame, log_w, w, carry, s);
+        launch_sample_weight_t<ESTH_MODEL_DH6>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
         break;
     case ESTH_MODEL_UNICYCLE10:
-        launch_sample_weight_t<ESTH_MODEL_UNICYCLE10>(layout, grid, state, N, R, frame, log_w, w, carry, s);
+        launch_sample_weight_t<ESTH_MODEL_UNICYCLE10>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
         break;
     case ESTH_MODEL_ROBOTARM5_CAMERA:
-        launch_sample_weight_t<ESTH_MODEL_ROBOTARM5_CAMERA>(layout, grid, state, N, R, frame, log_w, w, carry, s);
+        launch_sample_weight_t<ESTH_MODEL_ROBOTARM5_CAMERA>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
         break;
     default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
     }
@@ -368,21 +401,27 @@ int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_
 
 template <int MODEL, int L, int LOGM>
 static void launch_sample_sort_m(float* state, uint32_t N, uint32_t G, const float* R, const float* frame,
-                                 int log_w, float* w, int carry, float* st_out, float2* winner, uint32_t t,
-                                 float* bt_w, float* bt_s, float* part, const float* gate, float keep_thr,
-                                 int keep_all, hipStream_t s) {
-    hipLaunchKernelGGL((k_sample_sort<MODEL, L, LOGM>), dim3(G), dim3(ESTH_WAVE * esth_ss_waves<LOGM>()), 0, s, state,
-                       N, G, R, frame,
-                       log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, keep_thr, keep_all);
+                                 const esth_frame_arg* fa, int log_w, float* w, int carry, float* st_out,
+                                 float2* winner, uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
+                                 float keep_thr, int keep_all, const esth_mtgp_inline& mi, hipStream_t s) {
+    if (fa)
+        hipLaunchKernelGGL((k_sample_sort<MODEL, L, LOGM, true>), dim3(G + mi.n),
+                           dim3(ESTH_WAVE * esth_ss_waves<LOGM>()), 0, s, state, N, G, R, nullptr, *fa, log_w, w, carry,
+                           st_out, winner, t, bt_w, bt_s, part, gate, keep_thr, keep_all, mi);
+    else
+        hipLaunchKernelGGL((k_sample_sort<MODEL, L, LOGM, false>), dim3(G + mi.n),
+                           dim3(ESTH_WAVE * esth_ss_waves<LOGM>()), 0, s, state, N, G, R, frame, esth_frame_arg{},
+                           log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, keep_thr, keep_all, mi);
 }
 
 template <int MODEL, int L>
 static int launch_sample_sort_l(int lm, float* state, uint32_t N, uint32_t G, const float* R,
-                                const float* frame, int log_w, float* w, int carry, float* st_out, float2* winner,
-                                uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
-                                float keep_thr, int keep_all, hipStream_t s) {
-#define ESTH_SS_ARGS state, N, G, R, frame, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, keep_thr, \
-                     keep_all, s
+                                const float* frame, const esth_frame_arg* fa, int log_w, float* w, int carry,
+                                float* st_out, float2* winner, uint32_t t, float* bt_w, float* bt_s, float* part,
+                                const float* gate, float keep_thr, int keep_all, const esth_mtgp_inline& mi,
+                                hipStream_t s) {
+#define ESTH_SS_ARGS state, N, G, R, frame, fa, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, \
+                     keep_thr, keep_all, mi, s
     switch (lm) {
     case 7: launch_sample_sort_m<MODEL, L, 7>(ESTH_SS_ARGS); break;
     case 8: launch_sample_sort_m<MODEL, L, 8>(ESTH_SS_ARGS); break;
@@ -396,11 +435,12 @@ static int launch_sample_sort_l(int lm, float* state, uint32_t N, uint32_t G, co
 
 template <int MODEL>
 static int launch_sample_sort_t(int layout, int lm, float* state, uint32_t N, uint32_t G, const float* R,
-                                const float* frame, int log_w, float* w, int carry, float* st_out, float2* winner,
-                                uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
-                                float keep_thr, int keep_all, hipStream_t s) {
-#define ESTH_SS_ARGS lm, state, N, G, R, frame, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, \
-                     keep_thr, keep_all, s
+                                const float* frame, const esth_frame_arg* fa, int log_w, float* w, int carry,
+                                float* st_out, float2* winner, uint32_t t, float* bt_w, float* bt_s, float* part,
+                                const float* gate, float keep_thr, int keep_all, const esth_mtgp_inline& mi,
+                                hipStream_t s) {
+#define ESTH_SS_ARGS lm, state, N, G, R, frame, fa, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, \
+                     keep_thr, keep_all, mi, s
     if (layout == ESTH_LAYOUT_AOS) return launch_sample_sort_l<MODEL, ESTH_LAYOUT_AOS>(ESTH_SS_ARGS);
     if (layout == ESTH_LAYOUT_PACKED) return launch_sample_sort_l<MODEL, ESTH_LAYOUT_PACKED>(ESTH_SS_ARGS);
     return launch_sample_sort_l<MODEL, ESTH_LAYOUT_SOA>(ESTH_SS_ARGS);
@@ -408,6 +448,8 @@ static int launch_sample_sort_t(int layout, int lm, float* state, uint32_t N, ui
 }
 
 bool esth_sample_sort_fits(uint32_t m) { return m >= 128 && m <= 1024 && (m & (m - 1)) == 0; }
+/* 256-thread work-groups (4 waves) with LDS for an MTGP32 ring: m >= 512 */
+bool esth_sample_sort_inline_rng(uint32_t m) { return esth_sample_sort_fits(m) && m >= 512; }
 
 /* resident waves per SIMD of a kernel launched with `threads`-thread
  * work-groups: 512 VGPRs per lane slot (granule 8), at most 8 waves, and
@@ -431,16 +473,16 @@ template <int MODEL, int L>
 static bool sample_sort_pays_l(int lm) {
     const void* fn = nullptr;
     switch (lm) {
-    case 7: fn = (const void*)&k_sample_sort<MODEL, L, 7>; break;
-    case 8: fn = (const void*)&k_sample_sort<MODEL, L, 8>; break;
-    case 9: fn = (const void*)&k_sample_sort<MODEL, L, 9>; break;
-    case 10: fn = (const void*)&k_sample_sort<MODEL, L, 10>; break;
+    case 7: fn = (const void*)&k_sample_sort<MODEL, L, 7, true>; break;
+    case 8: fn = (const void*)&k_sample_sort<MODEL, L, 8, true>; break;
+    case 9: fn = (const void*)&k_sample_sort<MODEL, L, 9, true>; break;
+    case 10: fn = (const void*)&k_sample_sort<MODEL, L, 10, true>; break;
     default: return false;
     }
     const int ss_waves[11] = {0, 0, 0, 0, 0, 0, 0, esth_ss_waves<7>(), esth_ss_waves<8>(), esth_ss_waves<9>(),
                               esth_ss_waves<10>()};
     const int fused = esth_waves_per_simd(fn, ESTH_WAVE * ss_waves[lm]);
-    return fused > 0 && fused >= esth_waves_per_simd((const void*)&k_sample_weight<MODEL, L>, 256);
+    return fused > 0 && fused >= esth_waves_per_simd((const void*)&k_sample_weight<MODEL, L, true>, 256);
 }
 
 template <int MODEL>
@@ -469,12 +511,22 @@ bool esth_sample_sort_pays(int model, int layout, uint32_t m) {
 int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uint32_t G, const float* R,
                             const float* frame, int log_w, float* w, int carry, int layout, float* st_out,
                             float2* winner, uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
-                            float keep_thr, int keep_all, hipStream_t s) {
+                            float keep_thr, int keep_all, hipStream_t s, const float* host_frame,
+                            const esth_mtgp_inline* mtgp) {
     if (!esth_sample_sort_fits(m) || (uint64_t)m * G != N)
         return esth_fail(ESTH_ERR_UNSUPPORTED, "fused sample+sort needs 128 <= m <= 1024 and N = m G");
     const int lm = log2u(m);
-#define ESTH_SS_ARGS layout, lm, state, N, G, R, frame, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, gate, \
-                     keep_thr, keep_all, s
+    const esth_mtgp_inline mi = mtgp ? *mtgp : esth_mtgp_inline{};
+    if (mi.n && !esth_sample_sort_inline_rng(m))
+        return esth_fail(ESTH_ERR_UNSUPPORTED, "MTGP32 work-groups ride only in 256-thread sample+sort launches");
+    esth_frame_arg fa_v;
+    const esth_frame_arg* fa = nullptr;
+    if (host_frame) {
+        memcpy(fa_v.v, host_frame, sizeof(fa_v.v));
+        fa = &fa_v;
+    }
+#define ESTH_SS_ARGS layout, lm, state, N, G, R, frame, fa, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, \
+                     gate, keep_thr, keep_all, mi, s
     int rc;
     switch (model) {
     case ESTH_MODEL_ROBOTARM5: rc = launch_sample_sort_t<ESTH_MODEL_ROBOTARM5>(ESTH_SS_ARGS); break;
@@ -706,6 +758,28 @@ extern "C" int esth_stage_mtgp(uint32_t* d_status, int nstreams, int first_set, 
     return rc;
 }
 
+extern "C" int esth_stage_mtgp_steps(const uint32_t* d_status_in, uint32_t* d_status_out, int nstreams, int first_set,
+                                     int size, int nsteps, void* d_out, int float_mode) {
+    if (nstreams <= 0 || first_set < 0 || first_set + nstreams > ESTH_MTGP_SETS)
+        return esth_fail(ESTH_ERR_INVALID, "streams %d from set %d", nstreams, first_set);
+    if (size <= 0 || size % ESTH_MTGP_LARGE) return esth_fail(ESTH_ERR_INVALID, "size %d not a multiple of 768", size);
+    if (nsteps <= 0 || !d_status_in || !d_status_out || !d_out) return esth_fail(ESTH_ERR_INVALID, "steps/buffers");
+    std::vector<esth_mtgp_stream> hs(nstreams);
+    for (int j = 0; j < nstreams; ++j) hs[j] = {(uint32_t)(first_set + j), (uint32_t)size, (uint64_t)j * (uint64_t)size, (uint64_t)j};
+    esth_mtgp_stream* ds = nullptr;
+    uint32_t* dp = nullptr;
+    ESTH_HIP(hipMalloc((void**)&ds, sizeof(esth_mtgp_stream) * nstreams));
+    ESTH_HIP(hipMalloc((void**)&dp, sizeof(g_mtgp_table)));
+    ESTH_HIP(hipMemcpy(ds, hs.data(), sizeof(esth_mtgp_stream) * nstreams, hipMemcpyHostToDevice));
+    ESTH_HIP(hipMemcpy(dp, g_mtgp_table, sizeof(g_mtgp_table), hipMemcpyHostToDevice));
+    int rc = esth_launch_mtgp(dp, d_status_in, d_status_out, ds, nstreams, d_out, float_mode, 0, nsteps,
+                              (uint64_t)nstreams * (uint64_t)size, (uint64_t)nstreams * ESTH_MTGP_N);
+    ESTH_HIP(hipDeviceSynchronize());
+    hipFree(ds);
+    hipFree(dp);
+    return rc;
+}
+
 extern "C" int esth_stage_boxmuller(float* d, int n) {
     if (n <= 0) return ESTH_OK;
     hipLaunchKernelGGL(k_boxmuller, dim3(grid_for((uint64_t)(n + 1) / 2, 256)), dim3(256), 0, 0, d, n);
@@ -718,13 +792,10 @@ extern "C" int esth_stage_sample_weight(int model, float* d_state, uint32_t N, c
                                         int weights, float* d_w) {
     if (N == 0 || (N & 1u)) return esth_fail(ESTH_ERR_INVALID, "N=%u must be even and positive", N);
     if (!frame) return esth_fail(ESTH_ERR_INVALID, "null frame");
-    float* dfr = nullptr;
-    ESTH_HIP(hipMalloc((void**)&dfr, sizeof(float) * ESTH_FRAME_FLOATS));
-    ESTH_HIP(hipMemcpy(dfr, frame, sizeof(float) * ESTH_FRAME_FLOATS, hipMemcpyHostToDevice));
-    int rc = esth_launch_sample_weight(model, d_state, N, d_random, dfr, weights == ESTH_WEIGHTS_LOG, d_w, 0,
-                                       ESTH_LAYOUT_SOA, 0);
+    /* the frame travels as a kernel argument */
+    int rc = esth_launch_sample_weight(model, d_state, N, d_random, nullptr, weights == ESTH_WEIGHTS_LOG, d_w, 0,
+                                       ESTH_LAYOUT_SOA, 0, frame);
     ESTH_HIP(hipDeviceSynchronize());
-    hipFree(dfr);
     return rc;
 }
 

@@ -42,6 +42,8 @@ struct esth_engine {
 };
 
 int esth_engine_alloc(esth_engine* e, size_t bytes, void** p);
+/* charge (bytes > 0) or return (bytes < 0) budget without allocating */
+int esth_engine_account(esth_engine* e, int64_t bytes);
 void esth_engine_release(esth_engine* e, void* p, size_t bytes);
 int esth_check_desc(const esth_filter_desc* d, int* S, int* V);
 int esth_params_size(int model, size_t* sz);
@@ -49,9 +51,13 @@ const uint32_t* esth_param_table();
 
 /* launchers shared by filters, batches and stage entry points */
 int esth_launch_mtgp(const uint32_t* d_params, const uint32_t* d_status_in, uint32_t* d_status_out,
-                     const esth_mtgp_stream* d_streams, int nstreams, void* d_out, int float_mode, hipStream_t s);
+                     const esth_mtgp_stream* d_streams, int nstreams, void* d_out, int float_mode, hipStream_t s,
+                     int nsteps = 1, uint64_t out_step = 0, uint64_t status_step = 0);
+/* frame: the step's frame in device memory; host_frame (non-null): the
+ * frame passed by value as a kernel argument instead (frame unused) */
 int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* R, const float* frame,
-                              int log_w, float* w, int carry, int layout, hipStream_t s);
+                              int log_w, float* w, int carry, int layout, hipStream_t s,
+                              const float* host_frame = nullptr, const esth_mtgp_inline* mtgp = nullptr);
 int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_out, uint32_t N,
                      uint32_t G, float2* winner, int* perm, uint32_t t, float* bt_w, float* bt_s,
                      float* part, int log_w, int layout, hipStream_t s);
@@ -71,7 +77,10 @@ bool esth_sample_sort_pays(int model, int layout, uint32_t m);   /* fits, and co
 int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uint32_t G, const float* R,
                             const float* frame, int log_w, float* w, int carry, int layout, float* st_out,
                             float2* winner, uint32_t t, float* bt_w, float* bt_s, float* part, const float* gate,
-                            float keep_thr, int keep_all, hipStream_t s);
+                            float keep_thr, int keep_all, hipStream_t s,
+                            const float* host_frame = nullptr, const esth_mtgp_inline* mtgp = nullptr);
+/* whether esth_launch_sample_sort can carry MTGP32 work-groups at this m */
+bool esth_sample_sort_inline_rng(uint32_t m);
 int esth_launch_argmax(int S, const float2* winner, uint32_t G, int log_w, const float* part,
                        const float* sorted, uint32_t N, uint32_t m, float2* result, double* est,
                        hipStream_t s);

@@ -104,17 +104,30 @@ __device__ __forceinline__ float esth_bfly_max(float v) {
  * produces 768 words in three 256-wide phases, each one the recursion
  * x[n] = rec(x[n-351], x[n-350], x[n-351+pos]) for 256 consecutive n
  * (mtgp32-opencl.cl:186-242).  Output is the sequential stream of the
- * stream's parameter set, continuing across launches through `status`
- * (status_in may equal status_out: each work-group reads its status into LDS
- * before the first barrier and writes it back after the last). */
-static __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ params, uint32_t mask,
-                                              const uint32_t* status_in, uint32_t* status_out,
-                                              const esth_mtgp_stream* __restrict__ streams,
-                                              uint32_t* __restrict__ out, int float_mode) {
-    __shared__ uint32_t st[ESTH_MTGP_LARGE];
-    __shared__ uint32_t tbl[16], ttbl[16];
+ * stream's parameter set, continuing across launches through the status.
+ *
+ * One launch generates `nsteps` consecutive steps' words: step s's `size`
+ * words of stream j go to out + s * out_step + out_off(j), and the status
+ * after step s (the 351 words that generate step s + 1) to status_out +
+ * s * status_step + 351 * status(j).  status_in may equal status_out when
+ * nsteps is 1: each work-group reads its status into LDS before the first
+ * barrier and writes it back after the last.
+ *
+ * Every value a phase reads is older than the phase: x[n-351+pos] with
+ * pos <= 93 < 96 for all 200 parameter sets, and the tempering's
+ * x[n-352+pos] before it, so the four LDS reads are issued together ahead
+ * of the phase's write. */
+/* stream j of a launch, run by one 256-thread work-group; st: 800 words of
+ * LDS (the 768-word ring and the two 16-word tables) */
+__device__ __forceinline__ void esth_mtgp_wg(const uint32_t* __restrict__ params, uint32_t mask,
+                                             const uint32_t* status_in, uint32_t* status_out,
+                                             const esth_mtgp_stream* __restrict__ streams, uint32_t j,
+                                             uint32_t* __restrict__ out, int float_mode, int nsteps,
+                                             uint64_t out_step, uint64_t status_step, uint32_t* st) {
+    uint32_t* tbl = st + ESTH_MTGP_LARGE;
+    uint32_t* ttbl = tbl + 16;
     const int tid = threadIdx.x;
-    const esth_mtgp_stream sd = streams[blockIdx.x];
+    const esth_mtgp_stream sd = streams[j];
     const uint32_t* p = params + (size_t)sd.set * ESTH_MTGP_PARAM_WORDS;
     const int pos = (int)p[0];
     const uint32_t sh1 = p[1], sh2 = p[2];
@@ -123,39 +136,62 @@ static __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict_
         ttbl[tid] = float_mode ? p[36 + tid] : p[20 + tid];
     }
     const uint32_t* gin = status_in + (size_t)sd.status * ESTH_MTGP_N;
-    uint32_t* gst = status_out + (size_t)sd.status * ESTH_MTGP_N;
     st[ESTH_MTGP_LARGE - ESTH_MTGP_N + tid] = gin[tid];
     if (tid < ESTH_MTGP_N - ESTH_MTGP_THREADS)
         st[ESTH_MTGP_LARGE - ESTH_MTGP_N + ESTH_MTGP_THREADS + tid] = gin[ESTH_MTGP_THREADS + tid];
     __syncthreads();
-    uint32_t* o = out + sd.out_off;
     const int size = (int)sd.size;
     /* start offsets (mod 768) of x[n-351] for the three phases */
     const int b0 = ESTH_MTGP_LARGE - ESTH_MTGP_N;          /* 417 */
     const int b1 = 4 * ESTH_MTGP_THREADS - ESTH_MTGP_N;    /* 673 */
     const int b2 = 2 * ESTH_MTGP_THREADS - ESTH_MTGP_N;    /* 161 */
-    for (int i = 0; i < size; i += ESTH_MTGP_LARGE) {
+    for (int step = 0; step < nsteps; ++step) {
+        uint32_t* o = out + (size_t)step * out_step + sd.out_off;
+        for (int i = 0; i < size; i += ESTH_MTGP_LARGE) {
 #pragma unroll
-        for (int ph = 0; ph < 3; ++ph) {
-            const int b = ph == 0 ? b0 : (ph == 1 ? b1 : b2);
-            int i0 = b + tid, i1 = b + tid + 1, ip = b + tid + pos, it = b + tid + pos - 1;
-            if (ph == 1) { i0 %= ESTH_MTGP_LARGE; i1 %= ESTH_MTGP_LARGE; ip %= ESTH_MTGP_LARGE; it %= ESTH_MTGP_LARGE; }
-            uint32_t X = (st[i0] & mask) ^ st[i1];
-            X ^= X << sh1;
-            uint32_t Y = X ^ (st[ip] >> sh2);
-            uint32_t r = Y ^ tbl[Y & 0x0f];
-            st[ph * ESTH_MTGP_THREADS + tid] = r;
-            uint32_t T = st[it];
-            T ^= T >> 16;
-            T ^= T >> 8;
-            uint32_t v = float_mode ? ((r >> 9) ^ ttbl[T & 0x0f]) : (r ^ ttbl[T & 0x0f]);
-            o[i + ph * ESTH_MTGP_THREADS + tid] = v;
-            __syncthreads();
+            for (int ph = 0; ph < 3; ++ph) {
+                const int b = ph == 0 ? b0 : (ph == 1 ? b1 : b2);
+                int i0 = b + tid, i1 = b + tid + 1, ip = b + tid + pos, it = b + tid + pos - 1;
+                if (ph == 1) { i0 %= ESTH_MTGP_LARGE; i1 %= ESTH_MTGP_LARGE; ip %= ESTH_MTGP_LARGE; it %= ESTH_MTGP_LARGE; }
+                const uint32_t x0 = st[i0], x1 = st[i1], xp = st[ip];
+                uint32_t T = st[it];
+                uint32_t X = (x0 & mask) ^ x1;
+                X ^= X << sh1;
+                uint32_t Y = X ^ (xp >> sh2);
+                uint32_t r = Y ^ tbl[Y & 0x0f];
+                st[ph * ESTH_MTGP_THREADS + tid] = r;
+                T ^= T >> 16;
+                T ^= T >> 8;
+                uint32_t v = float_mode ? ((r >> 9) ^ ttbl[T & 0x0f]) : (r ^ ttbl[T & 0x0f]);
+                o[i + ph * ESTH_MTGP_THREADS + tid] = v;
+                __syncthreads();
+            }
         }
+        /* the status after this step: the last 351 words of the ring, whose
+         * newest word sits at 767 after a whole number of rounds */
+        uint32_t* gst = status_out + (size_t)step * status_step + (size_t)sd.status * ESTH_MTGP_N;
+        gst[tid] = st[ESTH_MTGP_LARGE - ESTH_MTGP_N + tid];
+        if (tid < ESTH_MTGP_N - ESTH_MTGP_THREADS)
+            gst[ESTH_MTGP_THREADS + tid] = st[4 * ESTH_MTGP_THREADS - ESTH_MTGP_N + tid];
     }
-    gst[tid] = st[ESTH_MTGP_LARGE - ESTH_MTGP_N + tid];
-    if (tid < ESTH_MTGP_N - ESTH_MTGP_THREADS)
-        gst[ESTH_MTGP_THREADS + tid] = st[4 * ESTH_MTGP_THREADS - ESTH_MTGP_N + tid];
+}
+
+static __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ params, uint32_t mask,
+                                              const uint32_t* status_in, uint32_t* status_out,
+                                              const esth_mtgp_stream* __restrict__ streams,
+                                              uint32_t* __restrict__ out, int float_mode, int nsteps,
+                                              uint64_t out_step, uint64_t status_step) {
+    __shared__ uint32_t st[ESTH_MTGP_LARGE + 32];
+    esth_mtgp_wg(params, mask, status_in, status_out, streams, blockIdx.x, out, float_mode, nsteps, out_step,
+                 status_step, st);
+}
+
+/* the next step's words from the first mi.n work-groups of a 256-thread
+ * sampling launch: true for those work-groups, which do nothing else */
+__device__ __forceinline__ bool esth_mtgp_inline_wg(const esth_mtgp_inline& mi, uint32_t* st) {
+    if ((int)blockIdx.x >= mi.n) return false;
+    esth_mtgp_wg(mi.params, mi.mask, mi.status_in, mi.status_out, mi.streams, blockIdx.x, mi.out, 1, 1, 0, 0, st);
+    return true;
 }
 
 /* ============================================================ Box-Muller */
@@ -349,12 +385,28 @@ __device__ __forceinline__ void esth_sample_weight_pair(float* __restrict__ stat
     *reinterpret_cast<float2*>(w + p) = wo;
 }
 
-template <int MODEL, int L>
+/* The step's frame reaches the sampling kernels either in device memory
+ * (frame_ptr: graph replay, whose captured arguments are fixed) or as a
+ * kernel argument (FA: eager steps; the frame then travels with the launch
+ * and needs no copy ordered before it).  Either way the kernels read it with
+ * scalar loads. */
+struct esth_frame_arg {
+    float v[ESTH_FRAME_FLOATS];
+};
+
+template <int MODEL, int L, bool FA>
 __global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state, uint32_t N,
                                                        const float* __restrict__ R,
-                                                       const float* __restrict__ frame, int log_w,
-                                                       float* __restrict__ w, int carry) {
-    const uint32_t p = 2u * (blockIdx.x * blockDim.x + threadIdx.x);
+                                                       const float* __restrict__ frame_ptr,
+                                                       const esth_frame_arg fa, int log_w,
+                                                       float* __restrict__ w, int carry,
+                                                       const esth_mtgp_inline mi) {
+    if (mi.n) {
+        __shared__ uint32_t mst[ESTH_MTGP_LARGE + 32];
+        if (esth_mtgp_inline_wg(mi, mst)) return;
+    }
+    const float* __restrict__ frame = FA ? fa.v : frame_ptr;
+    const uint32_t p = 2u * ((blockIdx.x - (uint32_t)mi.n) * blockDim.x + threadIdx.x);
     if (p < N) esth_sample_weight_pair<MODEL, L>(state, N, R, frame, log_w, w, p, carry);
 }
 
@@ -835,10 +887,11 @@ __host__ __device__ constexpr int esth_ss_waves() {
  * here from the gate's uniform (gate[g] - 1 > keep_thr) or, for the
  * data-dependent alias gate, always (keep_all).  The sort's wave runs beside
  * other work-groups' sampling instead of after all of them. */
-template <int MODEL, int L, int LOGM>
+template <int MODEL, int L, int LOGM, bool FA>
 __global__ __launch_bounds__(ESTH_WAVE * esth_ss_waves<LOGM>()) void k_sample_sort(float* __restrict__ state, uint32_t N,
                                                                  uint32_t G, const float* __restrict__ R,
-                                                                 const float* __restrict__ frame, int log_w,
+                                                                 const float* __restrict__ frame_ptr,
+                                                                 const esth_frame_arg fa, int log_w,
                                                                  float* __restrict__ w, int carry,
                                                                  float* __restrict__ st_out,
                                                                  float2* __restrict__ winner, uint32_t t,
@@ -846,13 +899,21 @@ __global__ __launch_bounds__(ESTH_WAVE * esth_ss_waves<LOGM>()) void k_sample_so
                                                                  float* __restrict__ bt_s,
                                                                  float* __restrict__ part,
                                                                  const float* __restrict__ gate, float keep_thr,
-                                                                 int keep_all) {
+                                                                 int keep_all, const esth_mtgp_inline mi) {
     using T = esth_model_traits<MODEL>;
     constexpr int M = 1 << LOGM, S = T::S;
     static_assert(LOGM >= 7 && LOGM <= 10, "one work-group per sub-filter of 128..1024 particles");
+    const float* __restrict__ frame = FA ? fa.v : frame_ptr;
     __shared__ __align__(16) float sub[(1 + S) * M];   /* weights, then plane c at (1 + c) M */
     constexpr int NT = ESTH_WAVE * esth_ss_waves<LOGM>();
-    const uint32_t g = blockIdx.x;
+    /* the next step's MTGP32 work-groups come first (the launcher adds them
+     * only where a work-group has 256 threads and the LDS to hold a ring) */
+    constexpr bool MT = NT == ESTH_MTGP_THREADS && (1 + S) * M >= ESTH_MTGP_LARGE + 32;
+    if constexpr (MT) {
+        if (mi.n && esth_mtgp_inline_wg(mi, reinterpret_cast<uint32_t*>(sub))) return;
+    }
+    const uint32_t g = blockIdx.x - (MT ? (uint32_t)mi.n : 0u);
+    if (g >= G) return;
     const bool keep = keep_all || (gate[g] - 1.0f) > keep_thr;
 #pragma unroll 1
     for (int q = 2 * (int)threadIdx.x; q < M; q += 2 * NT) {

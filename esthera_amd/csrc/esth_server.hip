@@ -195,6 +195,12 @@ struct esth_server {
     esth_arena arena;
     unsigned char* base = nullptr;
     uint64_t base_bytes = 0;
+    /* the arena's memory: a reserved virtual range whose chunks are backed by
+     * physical HBM when a track's block first needs them (vmm), so the
+     * budget costs nothing until tracks occupy it; else one hipMalloc */
+    bool vmm = false;
+    uint64_t chunk = 0, reserved = 0, mapped_bytes = 0;
+    std::vector<hipMemGenericAllocationHandle_t> chunks;
     hipStream_t stream = nullptr, rng_stream = nullptr;
     esth_wslot slot[ESTH_SRV_SLOTS];
     float* R[2] = {nullptr, nullptr};
@@ -212,6 +218,90 @@ struct esth_server {
     uint64_t last_slot_wave[ESTH_SRV_SLOTS] = {0, 0, 0, 0};   /* 1 + wave index last launched in a slot */
     std::vector<unsigned char> scratch;
 };
+
+#define ESTH_SRV_CHUNK (1ull << 30)
+
+/* the arena's backing: reserve base_bytes of virtual addresses (chunks mapped
+ * on demand) where the device supports it, else allocate it whole */
+static int arena_create(esth_server* s) {
+    int dev = s->e->device, vmm = 0;
+    if (hipDeviceGetAttribute(&vmm, hipDeviceAttributeVirtualMemoryManagementSupported, dev) != hipSuccess) vmm = 0;
+    if (vmm) {
+        hipMemAllocationProp prop = {};
+        prop.type = hipMemAllocationTypePinned;
+        prop.location.type = hipMemLocationTypeDevice;
+        prop.location.id = dev;
+        size_t gran = 0;
+        if (hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum) != hipSuccess || !gran)
+            vmm = 0;
+        else {
+            s->chunk = align_up(ESTH_SRV_CHUNK, gran);
+            s->reserved = align_up(s->base_bytes, s->chunk);
+            void* p = nullptr;
+            if (hipMemAddressReserve(&p, s->reserved, s->chunk, nullptr, 0) != hipSuccess || !p) {
+                (void)hipGetLastError();
+                vmm = 0;
+            } else {
+                s->base = (unsigned char*)p;
+            }
+        }
+    }
+    if (vmm) {
+        ESTH_TRY(esth_engine_account(s->e, s->base_bytes));
+        s->vmm = true;
+        s->chunks.assign(s->reserved / s->chunk, nullptr);
+        return ESTH_OK;
+    }
+    return esth_engine_alloc(s->e, s->base_bytes, (void**)&s->base);
+}
+
+/* back every chunk of [off, off + n) with HBM (vmm arenas) */
+static int arena_map(esth_server* s, uint64_t off, uint64_t n) {
+    if (!s->vmm || n == 0) return ESTH_OK;
+    hipMemAllocationProp prop = {};
+    prop.type = hipMemAllocationTypePinned;
+    prop.location.type = hipMemLocationTypeDevice;
+    prop.location.id = s->e->device;
+    hipMemAccessDesc acc = {};
+    acc.location.type = hipMemLocationTypeDevice;
+    acc.location.id = s->e->device;
+    acc.flags = hipMemAccessFlagsProtReadWrite;
+    for (uint64_t c = off / s->chunk; c <= (off + n - 1) / s->chunk; ++c) {
+        if (s->chunks[c]) continue;
+        hipMemGenericAllocationHandle_t h = nullptr;
+        void* at = s->base + c * s->chunk;
+        hipError_t err = hipMemCreate(&h, s->chunk, &prop, 0);
+        if (err == hipSuccess) {
+            err = hipMemMap(at, s->chunk, 0, h, 0);
+            if (err == hipSuccess) {
+                err = hipMemSetAccess(at, s->chunk, &acc, 1);
+                if (err != hipSuccess) (void)hipMemUnmap(at, s->chunk);
+            }
+            if (err != hipSuccess) (void)hipMemRelease(h);
+        }
+        if (err != hipSuccess) return esth_fail(ESTH_ERR_NOMEM, "server arena chunk %llu: %s", (unsigned long long)c,
+                                                hipGetErrorString(err));
+        s->chunks[c] = h;
+        s->mapped_bytes += s->chunk;
+    }
+    return ESTH_OK;
+}
+
+static void arena_destroy(esth_server* s) {
+    if (!s->base) return;
+    if (s->vmm) {
+        for (size_t c = 0; c < s->chunks.size(); ++c)
+            if (s->chunks[c]) {
+                (void)hipMemUnmap(s->base + c * s->chunk, s->chunk);
+                (void)hipMemRelease(s->chunks[c]);
+            }
+        (void)hipMemAddressFree(s->base, s->reserved);
+        esth_engine_account(s->e, -(int64_t)s->base_bytes);
+    } else {
+        esth_engine_release(s->e, s->base, s->base_bytes);
+    }
+    s->base = nullptr;
+}
 
 static int srv_free_buffers(esth_server* s) {
     for (auto& w : s->slot) {
@@ -242,7 +332,7 @@ extern "C" int esth_server_create(esth_engine* e, int max_tracks, size_t hbm_byt
     s->e = e;
     s->max_tracks = max_tracks;
     s->base_bytes = align_up(hbm_bytes, ESTH_SRV_ALIGN);
-    ESTH_TRY(esth_engine_alloc(e, s->base_bytes, (void**)&s->base));
+    ESTH_TRY(arena_create(s.get()));
     s->arena.init(s->base_bytes);
     hipError_t err = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (err == hipSuccess) err = hipStreamCreateWithFlags(&s->rng_stream, hipStreamNonBlocking);
@@ -251,7 +341,7 @@ extern "C" int esth_server_create(esth_engine* e, int max_tracks, size_t hbm_byt
         if (err == hipSuccess) err = hipEventCreateWithFlags(&w.ev_done, hipEventDisableTiming);
     }
     if (err != hipSuccess) {
-        esth_engine_release(e, s->base, s->base_bytes);
+        arena_destroy(s.get());
         return esth_fail(ESTH_ERR_DEVICE, "server streams: %s", hipGetErrorString(err));
     }
     *out = s.release();
@@ -270,7 +360,7 @@ extern "C" int esth_server_destroy(esth_server* s) {
     }
     hipStreamDestroy(s->stream);
     hipStreamDestroy(s->rng_stream);
-    esth_engine_release(s->e, s->base, s->base_bytes);
+    arena_destroy(s);
     delete s;
     return ESTH_OK;
 }
@@ -511,6 +601,10 @@ static void admit(esth_server* s, std::vector<esth_strack*>& inits) {
         esth_strack* t = s->admit_q.front();
         uint64_t off = 0;
         if (!s->arena.alloc(t->bytes, &off)) break;
+        if (arena_map(s, off, t->bytes) != ESTH_OK) {   /* no HBM to back it now: wait for closes */
+            s->arena.release(off, t->bytes);
+            break;
+        }
         s->admit_q.pop_front();
         s->queued--;
         t->off = off;

@@ -507,6 +507,14 @@ int esth_mtgp_seed_state(int set, uint32_t seed, uint32_t* state351);
  * size must be a multiple of 768.  float_mode: 1 floats in [1,2), 0 uint32. */
 int esth_stage_mtgp(uint32_t* d_status, int nstreams, int first_set, int size, void* d_out,
                     int float_mode);
+/* `nsteps` consecutive launches of esth_stage_mtgp in one (how a filter
+ * generates a block of steps' words): step s's words of stream j at
+ * d_out + (s * nstreams + j) * size, and the status after step s at
+ * d_status_out + s * nstreams * 351; d_status_in (nstreams * 351 words) must
+ * not overlap d_status_out.  Words and statuses equal nsteps calls of
+ * esth_stage_mtgp (mtgp32-opencl.cl:186-242; pf_generic.c:191-232 per step). */
+int esth_stage_mtgp_steps(const uint32_t* d_status_in, uint32_t* d_status_out, int nstreams, int first_set,
+                          int size, int nsteps, void* d_out, int float_mode);
 /* boxmuller (pf_generic.cl:465-483), in place on the first n floats. */
 int esth_stage_boxmuller(float* d_data, int n);
 /* sampling_importance (pf_generic.cl:206-250) with Box-Muller fused: particle

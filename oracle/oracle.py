@@ -25,12 +25,21 @@ def _opt(p):
 _lib = None
 
 
-def lib():
+def use_library(path):
+    """Load the oracle from another build of oracle/esth_oracle.c (the CPU
+    baseline's -O3 -march=native build); later calls use it."""
+    global _lib
+    _lib = None
+    return lib(path)
+
+
+def lib(path=None):
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
-        L = C.CDLL(LIB_PATH)
+        path = path or LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        L = C.CDLL(path)
         L.or_mtgp_seed.argtypes = [_u32p, C.c_uint32, _u32p]
         L.or_mtgp_generate.argtypes = [_u32p, C.c_uint32, _u32p, C.c_int, _u32p, C.c_int]
         L.or_mtgp_fill.argtypes = [_u32p, C.c_int, C.c_int, _u32p, C.c_int, _u32p, C.c_int]

@@ -61,6 +61,29 @@ def test_mtgp_all_200_sets(golden, seed, engine):
     assert_bits(got.ravel(), golden[f"f32_{seed}"].ravel())
 
 
+@pytest.mark.parametrize("nsteps,size", [(1, 768), (3, 1536), (8, 768 * 5)])
+def test_mtgp_steps_equal_consecutive_launches(golden, nsteps, size, engine):
+    """A block of steps in one launch (how filters generate their words)
+    equals one launch per step: every step's words and the status after every
+    step; the first 4096 words of each stream are the reference library's."""
+    ns = 6
+    st = np.concatenate([ea.mtgp_seed_state(i, 1234 + i + 1) for i in range(ns)])
+    din, dst_out = DB.from_array(st), DB(nsteps * ns * 351 * 4)
+    out = DB(nsteps * ns * size * 4)
+    ea.check(ea.lib().esth_stage_mtgp_steps(din.ptr, dst_out.ptr, ns, 0, size, nsteps, out.ptr, 0))
+    words = out.to_array(np.uint32, nsteps * ns * size).reshape(nsteps, ns, size)
+    stats = dst_out.to_array(np.uint32, nsteps * ns * 351).reshape(nsteps, ns * 351)
+    one = DB.from_array(st)
+    o1 = DB(ns * size * 4)
+    for k in range(nsteps):
+        ea.check(ea.lib().esth_stage_mtgp(one.ptr, ns, 0, size, o1.ptr, 0))
+        assert_bits(words[k].ravel(), o1.to_array(np.uint32, ns * size))
+        assert_bits(stats[k], one.to_array(np.uint32, ns * 351))
+    seq = np.concatenate([words[k] for k in range(nsteps)], axis=1)
+    n = min(seq.shape[1], 4096)
+    assert_bits(seq[:4, :n].ravel(), golden["u32_1234"][:, :n].ravel())
+
+
 def test_mtgp_state_written_back(mtgp_table, engine):
     st = np.concatenate([ea.mtgp_seed_state(i, 77 + i) for i in range(3)])
     dst = DB.from_array(st)

@@ -3,11 +3,14 @@ library output (golden fixtures), against closed forms, brute force and
 invariants.  No GPU."""
 import ctypes as C
 import itertools
+import os
 
 import numpy as np
 import pytest
 
 import oracle as O
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 ESTH_TOPO_RING, ESTH_TOPO_TORUS, ESTH_TOPO_ALL2ALL = 1, 2, 3
 
@@ -510,3 +513,30 @@ def test_alias_draw_distribution():
     chi2 = np.sum((counts[nz] - exp[nz]) ** 2 / exp[nz])
     assert chi2 < 40.0          # 13 dof: p ~ 2e-4
     assert np.all(counts[~nz] == 0)
+
+
+def test_cpu_baseline_build_gives_the_same_results(tmp_path):
+    """The CPU baseline's -O3 -march=native build of the oracle (make -C oracle
+    baseline) steps a filter bit-identically to the checker's build: the flags
+    change speed, not arithmetic (no contraction, no reassociation)."""
+    import ctypes
+    import subprocess
+    import esthera_amd as ea
+    r = subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "baseline", f"OUT={tmp_path}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=16, t=1, topology=ea.TOPO_TORUS,
+                          resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, seed=1234)
+    p, ctrl, meas, truth = ea.unicycle_scenario(1234, 32, 50.0, 3)
+    out = []
+    for path in (O.LIB_PATH, str(tmp_path / "liboracle_native.so")):
+        O.use_library(path)
+        f = O.Filter(desc, p, ctypes.sizeof(p))
+        f.init_uniform(np.zeros(3, np.float32))
+        for k in range(3):
+            f.step(ctrl[k], meas[k], 0.1)
+        out.append(f.read(3))
+        del f
+    O.use_library(O.LIB_PATH)
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
