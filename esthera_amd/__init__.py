@@ -227,6 +227,7 @@ _SIGS = {
     "esth_stage_boxmuller": ([_P, C.c_int], C.c_int),
     "esth_stage_sample_weight": ([C.c_int, _P, C.c_uint32, _P, _P, C.c_int, _P], C.c_int),
     "esth_stage_sort": ([_P, _P, _P, C.c_int, C.c_uint32, C.c_uint32, _P, _P], C.c_int),
+    "esth_filter_rng_mode": ([_P, _P, _P], C.c_int),
     "esth_stage_sort_noalloc": ([_P, _P, _P, C.c_int, C.c_uint32, C.c_uint32, _P, _P, _P], C.c_int),
     "esth_stage_argmax": ([_P, C.c_uint32, C.c_int, _P], C.c_int),
     "esth_stage_exchange": ([_P, _P, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int],
@@ -413,6 +414,13 @@ class Filter:
 
     def set_prefetch(self, on=True):
         check(lib().esth_filter_set_prefetch(self.h, 1 if on else 0))
+
+    def rng_mode(self):
+        """(inline, block_steps): whether the next step's MTGP32 words come
+        from the sampling launch, else the block length of the second stream."""
+        a, b = C.c_int(), C.c_int()
+        check(lib().esth_filter_rng_mode(self.h, C.byref(a), C.byref(b)))
+        return bool(a.value), b.value
 
     def set_fusion(self, on=True):
         """Sample+weight and sort in one launch where it pays (default);

@@ -186,11 +186,17 @@ static __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict_
                  status_step, st);
 }
 
+/* one step of stream j for a sampling launch, kept out of line so that its
+ * registers do not add to the sampling path's (a call per work-group) */
+__device__ __noinline__ void esth_mtgp_wg_call(const esth_mtgp_inline mi, uint32_t j, uint32_t* st) {
+    esth_mtgp_wg(mi.params, mi.mask, mi.status_in, mi.status_out, mi.streams, j, mi.out, 1, 1, 0, 0, st);
+}
+
 /* the next step's words from the first mi.n work-groups of a 256-thread
  * sampling launch: true for those work-groups, which do nothing else */
 __device__ __forceinline__ bool esth_mtgp_inline_wg(const esth_mtgp_inline& mi, uint32_t* st) {
     if ((int)blockIdx.x >= mi.n) return false;
-    esth_mtgp_wg(mi.params, mi.mask, mi.status_in, mi.status_out, mi.streams, blockIdx.x, mi.out, 1, 1, 0, 0, st);
+    esth_mtgp_wg_call(mi, blockIdx.x, st);
     return true;
 }
 

@@ -283,6 +283,13 @@ int esth_filter_set_prefetch(esth_filter* f, int enable);
  * (may be NULL) receives whether steps use the fused launch.  Results are
  * bit-identical either way.  Timing level 2 always runs the two launches. */
 int esth_filter_set_fusion(esth_filter* f, int enable, int* active);
+/* How the filter's MTGP32 words are made: *inline_words = 1 when the next
+ * step's words come from extra work-groups of the step's sampling launch
+ * (chosen where the sampling hides them), 0 when they come from a second
+ * stream in blocks of *block_steps steps (mtgp32-opencl.cl:186-242 per step;
+ * no counterpart in the reference, which generates them before each step,
+ * pf_generic.c:191-232). */
+int esth_filter_rng_mode(esth_filter* f, int* inline_words, int* block_steps);
 /* Capture one step into a HIP graph and replay it for later steps with the
  * same control/measurement layout (the frame buffer is updated per call). */
 int esth_filter_use_graph(esth_filter* f, int enable);

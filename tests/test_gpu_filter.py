@@ -408,3 +408,49 @@ def test_small_unicycle_500_steps_rmse(engine):
     assert abs(rmse_g - rmse_o) < 1e-6 * max(rmse_o, 1.0)
     assert rmse_g < 0.2, rmse_g
     f.close()
+
+
+def test_rng_inline_and_stream_paths_identical(engine):
+    """The next step's MTGP32 words from extra work-groups of the sampling
+    launch (taken where the sampling hides them: the unicycle with 40
+    landmarks at m = 512), generated on the main stream (prefetch off), and
+    replayed in a graph: the same particles as the oracle at every step,
+    across a snapshot taken and restored mid-run; the same filter with 4
+    landmarks takes its words from the second stream."""
+    import ctypes
+    desc = ea.filter_desc(ea.MODEL_UNICYCLE, m=512, G=64, t=4, topology=ea.TOPO_TORUS,
+                          resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, seed=31)
+    light, _, _, _ = ea.unicycle_scenario(31, 4, 0.0, 1)
+    lf = ea.Filter(engine, desc, light)
+    assert lf.rng_mode() == (False, 2)
+    lf.close()
+    params, ctrl, meas, truth = ea.unicycle_scenario(31, 40, 50.0, 12)
+    fs = [ea.Filter(engine, desc, params) for _ in range(3)]
+    assert fs[0].rng_mode() == (True, 1), "inline words expected for 40 landmarks at m = 512"
+    fs[1].set_prefetch(False)
+    fs[2].use_graph(True)
+    o = O.Filter(desc, params, ctypes.sizeof(params))
+    init = np.zeros(3, np.float32)
+    for f in fs:
+        f.init_uniform(init)
+    o.init_uniform(init)
+    snap = None
+    for k in range(12):
+        for f in fs:
+            f.step(ctrl[k], meas[k], 0.1)
+        o.step(ctrl[k], meas[k], 0.1)
+        os_, ow = o.read(3)
+        for i, f in enumerate(fs):
+            gs, gw = f.particles()
+            assert np.array_equal(gs.view(np.uint32), os_.view(np.uint32)), f"filter {i} state, step {k}"
+            assert np.array_equal(gw.view(np.uint32), ow.view(np.uint32)), f"filter {i} weights, step {k}"
+        if k == 4:
+            snap = fs[0].snapshot()
+    r = ea.Filter(engine, desc, params)
+    r.restore(snap)
+    for k in range(5, 12):
+        r.step(ctrl[k], meas[k], 0.1)
+    gs, _ = r.particles()
+    assert np.array_equal(gs.view(np.uint32), o.read(3)[0].view(np.uint32))
+    for f in fs + [r]:
+        f.close()
