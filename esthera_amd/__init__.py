@@ -21,7 +21,7 @@ ESTH_OK = 0
 ESTH_ERR_INVALID = -1
 ESTH_ERR_BUDGET = -4
 ESTH_ERR_UNSUPPORTED = -5
-MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6, MODEL_UNICYCLE10, MODEL_ROBOTARM5_CAMERA = 0, 1, 2, 3, 4
+MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6, MODEL_UNICYCLE10, MODEL_ROBOTARM5_CAMERA, MODEL_RANDOMWALK3 = range(6)
 TOPO_NONE, TOPO_RING, TOPO_TORUS, TOPO_ALL2ALL = 0, 1, 2, 3
 RESAMPLE_MULTINOMIAL, RESAMPLE_SYSTEMATIC, RESAMPLE_ALIAS = 0, 1, 2
 WEIGHTS_LINEAR, WEIGHTS_LOG = 0, 1
@@ -29,8 +29,10 @@ RNG_MTGP_COMPAT, RNG_MTGP_PER_FILTER = 0, 1
 LAYOUT_SOA, LAYOUT_AOS, LAYOUT_PACKED = 0, 1, 2
 FRAME_FLOATS = 336
 MAX_LANDMARKS = 64
-STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12, MODEL_UNICYCLE10: 3, MODEL_ROBOTARM5_CAMERA: 9}
-NOISE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 6, MODEL_UNICYCLE10: 3, MODEL_ROBOTARM5_CAMERA: 9}
+STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12, MODEL_UNICYCLE10: 3, MODEL_ROBOTARM5_CAMERA: 9,
+             MODEL_RANDOMWALK3: 3}
+NOISE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 6, MODEL_UNICYCLE10: 3, MODEL_ROBOTARM5_CAMERA: 9,
+             MODEL_RANDOMWALK3: 3}
 
 
 # ------------------------------------------------------------- structures
@@ -98,6 +100,18 @@ class CameraParams(C.Structure):
     _fields_ = [("fc", C.c_float * 2), ("cc", C.c_float * 2), ("kc", C.c_float * 4)]
 
 
+class RandomWalk3Params(C.Structure):
+    _fields_ = [("sigma_q", C.c_float), ("sigma_r", C.c_float)]
+
+
+class RandomWalk3Control(C.Structure):
+    _fields_ = [("v", C.c_float * 3)]
+
+
+class RandomWalk3Measurement(C.Structure):
+    _fields_ = [("p", C.c_float * 3)]
+
+
 class Result(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("track_id", C.c_uint64), ("status", C.c_int), ("kind", C.c_int),
                 ("submit_us", C.c_double), ("done_us", C.c_double), ("estimate", Estimate)]
@@ -126,12 +140,14 @@ class ServeReport(C.Structure):
 
 
 PARAMS_TYPE = {MODEL_ROBOTARM5: None, MODEL_UNICYCLE: UnicycleParams, MODEL_DH6: DH6Params,
-               MODEL_UNICYCLE10: Unicycle10Params, MODEL_ROBOTARM5_CAMERA: CameraParams}
+               MODEL_UNICYCLE10: Unicycle10Params, MODEL_ROBOTARM5_CAMERA: CameraParams,
+               MODEL_RANDOMWALK3: RandomWalk3Params}
 CONTROL_TYPE = {MODEL_ROBOTARM5: RobotArm5Control, MODEL_UNICYCLE: UnicycleControl, MODEL_DH6: DH6Control,
-                MODEL_UNICYCLE10: UnicycleControl, MODEL_ROBOTARM5_CAMERA: RobotArm5Control}
+                MODEL_UNICYCLE10: UnicycleControl, MODEL_ROBOTARM5_CAMERA: RobotArm5Control,
+                MODEL_RANDOMWALK3: RandomWalk3Control}
 MEAS_TYPE = {MODEL_ROBOTARM5: RobotArm5Measurement, MODEL_UNICYCLE: UnicycleMeasurement,
              MODEL_DH6: DH6Measurement, MODEL_UNICYCLE10: Unicycle10Measurement,
-             MODEL_ROBOTARM5_CAMERA: RobotArm5Measurement}
+             MODEL_ROBOTARM5_CAMERA: RobotArm5Measurement, MODEL_RANDOMWALK3: RandomWalk3Measurement}
 
 
 class EsthError(RuntimeError):
@@ -681,6 +697,25 @@ def dh6_scenario(seed, steps, dt=0.01):
     truth = np.zeros((steps + 1) * 12, np.float32)
     check(lib().esth_dh6_scenario(seed, steps, dt, ptr(p), meas, ptr(truth)))
     return p, meas, truth.reshape(steps + 1, 12)
+
+
+def randomwalk3_scenario(seed, steps, dt=0.1, sigma_q=0.05, sigma_r=0.2):
+    """Seeded random-walk trace: (params, controls, measurements, truth rows
+    steps x 3); the truth walks with a constant velocity plus N(0, sigma_q)
+    steps, each measurement is the truth plus N(0, sigma_r) noise."""
+    rng = np.random.default_rng(seed)
+    p = RandomWalk3Params(sigma_q, sigma_r)
+    v = rng.uniform(-1.0, 1.0, 3).astype(np.float32)
+    ctrl = (RandomWalk3Control * steps)()
+    meas = (RandomWalk3Measurement * steps)()
+    truth = np.zeros((steps, 3), np.float32)
+    x = np.zeros(3, np.float64)
+    for k in range(steps):
+        x = x + v * dt + rng.normal(0.0, sigma_q, 3)
+        truth[k] = x
+        ctrl[k].v[:] = v.tolist()
+        meas[k].p[:] = (x + rng.normal(0.0, sigma_r, 3)).astype(np.float32).tolist()
+    return p, ctrl, meas, truth
 
 
 def robotarm5_camera_scenario(seed, steps, dt=0.1, params=None):

@@ -24,6 +24,7 @@
 
 #include "esth_internal.h"
 #include "esth_kernels.h"
+#include "esth_dispatch.h"
 
 #define ESTH_BATCH_TIMING_STEPS 8192
 
@@ -113,17 +114,6 @@ static bool kb_fuse_pays_m(int lm) {
     }
 }
 
-template <int S>
-static bool kb_fuse_pays_s(int model, int lm) {
-    switch (model) {
-    case ESTH_MODEL_ROBOTARM5: return kb_fuse_pays_m<ESTH_MODEL_ROBOTARM5, S>(lm);
-    case ESTH_MODEL_UNICYCLE: return kb_fuse_pays_m<ESTH_MODEL_UNICYCLE, S>(lm);
-    case ESTH_MODEL_UNICYCLE10: return kb_fuse_pays_m<ESTH_MODEL_UNICYCLE10, S>(lm);
-    case ESTH_MODEL_ROBOTARM5_CAMERA: return kb_fuse_pays_m<ESTH_MODEL_ROBOTARM5_CAMERA, S>(lm);
-    case ESTH_MODEL_DH6: return kb_fuse_pays_m<ESTH_MODEL_DH6, S>(lm);
-    default: return false;
-    }
-}
 
 /* answers cached per (model, class): 0 unknown, 1 no, 2 yes */
 static std::atomic<int> g_kb_fuse[8][11];
@@ -132,13 +122,10 @@ bool esth_kb_fuse_pays(int model, int S, int lm) {
     if (model < 0 || model >= 8 || lm < 0 || lm > 10) return false;
     const int c = g_kb_fuse[model][lm].load(std::memory_order_relaxed);
     if (c) return c == 2;
-    bool yes = false;
-    switch (S) {
-    case ESTH_UNI_S: yes = kb_fuse_pays_s<ESTH_UNI_S>(model, lm); break;
-    case ESTH_RA5_S: yes = kb_fuse_pays_s<ESTH_RA5_S>(model, lm); break;
-    case ESTH_DH6_S: yes = kb_fuse_pays_s<ESTH_DH6_S>(model, lm); break;
-    default: break;
-    }
+    const bool yes = esth_with_model_or(model, false, [&](auto id) {
+        constexpr int MODEL = decltype(id)::value;
+        return esth_model_traits<MODEL>::S == S && kb_fuse_pays_m<MODEL, esth_model_traits<MODEL>::S>(lm);
+    });
     g_kb_fuse[model][lm].store(yes ? 2 : 1, std::memory_order_relaxed);
     return yes;
 }
@@ -466,31 +453,15 @@ static int launch_kbss_m(int lm, const esth_bfilter* F, const uint2* sub, uint32
     }
 }
 
-template <int S>
-static int launch_kbss_s(int model, int lm, const esth_bfilter* F, const uint2* sub, uint32_t n, const float* R,
-                         const float* frames, int log_w, float thr, int keep_all, hipStream_t st) {
-#define ESTH_KBSS_ARGS lm, F, sub, n, R, frames, log_w, thr, keep_all, st
-    switch (model) {
-    case ESTH_MODEL_ROBOTARM5: return launch_kbss_m<ESTH_MODEL_ROBOTARM5, S>(ESTH_KBSS_ARGS);
-    case ESTH_MODEL_UNICYCLE: return launch_kbss_m<ESTH_MODEL_UNICYCLE, S>(ESTH_KBSS_ARGS);
-    case ESTH_MODEL_UNICYCLE10: return launch_kbss_m<ESTH_MODEL_UNICYCLE10, S>(ESTH_KBSS_ARGS);
-    case ESTH_MODEL_ROBOTARM5_CAMERA: return launch_kbss_m<ESTH_MODEL_ROBOTARM5_CAMERA, S>(ESTH_KBSS_ARGS);
-    case ESTH_MODEL_DH6: return launch_kbss_m<ESTH_MODEL_DH6, S>(ESTH_KBSS_ARGS);
-    default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
-    }
-#undef ESTH_KBSS_ARGS
-}
-
 int esth_launch_kb_sample_sort(int model, int S, int lm, const esth_bfilter* F, const uint2* sub, uint32_t n,
                                const float* R, const float* frames, int log_w, float thr, int keep_all,
                                hipStream_t st) {
     if (n == 0) return ESTH_OK;
-    switch (S) {
-    case ESTH_UNI_S: return launch_kbss_s<ESTH_UNI_S>(model, lm, F, sub, n, R, frames, log_w, thr, keep_all, st);
-    case ESTH_RA5_S: return launch_kbss_s<ESTH_RA5_S>(model, lm, F, sub, n, R, frames, log_w, thr, keep_all, st);
-    case ESTH_DH6_S: return launch_kbss_s<ESTH_DH6_S>(model, lm, F, sub, n, R, frames, log_w, thr, keep_all, st);
-    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
-    }
+    return esth_with_model(model, [&](auto id) {
+        constexpr int MODEL = decltype(id)::value;
+        if (esth_model_traits<MODEL>::S != S) return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
+        return launch_kbss_m<MODEL, esth_model_traits<MODEL>::S>(lm, F, sub, n, R, frames, log_w, thr, keep_all, st);
+    });
 }
 
 #define ESTH_BCLASS(FN, S, ...)                        \
@@ -511,23 +482,12 @@ static int batch_kernels_s(esth_batch* b, int p) {
     hipStream_t s = b->stream;
     const uint32_t nsub = b->nsub;
     const int grid_w = (int)(((uint64_t)b->nchunks * ESTH_WAVE + 255) / 256);
-    if (b->nchunks) switch (b->d.model) {
-    case ESTH_MODEL_ROBOTARM5:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
-        break;
-    case ESTH_MODEL_UNICYCLE:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
-        break;
-    case ESTH_MODEL_UNICYCLE10:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
-        break;
-    case ESTH_MODEL_ROBOTARM5_CAMERA:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5_CAMERA>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
-        break;
-    default:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, s, b->d_F, b->d_chunks, b->nchunks, R, frames, log_w);
-        break;
-    }
+    if (b->nchunks)
+        ESTH_TRY(esth_with_model(b->d.model, [&](auto id) {
+            hipLaunchKernelGGL(kb_sample_weight<decltype(id)::value>, dim3(grid_w), dim3(256), 0, s, b->d_F,
+                               b->d_chunks, b->nchunks, R, frames, log_w);
+            return ESTH_OK;
+        }));
     for (const esth_bclass& c : b->cls) {
         if (c.fused)
             ESTH_TRY(esth_launch_kb_sample_sort(b->d.model, S, c.logm, b->d_F, c.d_sub, c.n, R, frames, log_w,
@@ -557,12 +517,7 @@ static int batch_kernels_s(esth_batch* b, int p) {
 }
 
 static int batch_kernels(esth_batch* b, int p) {
-    switch (b->S) {
-    case ESTH_UNI_S: return batch_kernels_s<ESTH_UNI_S>(b, p);
-    case ESTH_RA5_S: return batch_kernels_s<ESTH_RA5_S>(b, p);
-    case ESTH_DH6_S: return batch_kernels_s<ESTH_DH6_S>(b, p);
-    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", b->S);
-    }
+    return esth_with_model_state_dim(b->S, [&](auto sv) { return batch_kernels_s<decltype(sv)::value>(b, p); });
 }
 
 static int launch_rng_b(esth_batch* b, uint64_t k, hipStream_t s) {

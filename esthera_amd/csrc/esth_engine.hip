@@ -11,6 +11,7 @@
 
 #include "esth_internal.h"
 #include "esth_kernels.h"
+#include "esth_dispatch.h"
 
 /* ================================================================ errors */
 static thread_local std::string g_last_error;
@@ -59,42 +60,38 @@ extern "C" int esth_mtgp_seed_state(int set, uint32_t seed, uint32_t* st) {
 /* ================================================================ models */
 extern "C" int esth_model_lookup(const char* name) {
     if (!name) return -1;
-    if (!strcmp(name, "robotarm5") || !strcmp(name, "robotarm")) return ESTH_MODEL_ROBOTARM5;
-    if (!strcmp(name, "unicycle")) return ESTH_MODEL_UNICYCLE;
-    if (!strcmp(name, "dh6")) return ESTH_MODEL_DH6;
-    if (!strcmp(name, "unicycle10")) return ESTH_MODEL_UNICYCLE10;
-    if (!strcmp(name, "robotarm5_camera")) return ESTH_MODEL_ROBOTARM5_CAMERA;
+    if (!strcmp(name, "robotarm")) return ESTH_MODEL_ROBOTARM5;
+#define ESTH_X(ID, NAME, PFX, S_, V_, PSZ, CSZ, MSZ) \
+    if (!strcmp(name, NAME)) return ID;
+    ESTH_MODEL_LIST(ESTH_X)
+#undef ESTH_X
     return -1;
 }
 
 int esth_params_size(int model, size_t* sz) {
     switch (model) {
-    case ESTH_MODEL_ROBOTARM5: *sz = 0; return ESTH_OK;
-    case ESTH_MODEL_UNICYCLE: *sz = sizeof(esth_unicycle_params); return ESTH_OK;
-    case ESTH_MODEL_DH6: *sz = sizeof(esth_dh6_params); return ESTH_OK;
-    case ESTH_MODEL_UNICYCLE10: *sz = sizeof(esth_unicycle10_params); return ESTH_OK;
-    case ESTH_MODEL_ROBOTARM5_CAMERA: *sz = sizeof(esth_camera_params); return ESTH_OK;
+#define ESTH_X(ID, NAME, PFX, S_, V_, PSZ, CSZ, MSZ) \
+    case ID: *sz = PSZ; return ESTH_OK;
+    ESTH_MODEL_LIST(ESTH_X)
+#undef ESTH_X
     default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
     }
 }
 
 extern "C" int esth_model_info(int model, int* S, int* V, size_t* ps, size_t* cs, size_t* ms) {
-    int s, v;
-    if (esth_model_dims(model, &s, &v)) return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
-    if (S) *S = s;
-    if (V) *V = v;
-    size_t p = 0;
-    esth_params_size(model, &p);
-    if (ps) *ps = p;
-    const bool arm = model == ESTH_MODEL_ROBOTARM5 || model == ESTH_MODEL_ROBOTARM5_CAMERA;
-    if (cs) *cs = arm                       ? sizeof(esth_robotarm5_control)
-                  : model == ESTH_MODEL_DH6 ? sizeof(esth_dh6_control)
-                                            : sizeof(esth_unicycle_control);
-    if (ms) *ms = arm                            ? sizeof(esth_robotarm5_measurement)
-                  : model == ESTH_MODEL_UNICYCLE ? sizeof(esth_unicycle_measurement)
-                  : model == ESTH_MODEL_DH6      ? sizeof(esth_dh6_measurement)
-                                                 : sizeof(esth_unicycle10_measurement);
-    return ESTH_OK;
+    switch (model) {
+#define ESTH_X(ID, NAME, PFX, S_, V_, PSZ, CSZ, MSZ) \
+    case ID:                                        \
+        if (S) *S = S_;                             \
+        if (V) *V = V_;                             \
+        if (ps) *ps = PSZ;                          \
+        if (cs) *cs = CSZ;                          \
+        if (ms) *ms = MSZ;                          \
+        return ESTH_OK;
+    ESTH_MODEL_LIST(ESTH_X)
+#undef ESTH_X
+    default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
+    }
 }
 
 /* particle_filter.cu:60-74 */
@@ -324,24 +321,10 @@ int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* 
         fa = &fa_v;
     }
     const int grid = grid_for(N / 2, 256);
-    switch (model) {
-    case ESTH_MODEL_ROBOTARM5:
-        launch_sample_weight_t<ESTH_MODEL_ROBOTARM5>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
-        break;
-    case ESTH_MODEL_UNICYCLE:
-        launch_sample_weight_t<ESTH_MODEL_UNICYCLE>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
-        break;
-    case ESTH_MODEL_DH6:
-        launch_sample_weight_t<ESTH_MODEL_DH6>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
-        break;
-    case ESTH_MODEL_UNICYCLE10:
-        launch_sample_weight_t<ESTH_MODEL_UNICYCLE10>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
-        break;
-    case ESTH_MODEL_ROBOTARM5_CAMERA:
-        launch_sample_weight_t<ESTH_MODEL_ROBOTARM5_CAMERA>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
-        break;
-    default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
-    }
+    ESTH_TRY(esth_with_model(model, [&](auto id) {
+        launch_sample_weight_t<decltype(id)::value>(layout, grid, state, N, R, frame, fa, log_w, w, carry, mi, s);
+        return ESTH_OK;
+    }));
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }
@@ -387,13 +370,11 @@ int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_
                      int layout, hipStream_t s) {
     const int lm = log2u(m);
 #define ESTH_SORT_ARGS w, st_in, st_out, N, G, winner, perm, t, bt_w, bt_s, part, log_w, layout, s
-    switch (S) {
-    case 1: ESTH_DISPATCH_LOGM(launch_sort_t, 1, ESTH_SORT_ARGS); break;
-    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_UNI_S, ESTH_SORT_ARGS); break;
-    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_RA5_S, ESTH_SORT_ARGS); break;
-    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_sort_t, ESTH_DH6_S, ESTH_SORT_ARGS); break;
-    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
-    }
+    ESTH_TRY(esth_with_state_dim(S, [&](auto sv) {
+        constexpr int SV = decltype(sv)::value;
+        ESTH_DISPATCH_LOGM(launch_sort_t, SV, ESTH_SORT_ARGS);
+        return ESTH_OK;
+    }));
 #undef ESTH_SORT_ARGS
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
@@ -498,14 +479,8 @@ static bool sample_sort_pays_t(int layout, int lm) {
 bool esth_sample_sort_pays(int model, int layout, uint32_t m) {
     if (!esth_sample_sort_fits(m)) return false;
     const int lm = log2u(m);
-    switch (model) {
-    case ESTH_MODEL_ROBOTARM5: return sample_sort_pays_t<ESTH_MODEL_ROBOTARM5>(layout, lm);
-    case ESTH_MODEL_UNICYCLE: return sample_sort_pays_t<ESTH_MODEL_UNICYCLE>(layout, lm);
-    case ESTH_MODEL_DH6: return sample_sort_pays_t<ESTH_MODEL_DH6>(layout, lm);
-    case ESTH_MODEL_UNICYCLE10: return sample_sort_pays_t<ESTH_MODEL_UNICYCLE10>(layout, lm);
-    case ESTH_MODEL_ROBOTARM5_CAMERA: return sample_sort_pays_t<ESTH_MODEL_ROBOTARM5_CAMERA>(layout, lm);
-    default: return false;
-    }
+    return esth_with_model_or(model, false,
+                              [&](auto id) { return sample_sort_pays_t<decltype(id)::value>(layout, lm); });
 }
 
 int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uint32_t G, const float* R,
@@ -527,15 +502,7 @@ int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uin
     }
 #define ESTH_SS_ARGS layout, lm, state, N, G, R, frame, fa, log_w, w, carry, st_out, winner, t, bt_w, bt_s, part, \
                      gate, keep_thr, keep_all, mi, s
-    int rc;
-    switch (model) {
-    case ESTH_MODEL_ROBOTARM5: rc = launch_sample_sort_t<ESTH_MODEL_ROBOTARM5>(ESTH_SS_ARGS); break;
-    case ESTH_MODEL_UNICYCLE: rc = launch_sample_sort_t<ESTH_MODEL_UNICYCLE>(ESTH_SS_ARGS); break;
-    case ESTH_MODEL_DH6: rc = launch_sample_sort_t<ESTH_MODEL_DH6>(ESTH_SS_ARGS); break;
-    case ESTH_MODEL_UNICYCLE10: rc = launch_sample_sort_t<ESTH_MODEL_UNICYCLE10>(ESTH_SS_ARGS); break;
-    case ESTH_MODEL_ROBOTARM5_CAMERA: rc = launch_sample_sort_t<ESTH_MODEL_ROBOTARM5_CAMERA>(ESTH_SS_ARGS); break;
-    default: return esth_fail(ESTH_ERR_INVALID, "unknown model %d", model);
-    }
+    const int rc = esth_with_model(model, [&](auto id) { return launch_sample_sort_t<decltype(id)::value>(ESTH_SS_ARGS); });
 #undef ESTH_SS_ARGS
     if (rc != ESTH_OK) return rc;
     ESTH_HIP(hipGetLastError());
@@ -544,13 +511,11 @@ int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uin
 
 int esth_launch_argmax(int S, const float2* winner, uint32_t G, int log_w, const float* part, const float* sorted,
                        uint32_t N, uint32_t m, float2* result, double* est, hipStream_t s) {
-    switch (S) {
-    case 1: hipLaunchKernelGGL(k_argmax<1>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
-    case ESTH_UNI_S: hipLaunchKernelGGL(k_argmax<ESTH_UNI_S>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
-    case ESTH_RA5_S: hipLaunchKernelGGL(k_argmax<ESTH_RA5_S>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
-    case ESTH_DH6_S: hipLaunchKernelGGL(k_argmax<ESTH_DH6_S>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m, result, est); break;
-    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
-    }
+    ESTH_TRY(esth_with_state_dim(S, [&](auto sv) {
+        hipLaunchKernelGGL(k_argmax<decltype(sv)::value>, dim3(1), dim3(1024), 0, s, winner, G, log_w, part, sorted, N, m,
+                           result, est);
+        return ESTH_OK;
+    }));
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }
@@ -593,24 +558,20 @@ static int launch_alias(int S, float* st_main, const float* st_tmp, const float*
     if (m >= ESTH_WAVE) {
         const int lm = log2u(m);
 #define ESTH_AW_ARGS st_main, st_tmp, w, rnd, N, G, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout, s
-        switch (S) {
-        case 1: ESTH_DISPATCH_LOGM(launch_alias_w_t, 1, ESTH_AW_ARGS); break;
-        case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_alias_w_t, ESTH_UNI_S, ESTH_AW_ARGS); break;
-        case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_alias_w_t, ESTH_RA5_S, ESTH_AW_ARGS); break;
-        case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_alias_w_t, ESTH_DH6_S, ESTH_AW_ARGS); break;
-        default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
-        }
+        ESTH_TRY(esth_with_state_dim(S, [&](auto sv) {
+            constexpr int SV = decltype(sv)::value;
+            ESTH_DISPATCH_LOGM(launch_alias_w_t, SV, ESTH_AW_ARGS);
+            return ESTH_OK;
+        }));
 #undef ESTH_AW_ARGS
         ESTH_HIP(hipGetLastError());
         return ESTH_OK;
     }
-    switch (S) {
-    case 1: hipLaunchKernelGGL(k_alias<1>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout); break;
-    case ESTH_UNI_S: hipLaunchKernelGGL(k_alias<ESTH_UNI_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout); break;
-    case ESTH_RA5_S: hipLaunchKernelGGL(k_alias<ESTH_RA5_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout); break;
-    case ESTH_DH6_S: hipLaunchKernelGGL(k_alias<ESTH_DH6_S>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout); break;
-    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
-    }
+    ESTH_TRY(esth_with_state_dim(S, [&](auto sv) {
+        constexpr int SV = decltype(sv)::value;
+        hipLaunchKernelGGL(k_alias<SV>, dim3(G), dim3(m), 0, s, st_main, st_tmp, w, rnd, N, m, thr, adaptive, log_w, sel, prob_out, alias_out, carry, layout);
+        return ESTH_OK;
+    }));
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }
@@ -623,13 +584,11 @@ int esth_launch_resample(int resampler, int S, float* st_main, const float* st_t
                             layout, s);
     const int lm = log2u(m);
     const int sys = resampler == ESTH_RESAMPLE_SYSTEMATIC;
-    switch (S) {
-    case 1: ESTH_DISPATCH_LOGM(launch_resample_t, 1, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s); break;
-    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_UNI_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s); break;
-    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_RA5_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s); break;
-    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_resample_t, ESTH_DH6_S, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s); break;
-    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
-    }
+    ESTH_TRY(esth_with_state_dim(S, [&](auto sv) {
+        constexpr int SV = decltype(sv)::value;
+        ESTH_DISPATCH_LOGM(launch_resample_t, SV, st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, sel, layout, s);
+        return ESTH_OK;
+    }));
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }
@@ -674,12 +633,11 @@ int esth_launch_resample_argmax(int resampler, int S, float* st_main, const floa
         xch = {bt_w, bt_s, top, w, G, t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, dimx, dimy};
     }
 #define ESTH_RA_ARGS st_main, st_tmp, w, rnd, N, G, thr, sys, log_w, winner, part, best_src, plane_stride, g_stride, result, est, xch, carry, layout, s
-    switch (S) {
-    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_UNI_S, ESTH_RA_ARGS); break;
-    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_RA5_S, ESTH_RA_ARGS); break;
-    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_resample_argmax_t, ESTH_DH6_S, ESTH_RA_ARGS); break;
-    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
-    }
+    ESTH_TRY(esth_with_model_state_dim(S, [&](auto sv) {
+        constexpr int SV = decltype(sv)::value;
+        ESTH_DISPATCH_LOGM(launch_resample_argmax_t, SV, ESTH_RA_ARGS);
+        return ESTH_OK;
+    }));
 #undef ESTH_RA_ARGS
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
@@ -715,12 +673,11 @@ int esth_launch_alias_argmax(int S, float* st_main, const float* st_tmp, float* 
         xch = {bt_w, bt_s, top, w, G, t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, dimx, dimy};
     }
 #define ESTH_AA_ARGS st_main, st_tmp, w, rnd, N, G, thr, adaptive, log_w, winner, part, best_src, plane_stride, g_stride, result, est, xch, carry, layout, s
-    switch (S) {
-    case ESTH_UNI_S: ESTH_DISPATCH_LOGM(launch_alias_argmax_t, ESTH_UNI_S, ESTH_AA_ARGS); break;
-    case ESTH_RA5_S: ESTH_DISPATCH_LOGM(launch_alias_argmax_t, ESTH_RA5_S, ESTH_AA_ARGS); break;
-    case ESTH_DH6_S: ESTH_DISPATCH_LOGM(launch_alias_argmax_t, ESTH_DH6_S, ESTH_AA_ARGS); break;
-    default: return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
-    }
+    ESTH_TRY(esth_with_model_state_dim(S, [&](auto sv) {
+        constexpr int SV = decltype(sv)::value;
+        ESTH_DISPATCH_LOGM(launch_alias_argmax_t, SV, ESTH_AA_ARGS);
+        return ESTH_OK;
+    }));
 #undef ESTH_AA_ARGS
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
@@ -802,7 +759,7 @@ extern "C" int esth_stage_sample_weight(int model, float* d_state, uint32_t N, c
 extern "C" int esth_stage_sort(float* d_w, const float* d_in, float* d_out, int S, uint32_t m, uint32_t G, float* d_winner,
                                int* d_perm) {
     if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES) return esth_fail(ESTH_ERR_INVALID, "m=%u", m);
-    if (S != 1 && S != ESTH_UNI_S && S != ESTH_RA5_S && S != ESTH_DH6_S)
+    if (!esth_state_dim_known(S))
         return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
     const uint32_t N = m * G;
     float* part = nullptr;
@@ -856,7 +813,7 @@ extern "C" int esth_stage_sort_noalloc(float* d_w, const float* d_in, float* d_o
                                        float* d_winner, int* d_perm, void* stream) {
     if (!is_pow2(m) || m < 2 || m > ESTH_MAX_NUM_PARTICLES) return esth_fail(ESTH_ERR_INVALID, "m=%u", m);
     if (G == 0 || G > ESTH_MAX_NUM_BLOCKS) return esth_fail(ESTH_ERR_INVALID, "G=%u", G);
-    if (S != 1 && S != ESTH_UNI_S && S != ESTH_RA5_S && S != ESTH_DH6_S)
+    if (!esth_state_dim_known(S))
         return esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension %d", S);
     if (!d_w || !d_in || !d_out) return esth_fail(ESTH_ERR_INVALID, "null buffer");
     const uint64_t need = (uint64_t)G * (2 + S) + (d_winner ? 0 : 2ull * G);

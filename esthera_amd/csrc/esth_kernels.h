@@ -219,33 +219,22 @@ static __global__ void k_boxmuller(float* __restrict__ d, int n) {
  * and the normals never touch HBM. */
 template <int MODEL>
 struct esth_model_traits;
-template <>
-struct esth_model_traits<ESTH_MODEL_ROBOTARM5> {
-    static constexpr int S = ESTH_RA5_S, V = ESTH_RA5_V;
-    __device__ static float iw(float* s, const float* z, const float* fr) { return esth_robotarm5_sampling_iw(s, z, fr); }
-};
-template <>
-struct esth_model_traits<ESTH_MODEL_UNICYCLE> {
-    static constexpr int S = ESTH_UNI_S, V = ESTH_UNI_V;
-    __device__ static float iw(float* s, const float* z, const float* fr) { return esth_unicycle_sampling_iw(s, z, fr); }
-};
-template <>
-struct esth_model_traits<ESTH_MODEL_DH6> {
-    static constexpr int S = ESTH_DH6_S, V = ESTH_DH6_V;
-    __device__ static float iw(float* s, const float* z, const float* fr) { return esth_dh6_sampling_iw(s, z, fr); }
-};
-template <>
-struct esth_model_traits<ESTH_MODEL_UNICYCLE10> {
-    static constexpr int S = ESTH_U10_S, V = ESTH_U10_V;
-    __device__ static float iw(float* s, const float* z, const float* fr) { return esth_unicycle10_sampling_iw(s, z, fr); }
-};
-template <>
-struct esth_model_traits<ESTH_MODEL_ROBOTARM5_CAMERA> {
-    static constexpr int S = ESTH_RA5_S, V = ESTH_RA5_V;
-    __device__ static float iw(float* s, const float* z, const float* fr) {
-        return esth_robotarm5_camera_sampling_iw(s, z, fr);
-    }
-};
+/* one specialisation per line of esth_model_list.h: state floats S, normals
+ * V, and the model's one- and two-particle hooks */
+#define ESTH_X(ID, NAME, PFX, S_, V_, PSZ, CSZ, MSZ)                                                        \
+    template <>                                                                                            \
+    struct esth_model_traits<ID> {                                                                         \
+        static constexpr int S = S_, V = V_;                                                               \
+        __device__ static float iw(float* s, const float* z, const float* fr) {                            \
+            return esth_##PFX##_sampling_iw(s, z, fr);                                                     \
+        }                                                                                                  \
+        __device__ static void iw_pair(float* s0, float* s1, const float* z0, const float* z1, const float* fr, \
+                                       float* lw0, float* lw1) {                                           \
+            esth_##PFX##_sampling_iw_pair(s0, s1, z0, z1, fr, lw0, lw1);                                    \
+        }                                                                                                  \
+    };
+ESTH_MODEL_LIST(ESTH_X)
+#undef ESTH_X
 
 /* particles p (s0) and p+1 (s1) given their uniform pairs (u0[k], u1[k]) =
  * (R[k N + p], R[k N + p + 1]): Box-Muller, propagation and log-weight */
@@ -259,26 +248,7 @@ __device__ __forceinline__ void esth_sample_pair_regs(float (&s0)[esth_model_tra
     float z0[T::V], z1[T::V];
 #pragma unroll
     for (int k = 0; k < T::V; ++k) esth_boxmuller_pair(u0[k], u1[k], &z0[k], &z1[k]);
-    if constexpr (MODEL == ESTH_MODEL_UNICYCLE) {
-        /* both particles walk the landmarks together: one scalar load of a
-         * landmark feeds two independent chains (same operations per particle) */
-        float c0, n0, c1, n1;
-        esth_unicycle_propagate(s0, z0, frame, &c0, &n0);
-        esth_unicycle_propagate(s1, z1, frame, &c1, &n1);
-        const int L = (int)frame[8];
-        float r0 = 0.0f, b0 = 0.0f, r1 = 0.0f, b1 = 0.0f;
-#pragma unroll 4
-        for (int j = 0; j < L; ++j) {
-            const float* lm = frame + ESTH_UNI_HDR + 5 * j;
-            esth_unicycle_landmark(lm, s0[0], s0[1], c0, n0, &r0, &b0);
-            esth_unicycle_landmark(lm, s1[0], s1[1], c1, n1, &r1, &b1);
-        }
-        lw0 = esth_unicycle_finish(r0, b0, frame);
-        lw1 = esth_unicycle_finish(r1, b1, frame);
-    } else {
-        lw0 = T::iw(s0, z0, frame);
-        lw1 = T::iw(s1, z1, frame);
-    }
+    T::iw_pair(s0, s1, z0, z1, frame, &lw0, &lw1);
 }
 
 /* particles p and p+1 (p even) of a state in layout L: SOA one float2 per

@@ -297,14 +297,165 @@ ESTH_HD float esth_unicycle10_sampling_iw(float* s, const float* z, const float*
     return fr[6] - value;
 }
 
-/* ---------------------------------------------------------- frame build */
+/* ------------------------------------------------------- frame builders
+ * esth_<model>_frame fills the model's part of a zeroed frame whose [0] is
+ * dt (at most nfloats floats); 0, or -1 for bad inputs.
+ * esth_<model>_frame_floats: the floats of the frame its hook reads. */
+ESTH_HD int esth_robotarm5_frame(const void* params, const void* control, const void* measurement, float dt,
+                                 float* fr, int nfloats) {
+    (void)params;
+    (void)dt;
+    (void)nfloats;
+    const esth_robotarm5_control* c = (const esth_robotarm5_control*)control;
+    const esth_robotarm5_measurement* m = (const esth_robotarm5_measurement*)measurement;
+    if (!c || !m) return -1;
+    for (int i = 0; i < 5; ++i) { fr[1 + i] = c->angles[i]; fr[6 + i] = m->angles[i]; }
+    fr[11] = m->x;
+    fr[12] = m->y;
+    return 0;
+}
+ESTH_HD int esth_robotarm5_frame_floats(const void* params) { (void)params; return 16; }
+
+ESTH_HD int esth_robotarm5_camera_frame(const void* params, const void* control, const void* measurement, float dt,
+                                        float* fr, int nfloats) {
+    const esth_camera_params* p = (const esth_camera_params*)params;
+    if (ESTH_RA5C_FLOATS > nfloats || !p) return -1;
+    if (esth_robotarm5_frame(params, control, measurement, dt, fr, nfloats)) return -1;
+    fr[13] = p->fc[0];
+    fr[14] = p->fc[1];
+    fr[15] = p->cc[0];
+    fr[16] = p->cc[1];
+    for (int i = 0; i < 4; ++i) fr[17 + i] = p->kc[i];
+    return 0;
+}
+ESTH_HD int esth_robotarm5_camera_frame_floats(const void* params) { (void)params; return ESTH_RA5C_FLOATS; }
+
+ESTH_HD int esth_unicycle_frame(const void* params, const void* control, const void* measurement, float dt,
+                                float* fr, int nfloats) {
+    const esth_unicycle_params* p = (const esth_unicycle_params*)params;
+    const esth_unicycle_control* c = (const esth_unicycle_control*)control;
+    const esth_unicycle_measurement* m = (const esth_unicycle_measurement*)measurement;
+    if (!p || !c || !m) return -1;
+    if (p->num_landmarks < 0 || p->num_landmarks > ESTH_UNICYCLE_MAX_LANDMARKS) return -1;
+    if (ESTH_UNI_HDR + 5 * p->num_landmarks > nfloats) return -1;
+    if (!(p->sigma_range > 0.0f) || !(p->sigma_bearing > 0.0f)) return -1;
+    fr[1] = c->v * dt;
+    fr[2] = c->omega * dt;
+    fr[3] = p->sigma_x;
+    fr[4] = p->sigma_y;
+    fr[5] = p->sigma_theta;
+    fr[6] = 0.5f / (p->sigma_range * p->sigma_range);
+    fr[7] = 0.5f / (p->sigma_bearing * p->sigma_bearing);
+    fr[8] = (float)p->num_landmarks;
+    for (int j = 0; j < p->num_landmarks; ++j) {
+        float* lm = fr + ESTH_UNI_HDR + 5 * j;
+        float sb, cb;
+        esth_sincosf(m->bearing[j], &sb, &cb);
+        lm[0] = p->lx[j];
+        lm[1] = p->ly[j];
+        lm[2] = m->range[j];
+        lm[3] = cb;
+        lm[4] = sb;
+    }
+    return 0;
+}
+ESTH_HD int esth_unicycle_frame_floats(const void* params) {
+    return params ? ESTH_UNI_HDR + 5 * ((const esth_unicycle_params*)params)->num_landmarks : ESTH_FRAME_FLOATS;
+}
+
+ESTH_HD int esth_unicycle10_frame(const void* params, const void* control, const void* measurement, float dt,
+                                  float* fr, int nfloats) {
+    (void)dt;
+    const esth_unicycle10_params* p = (const esth_unicycle10_params*)params;
+    const esth_unicycle_control* c = (const esth_unicycle_control*)control;
+    const esth_unicycle10_measurement* m = (const esth_unicycle10_measurement*)measurement;
+    if (ESTH_U10_FLOATS > nfloats || !p || !c || !m) return -1;
+    fr[1] = c->v;
+    fr[2] = c->omega;
+    fr[3] = p->noise_v;
+    fr[4] = p->noise_omega;
+    fr[5] = p->noise_gamma;
+    fr[6] = ESTH_U10_LOG_NORM;
+    for (int j = 0; j < 10; ++j) {
+        fr[ESTH_U10_HDR + j] = m->range[j];
+        float* sp = fr + ESTH_U10_HDR + 10 + 3 * j;
+        sp[0] = p->sx[j];
+        sp[1] = p->sy[j];
+        sp[2] = p->sz[j];
+    }
+    return 0;
+}
+ESTH_HD int esth_unicycle10_frame_floats(const void* params) { (void)params; return ESTH_U10_FLOATS; }
+
+ESTH_HD int esth_dh6_frame(const void* params, const void* control, const void* measurement, float dt, float* fr,
+                           int nfloats) {
+    (void)control;
+    const esth_dh6_params* p = (const esth_dh6_params*)params;
+    const esth_dh6_measurement* m = (const esth_dh6_measurement*)measurement;
+    if (ESTH_DH6_HDR + 24 > nfloats || !p || !m) return -1;
+    if (!(p->sigma_pos > 0.0f) || !(p->sigma_enc > 0.0f)) return -1;
+    fr[1] = 0.5f * (dt * dt);
+    fr[2] = p->sigma_acc;
+    fr[3] = 0.5f / (p->sigma_pos * p->sigma_pos);
+    fr[4] = 0.5f / (p->sigma_enc * p->sigma_enc);
+    for (int i = 0; i < 3; ++i) fr[5 + i] = m->p[i];
+    for (int i = 0; i < 6; ++i) fr[8 + i] = m->q[i];
+    for (int i = 0; i < 6; ++i) {
+        float* lk = fr + ESTH_DH6_HDR + 4 * i;
+        float sa, ca;
+        esth_sincosf(p->alpha[i], &sa, &ca);
+        lk[0] = p->a[i];
+        lk[1] = p->d[i];
+        lk[2] = ca;
+        lk[3] = sa;
+    }
+    return 0;
+}
+ESTH_HD int esth_dh6_frame_floats(const void* params) { (void)params; return ESTH_DH6_HDR + 24; }
+
+/* two particles through a model's hook, one after the other (a model may
+ * define its own esth_<model>_sampling_iw_pair to interleave them) */
+#define ESTH_PAIR_HOOK_FROM_SINGLE(PFX)                                                                        \
+    ESTH_HD void esth_##PFX##_sampling_iw_pair(float* s0, float* s1, const float* z0, const float* z1,          \
+                                               const float* fr, float* lw0, float* lw1) {                       \
+        *lw0 = esth_##PFX##_sampling_iw(s0, z0, fr);                                                           \
+        *lw1 = esth_##PFX##_sampling_iw(s1, z1, fr);                                                           \
+    }
+ESTH_PAIR_HOOK_FROM_SINGLE(robotarm5)
+ESTH_PAIR_HOOK_FROM_SINGLE(robotarm5_camera)
+ESTH_PAIR_HOOK_FROM_SINGLE(dh6)
+ESTH_PAIR_HOOK_FROM_SINGLE(unicycle10)
+
+/* the unicycle's particles walk the landmarks together: one scalar load of a
+ * landmark feeds two independent chains (the same operations per particle) */
+ESTH_HD void esth_unicycle_sampling_iw_pair(float* s0, float* s1, const float* z0, const float* z1, const float* fr,
+                                            float* lw0, float* lw1) {
+    float c0, n0, c1, n1;
+    esth_unicycle_propagate(s0, z0, fr, &c0, &n0);
+    esth_unicycle_propagate(s1, z1, fr, &c1, &n1);
+    const int L = (int)fr[8];
+    float r0 = 0.0f, b0 = 0.0f, r1 = 0.0f, b1 = 0.0f;
+#ifdef __HIP_DEVICE_COMPILE__
+#pragma unroll 4
+#endif
+    for (int j = 0; j < L; ++j) {
+        const float* lm = fr + ESTH_UNI_HDR + 5 * j;
+        esth_unicycle_landmark(lm, s0[0], s0[1], c0, n0, &r0, &b0);
+        esth_unicycle_landmark(lm, s1[0], s1[1], c1, n1, &r1, &b1);
+    }
+    *lw0 = esth_unicycle_finish(r0, b0, fr);
+    *lw1 = esth_unicycle_finish(r1, b1, fr);
+}
+
+/* ----------------------------------------------------------- the models */
+#include "esth_model_list.h"
+
 ESTH_HD int esth_model_dims(int model, int* S, int* V) {
     switch (model) {
-    case ESTH_MODEL_ROBOTARM5: *S = ESTH_RA5_S; *V = ESTH_RA5_V; return 0;
-    case ESTH_MODEL_UNICYCLE:  *S = ESTH_UNI_S; *V = ESTH_UNI_V; return 0;
-    case ESTH_MODEL_DH6:       *S = ESTH_DH6_S; *V = ESTH_DH6_V; return 0;
-    case ESTH_MODEL_UNICYCLE10: *S = ESTH_U10_S; *V = ESTH_U10_V; return 0;
-    case ESTH_MODEL_ROBOTARM5_CAMERA: *S = ESTH_RA5_S; *V = ESTH_RA5_V; return 0;
+#define ESTH_X(ID, NAME, PFX, S_, V_, PSZ, CSZ, MSZ) \
+    case ID: *S = S_; *V = V_; return 0;
+    ESTH_MODEL_LIST(ESTH_X)
+#undef ESTH_X
     default: return -1;
     }
 }
@@ -313,112 +464,26 @@ ESTH_HD int esth_model_dims(int model, int* S, int* V) {
 ESTH_HD int esth_build_frame_n(int model, const void* params, const void* control, const void* measurement,
                                float dt, float* fr, int nfloats) {
     for (int i = 0; i < nfloats; ++i) fr[i] = 0.0f;
-    if (model == ESTH_MODEL_UNICYCLE && params &&
-        ESTH_UNI_HDR + 5 * ((const esth_unicycle_params*)params)->num_landmarks > nfloats)
-        return -1;
-    if (model == ESTH_MODEL_DH6 && ESTH_DH6_HDR + 24 > nfloats) return -1;
-    if (model == ESTH_MODEL_UNICYCLE10 && ESTH_U10_FLOATS > nfloats) return -1;
-    if (model == ESTH_MODEL_ROBOTARM5_CAMERA && ESTH_RA5C_FLOATS > nfloats) return -1;
     if (nfloats < 16) return -1;
     fr[0] = dt;
-    if (model == ESTH_MODEL_ROBOTARM5 || model == ESTH_MODEL_ROBOTARM5_CAMERA) {
-        const esth_robotarm5_control* c = (const esth_robotarm5_control*)control;
-        const esth_robotarm5_measurement* m = (const esth_robotarm5_measurement*)measurement;
-        if (!c || !m) return -1;
-        for (int i = 0; i < 5; ++i) { fr[1 + i] = c->angles[i]; fr[6 + i] = m->angles[i]; }
-        fr[11] = m->x;
-        fr[12] = m->y;
-        if (model == ESTH_MODEL_ROBOTARM5_CAMERA) {
-            const esth_camera_params* p = (const esth_camera_params*)params;
-            if (!p) return -1;
-            fr[13] = p->fc[0];
-            fr[14] = p->fc[1];
-            fr[15] = p->cc[0];
-            fr[16] = p->cc[1];
-            for (int i = 0; i < 4; ++i) fr[17 + i] = p->kc[i];
-        }
-        return 0;
+    switch (model) {
+#define ESTH_X(ID, NAME, PFX, S_, V_, PSZ, CSZ, MSZ) \
+    case ID: return esth_##PFX##_frame(params, control, measurement, dt, fr, nfloats);
+    ESTH_MODEL_LIST(ESTH_X)
+#undef ESTH_X
+    default: return -1;
     }
-    if (model == ESTH_MODEL_UNICYCLE) {
-        const esth_unicycle_params* p = (const esth_unicycle_params*)params;
-        const esth_unicycle_control* c = (const esth_unicycle_control*)control;
-        const esth_unicycle_measurement* m = (const esth_unicycle_measurement*)measurement;
-        if (!p || !c || !m) return -1;
-        if (p->num_landmarks < 0 || p->num_landmarks > ESTH_UNICYCLE_MAX_LANDMARKS) return -1;
-        if (!(p->sigma_range > 0.0f) || !(p->sigma_bearing > 0.0f)) return -1;
-        fr[1] = c->v * dt;
-        fr[2] = c->omega * dt;
-        fr[3] = p->sigma_x;
-        fr[4] = p->sigma_y;
-        fr[5] = p->sigma_theta;
-        fr[6] = 0.5f / (p->sigma_range * p->sigma_range);
-        fr[7] = 0.5f / (p->sigma_bearing * p->sigma_bearing);
-        fr[8] = (float)p->num_landmarks;
-        for (int j = 0; j < p->num_landmarks; ++j) {
-            float* lm = fr + ESTH_UNI_HDR + 5 * j;
-            float sb, cb;
-            esth_sincosf(m->bearing[j], &sb, &cb);
-            lm[0] = p->lx[j];
-            lm[1] = p->ly[j];
-            lm[2] = m->range[j];
-            lm[3] = cb;
-            lm[4] = sb;
-        }
-        return 0;
-    }
-    if (model == ESTH_MODEL_UNICYCLE10) {
-        const esth_unicycle10_params* p = (const esth_unicycle10_params*)params;
-        const esth_unicycle_control* c = (const esth_unicycle_control*)control;
-        const esth_unicycle10_measurement* m = (const esth_unicycle10_measurement*)measurement;
-        if (!p || !c || !m) return -1;
-        fr[1] = c->v;
-        fr[2] = c->omega;
-        fr[3] = p->noise_v;
-        fr[4] = p->noise_omega;
-        fr[5] = p->noise_gamma;
-        fr[6] = ESTH_U10_LOG_NORM;
-        for (int j = 0; j < 10; ++j) {
-            fr[ESTH_U10_HDR + j] = m->range[j];
-            float* sp = fr + ESTH_U10_HDR + 10 + 3 * j;
-            sp[0] = p->sx[j];
-            sp[1] = p->sy[j];
-            sp[2] = p->sz[j];
-        }
-        return 0;
-    }
-    if (model == ESTH_MODEL_DH6) {
-        const esth_dh6_params* p = (const esth_dh6_params*)params;
-        const esth_dh6_measurement* m = (const esth_dh6_measurement*)measurement;
-        if (!p || !m) return -1;
-        if (!(p->sigma_pos > 0.0f) || !(p->sigma_enc > 0.0f)) return -1;
-        fr[1] = 0.5f * (dt * dt);
-        fr[2] = p->sigma_acc;
-        fr[3] = 0.5f / (p->sigma_pos * p->sigma_pos);
-        fr[4] = 0.5f / (p->sigma_enc * p->sigma_enc);
-        for (int i = 0; i < 3; ++i) fr[5 + i] = m->p[i];
-        for (int i = 0; i < 6; ++i) fr[8 + i] = m->q[i];
-        for (int i = 0; i < 6; ++i) {
-            float* lk = fr + ESTH_DH6_HDR + 4 * i;
-            float sa, ca;
-            esth_sincosf(p->alpha[i], &sa, &ca);
-            lk[0] = p->a[i];
-            lk[1] = p->d[i];
-            lk[2] = ca;
-            lk[3] = sa;
-        }
-        return 0;
-    }
-    return -1;
 }
 
 /* floats of a model's frame that its sampling_iw reads */
 ESTH_HD int esth_model_frame_floats(int model, const void* params) {
-    if (model == ESTH_MODEL_UNICYCLE)
-        return params ? ESTH_UNI_HDR + 5 * ((const esth_unicycle_params*)params)->num_landmarks : ESTH_FRAME_FLOATS;
-    if (model == ESTH_MODEL_DH6) return ESTH_DH6_HDR + 24;
-    if (model == ESTH_MODEL_UNICYCLE10) return ESTH_U10_FLOATS;
-    if (model == ESTH_MODEL_ROBOTARM5_CAMERA) return ESTH_RA5C_FLOATS;
-    return 16;
+    switch (model) {
+#define ESTH_X(ID, NAME, PFX, S_, V_, PSZ, CSZ, MSZ) \
+    case ID: return esth_##PFX##_frame_floats(params);
+    ESTH_MODEL_LIST(ESTH_X)
+#undef ESTH_X
+    default: return 16;
+    }
 }
 
 ESTH_HD int esth_build_frame_impl(int model, const void* params, const void* control,

@@ -35,6 +35,7 @@
 
 #include "esth_internal.h"
 #include "esth_kernels.h"
+#include "esth_dispatch.h"
 
 #define ESTH_SRV_SLOTS 4
 #define ESTH_SRV_EST (4 + 2 * ESTH_MAX_STATE_DIM)
@@ -695,23 +696,12 @@ static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wa
     const uint2* ch = chunks + g.chk0;
     const uint32_t nch = (uint32_t)g.nchk;
     const int grid_w = (int)(((uint64_t)nch * ESTH_WAVE + 255) / 256);
-    if (nch) switch (k.model) {
-    case ESTH_MODEL_ROBOTARM5:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
-        break;
-    case ESTH_MODEL_UNICYCLE:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
-        break;
-    case ESTH_MODEL_UNICYCLE10:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_UNICYCLE10>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
-        break;
-    case ESTH_MODEL_ROBOTARM5_CAMERA:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_ROBOTARM5_CAMERA>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
-        break;
-    default:
-        hipLaunchKernelGGL(kb_sample_weight<ESTH_MODEL_DH6>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R, frames, log_w);
-        break;
-    }
+    if (nch)
+        ESTH_TRY(esth_with_model(k.model, [&](auto id) {
+            hipLaunchKernelGGL(kb_sample_weight<decltype(id)::value>, dim3(grid_w), dim3(256), 0, st, Fg, ch, nch, R,
+                               frames, log_w);
+            return ESTH_OK;
+        }));
     for (int lm = 6; lm <= 10; ++lm) {
         if (!g.ncls[lm]) continue;
         if (esth_kb_fuse_pays(k.model, S, lm))
@@ -924,14 +914,9 @@ static int launch_wave(esth_server* s, bool* launched) {
         const uint2* sc = (const uint2*)(dt + o_cls);
         const uint2* ck = (const uint2*)(dt + o_chk);
         const float* fr = (const float*)(dt + o_frm);
-        int rc;
-        switch (upd[g.f0].t->S) {
-        case ESTH_UNI_S: rc = group_kernels<ESTH_UNI_S>(s, k, g, dF, sa, sc, ck, s->R[p], fr); break;
-        case ESTH_RA5_S: rc = group_kernels<ESTH_RA5_S>(s, k, g, dF, sa, sc, ck, s->R[p], fr); break;
-        case ESTH_DH6_S: rc = group_kernels<ESTH_DH6_S>(s, k, g, dF, sa, sc, ck, s->R[p], fr); break;
-        default: rc = esth_fail(ESTH_ERR_UNSUPPORTED, "state dimension");
-        }
-        ESTH_TRY(rc);
+        ESTH_TRY(esth_with_model_state_dim(upd[g.f0].t->S, [&](auto sv) {
+            return group_kernels<decltype(sv)::value>(s, k, g, dF, sa, sc, ck, s->R[p], fr);
+        }));
     }
     ESTH_HIP(hipGetLastError());
     if (nw) ESTH_HIP(hipMemcpyAsync(sl.hest, sl.dest, sizeof(double) * ESTH_SRV_EST * nw, hipMemcpyDeviceToHost, s->stream));

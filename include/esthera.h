@@ -76,13 +76,19 @@ int esth_version(void);
  *              57-143): the base point is projected to pixels by the pinhole
  *              + radial/tangential distortion model before the comparison;
  *              noise in the CUDA order (angles first); state as robotarm5.
+ *  randomwalk3: a 3-D random walk observed directly (p += v dt + sigma_q z,
+ *              log w = -|m - p|^2 / (2 sigma_r^2)); state x, y, z (3); 3
+ *              normals.  No reference counterpart: the example of adding a
+ *              model to the engine (its hook and frame builder in one header
+ *              under esthera_amd/csrc/models/, one line in the model list).
  */
 #define ESTH_MODEL_ROBOTARM5  0
 #define ESTH_MODEL_UNICYCLE   1
 #define ESTH_MODEL_DH6        2
 #define ESTH_MODEL_UNICYCLE10 3
 #define ESTH_MODEL_ROBOTARM5_CAMERA 4
-#define ESTH_NUM_MODELS       5
+#define ESTH_MODEL_RANDOMWALK3 5
+#define ESTH_NUM_MODELS       6
 
 /* robotarm5 (pf_model.h:34-56) */
 typedef struct { float angles[5]; } esth_robotarm5_control;
@@ -129,8 +135,14 @@ typedef struct { float fc[2], cc[2], kc[4]; } esth_camera_params;
 /* the reference's calibration (pf_model_soa_packed_code.cuh:33-43) */
 void esth_camera_params_default(esth_camera_params* p);
 
+/* randomwalk3: sigma_q (process), sigma_r (measurement); control is the
+ * velocity, measurement the observed position */
+typedef struct { float sigma_q, sigma_r; } esth_randomwalk3_params;
+typedef struct { float v[3]; } esth_randomwalk3_control;
+typedef struct { float p[3]; } esth_randomwalk3_measurement;
+
 /* Model id for "robotarm5" | "unicycle" | "dh6" | "unicycle10" |
- * "robotarm5_camera"; -1 if unknown. */
+ * "robotarm5_camera" | "randomwalk3"; -1 if unknown. */
 int esth_model_lookup(const char* name);
 /* Sizes for a model: state floats per particle, normals per particle, and
  * the byte sizes of its params / control / measurement structs. */

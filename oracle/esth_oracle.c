@@ -124,12 +124,12 @@ int or_sample_weight(int model, float* state, uint32_t N, const float* normals, 
         float s[ESTH_MAX_STATE_DIM], z[ESTH_MAX_STATE_DIM];
         for (int c = 0; c < S; ++c) s[c] = state[(size_t)c * N + i];
         for (int k = 0; k < V; ++k) z[k] = normals[(size_t)k * N + i];
-        float lw;
-        if (model == ESTH_MODEL_ROBOTARM5) lw = esth_robotarm5_sampling_iw(s, z, frame);
-        else if (model == ESTH_MODEL_UNICYCLE) lw = esth_unicycle_sampling_iw(s, z, frame);
-        else if (model == ESTH_MODEL_UNICYCLE10) lw = esth_unicycle10_sampling_iw(s, z, frame);
-        else if (model == ESTH_MODEL_ROBOTARM5_CAMERA) lw = esth_robotarm5_camera_sampling_iw(s, z, frame);
-        else lw = esth_dh6_sampling_iw(s, z, frame);
+        float lw = 0.0f;
+        /* the hook of the listed model (every listed model, no hand-kept table) */
+#define OR_X(ID, NAME, PFX, S_, V_, PSZ, CSZ, MSZ) \
+        if (model == ID) lw = esth_##PFX##_sampling_iw(s, z, frame);
+        ESTH_MODEL_LIST(OR_X)
+#undef OR_X
         for (int c = 0; c < S; ++c) state[(size_t)c * N + i] = s[c];
         w[i] = (weights == ESTH_WEIGHTS_LOG) ? lw : esth_expf(lw);
     }
@@ -256,6 +256,18 @@ static double u10_f64(double* s, const double* z, const esth_unicycle10_params* 
     return log(1e5) - 5.0 * log(2.0 * M_PI) - value;
 }
 
+/* randomwalk3 from its definition: p += v dt + sigma_q z, log w = -|m - p|^2 / (2 sigma_r^2) */
+static double rw3_f64(double* s, const double* z, const esth_randomwalk3_params* p,
+                      const esth_randomwalk3_control* c, const esth_randomwalk3_measurement* m, double dt) {
+    double e2 = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        s[i] += (double)c->v[i] * (double)(float)dt + (double)p->sigma_q * z[i];
+        const double e = (double)m->p[i] - s[i];
+        e2 += e * e;
+    }
+    return -0.5 * e2 / ((double)p->sigma_r * p->sigma_r);
+}
+
 static double dh6_f64(double* s, const double* z, const esth_dh6_params* p,
                       const esth_dh6_measurement* m, double dt) {
     for (int i = 0; i < 6; ++i) {
@@ -311,8 +323,13 @@ int or_sample_weight_f64(int model, const float* state_in, uint32_t N, const flo
         else if (model == ESTH_MODEL_ROBOTARM5_CAMERA)
             lw = ra5c_f64(s, z, (const esth_camera_params*)params, (const esth_robotarm5_control*)ctrl,
                           (const esth_robotarm5_measurement*)meas, dt);
-        else
+        else if (model == ESTH_MODEL_RANDOMWALK3)
+            lw = rw3_f64(s, z, (const esth_randomwalk3_params*)params, (const esth_randomwalk3_control*)ctrl,
+                         (const esth_randomwalk3_measurement*)meas, dt);
+        else if (model == ESTH_MODEL_DH6)
             lw = dh6_f64(s, z, (const esth_dh6_params*)params, (const esth_dh6_measurement*)meas, dt);
+        else
+            lw = NAN;   /* a listed model without a twin: its FP64 posterior reads NaN */
         for (int c = 0; c < S; ++c) state_out[(size_t)c * N + i] = s[c];
         lw_out[i] = lw;
     }

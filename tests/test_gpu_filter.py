@@ -31,6 +31,9 @@ def make_case(model, steps):
         meas, ctrl, dts, truth = ea.load_unicycle10_trace(os.path.join(HERE, "golden", "unicycle_input.txt"))
         init = np.array([20.0, 40.0, -np.pi / 2], np.float32)
         return ea.unicycle10_params_default(), ctrl, meas, init, truth, dts
+    if model == ea.MODEL_RANDOMWALK3:
+        p, ctrl, meas, truth = ea.randomwalk3_scenario(31, steps)
+        return p, ctrl, meas, np.zeros(3, np.float32), truth, 0.1
     if model == ea.MODEL_ROBOTARM5_CAMERA:
         ctrl, meas, truth = ea.robotarm5_camera_scenario(21, steps, 0.1)
         return ea.camera_params_default(), ctrl, meas, truth[0].copy(), truth[1:], 0.1
@@ -107,6 +110,12 @@ CASES = [
     ("dh6-alias-torus-m1024", dict(model=ea.MODEL_DH6, m=1024, G=8, t=4, topology=ea.TOPO_TORUS,
                                    resampler=ea.RESAMPLE_ALIAS, weights=ea.WEIGHTS_LOG, threshold=0.7, carry=1)),
     ("robotarm-alias-m32", dict(model=ea.MODEL_ROBOTARM5, m=32, G=16, t=2, resampler=ea.RESAMPLE_ALIAS)),
+    # the example model, picked up from the model list alone
+    ("randomwalk3-systematic-torus-log", dict(model=ea.MODEL_RANDOMWALK3, m=512, G=16, t=4, topology=ea.TOPO_TORUS,
+                                              resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG)),
+    ("randomwalk3-reference-linear", dict(model=ea.MODEL_RANDOMWALK3, m=64, G=16, t=1)),
+    ("randomwalk3-alias-carry-gate", dict(model=ea.MODEL_RANDOMWALK3, m=256, G=8, t=2, resampler=ea.RESAMPLE_ALIAS,
+                                          weights=ea.WEIGHTS_LOG, carry=1, threshold=0.5)),
     ("unicycle-small-m", dict(model=ea.MODEL_UNICYCLE, m=16, G=12, t=2, weights=ea.WEIGHTS_LOG,
                               rng=ea.RNG_MTGP_PER_FILTER)),
 ]

@@ -193,6 +193,10 @@ def _model_case(model, N, seed):
         ctrls, meass, truth = ea.robotarm5_camera_scenario(3, 4, 0.1)
         ctrl, meas, dt = ctrls[2], meass[2], 0.1
         st = np.concatenate([truth[2, c] + rng.normal(0, 0.05, N).astype(np.float32) for c in range(9)])
+    elif model == ea.MODEL_RANDOMWALK3:
+        params, ctrls, meass, truth = ea.randomwalk3_scenario(4, 3)
+        ctrl, meas, dt = ctrls[2], meass[2], 0.1
+        st = np.concatenate([truth[1, c] + rng.normal(0, 0.2, N).astype(np.float32) for c in range(3)])
     else:
         params, meass, truth = ea.dh6_scenario(7, 3)
         ctrl, meas = ea.DH6Control(), meass[2]
@@ -203,7 +207,7 @@ def _model_case(model, N, seed):
 
 
 @pytest.mark.parametrize("model", [ea.MODEL_ROBOTARM5, ea.MODEL_UNICYCLE, ea.MODEL_DH6, ea.MODEL_UNICYCLE10,
-                                   ea.MODEL_ROBOTARM5_CAMERA])
+                                   ea.MODEL_ROBOTARM5_CAMERA, ea.MODEL_RANDOMWALK3])
 @pytest.mark.parametrize("weights", [ea.WEIGHTS_LINEAR, ea.WEIGHTS_LOG])
 def test_sample_weight_bit_exact(model, weights, engine):
     N = 4096
