@@ -27,6 +27,7 @@ RESAMPLE_MULTINOMIAL, RESAMPLE_SYSTEMATIC, RESAMPLE_ALIAS = 0, 1, 2
 WEIGHTS_LINEAR, WEIGHTS_LOG = 0, 1
 RNG_MTGP_COMPAT, RNG_MTGP_PER_FILTER = 0, 1
 LAYOUT_SOA, LAYOUT_AOS, LAYOUT_PACKED = 0, 1, 2
+FUSION_AUTO, FUSION_OFF, FUSION_ON = -1, 0, 1
 FRAME_FLOATS = 336
 MAX_LANDMARKS = 64
 STATE_DIM = {MODEL_ROBOTARM5: 9, MODEL_UNICYCLE: 3, MODEL_DH6: 12, MODEL_UNICYCLE10: 3, MODEL_ROBOTARM5_CAMERA: 9,
@@ -169,6 +170,8 @@ _SIGS = {
     "esth_engine_create": ([C.c_int, C.c_size_t, _P], C.c_int),
     "esth_engine_destroy": ([_P], C.c_int),
     "esth_engine_memory": ([_P, _P, _P], C.c_int),
+    "esth_engine_set_batch_fusion": ([_P, C.c_uint32, C.c_int], C.c_int),
+    "esth_engine_batch_fusion": ([_P, C.c_uint32, _P], C.c_int),
     "esth_filter_create": ([_P, _P, _P, C.c_size_t, _P], C.c_int),
     "esth_filter_destroy": ([_P], C.c_int),
     "esth_filter_init_uniform": ([_P, _P], C.c_int),
@@ -349,6 +352,16 @@ class Engine:
         used, budget = C.c_size_t(), C.c_size_t()
         check(lib().esth_engine_memory(self.h, C.byref(used), C.byref(budget)))
         return used.value, budget.value
+
+    def set_batch_fusion(self, m, mode):
+        """Batches/servers: sub-filter size class m sampled and sorted in one
+        launch (FUSION_ON), two (FUSION_OFF) or as the engine judges (FUSION_AUTO)."""
+        check(lib().esth_engine_set_batch_fusion(self.h, m, mode))
+
+    def batch_fusion(self, m):
+        mode = C.c_int()
+        check(lib().esth_engine_batch_fusion(self.h, m, C.byref(mode)))
+        return mode.value
 
     def close(self):
         if self.h:

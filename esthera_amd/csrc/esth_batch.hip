@@ -91,43 +91,96 @@ struct esth_bclass {
     uint32_t n = 0;
     uint2* d_sub = nullptr;
     bool fused = false;   /* sample+weight and sort in one launch (kb_sample_sort) */
+    uint2* d_units = nullptr;   /* m = 64 fused: pairs of sub-filters (esth_pair_units) */
+    uint32_t nunits = 0;
 };
 
-/* kb_sample_sort<MODEL, LOGM, S> for size class lm keeps as many waves
- * resident per SIMD as kb_sample_weight<MODEL> */
+size_t esth_pair_units(const uint2* cls, size_t n, uint2* units) {
+    size_t k = 0;
+    for (size_t i = 0; i < n;) {
+        if (i + 1 < n && cls[i + 1].x == cls[i].x && cls[i + 1].y == cls[i].y + 1) {
+            units[k++] = cls[i];
+            i += 2;
+        } else {
+            units[k++] = make_uint2(cls[i].x, cls[i].y | ESTH_UNIT_SINGLE);
+            i += 1;
+        }
+    }
+    return k;
+}
+
+/* kb_sample_sort<MODEL, LOGM, S> for size class lm: 0 it cannot run (not
+ * compiled for the model, or it spills), 1 it runs with fewer waves resident
+ * per SIMD than kb_sample_weight<MODEL>, 2 with at least as many */
 template <int MODEL, int S>
-static bool kb_fuse_pays_m(int lm) {
+static int kb_fuse_grade_m(int lm) {
     if constexpr (esth_model_traits<MODEL>::S != S) {
-        return false;
+        return 0;
     } else {
         const void* fn = nullptr;
         switch (lm) {
+        case 6: fn = (const void*)&kb_sample_sort64<MODEL, S>; break;
         case 7: fn = (const void*)&kb_sample_sort<MODEL, 7, S>; break;
         case 8: fn = (const void*)&kb_sample_sort<MODEL, 8, S>; break;
         case 9: fn = (const void*)&kb_sample_sort<MODEL, 9, S>; break;
         case 10: fn = (const void*)&kb_sample_sort<MODEL, 10, S>; break;
-        default: return false;
+        default: return 0;
         }
-        const int waves = lm >= 9 ? 4 : (1 << lm) / 128;
+        const int waves = (lm >= 9 || lm == 6) ? 4 : (1 << lm) / 128;
         const int fused = esth_waves_per_simd(fn, ESTH_WAVE * waves);
-        return fused > 0 && fused >= esth_waves_per_simd((const void*)&kb_sample_weight<MODEL>, 256);
+        if (fused <= 0) return 0;
+        return fused >= esth_waves_per_simd((const void*)&kb_sample_weight<MODEL>, 256) ? 2 : 1;
     }
 }
 
+/* grades cached per (model, class): grade + 1, 0 unknown */
+static std::atomic<int> g_kb_fuse[16][11];
 
-/* answers cached per (model, class): 0 unknown, 1 no, 2 yes */
-static std::atomic<int> g_kb_fuse[8][11];
-
-bool esth_kb_fuse_pays(int model, int S, int lm) {
-    if (model < 0 || model >= 8 || lm < 0 || lm > 10) return false;
+static int kb_fuse_grade(int model, int S, int lm) {
+    if (model < 0 || model >= 16 || lm < 0 || lm > 10) return 0;
     const int c = g_kb_fuse[model][lm].load(std::memory_order_relaxed);
-    if (c) return c == 2;
-    const bool yes = esth_with_model_or(model, false, [&](auto id) {
+    if (c) return c - 1;
+    const int grade = esth_with_model_or(model, 0, [&](auto id) {
         constexpr int MODEL = decltype(id)::value;
-        return esth_model_traits<MODEL>::S == S && kb_fuse_pays_m<MODEL, esth_model_traits<MODEL>::S>(lm);
+        return esth_model_traits<MODEL>::S == S ? kb_fuse_grade_m<MODEL, esth_model_traits<MODEL>::S>(lm) : 0;
     });
-    g_kb_fuse[model][lm].store(yes ? 2 : 1, std::memory_order_relaxed);
-    return yes;
+    g_kb_fuse[model][lm].store(grade + 1, std::memory_order_relaxed);
+    return grade;
+}
+
+bool esth_kb_fuse_with(int mode, int model, int S, int lm) {
+    const int grade = kb_fuse_grade(model, S, lm);
+    if (grade == 0 || mode == ESTH_FUSION_OFF) return false;
+    return mode == ESTH_FUSION_ON || grade == 2;
+}
+
+int esth_kb_fuse_mode(const esth_engine* e, int lm) {
+    return (e && lm >= 0 && lm <= 10) ? e->batch_fusion[lm].load(std::memory_order_relaxed) : ESTH_FUSION_AUTO;
+}
+
+bool esth_kb_fuse_pays(const esth_engine* e, int model, int S, int lm) {
+    return esth_kb_fuse_with(esth_kb_fuse_mode(e, lm), model, S, lm);
+}
+
+static int fusion_class(uint32_t m) {
+    for (int lm = 6; lm <= 10; ++lm)
+        if (m == (1u << lm)) return lm;
+    return -1;
+}
+
+extern "C" int esth_engine_set_batch_fusion(esth_engine* e, uint32_t m, int mode) {
+    const int lm = fusion_class(m);
+    if (!e || lm < 0 || (mode != ESTH_FUSION_AUTO && mode != ESTH_FUSION_OFF && mode != ESTH_FUSION_ON))
+        return esth_fail(ESTH_ERR_INVALID, "batch fusion: m %u, mode %d", m, mode);
+    e->batch_fusion[lm].store(mode, std::memory_order_relaxed);
+    return ESTH_OK;
+}
+
+extern "C" int esth_engine_batch_fusion(const esth_engine* e, uint32_t m, int* mode) {
+    const int lm = fusion_class(m);
+    if (!e || !mode || lm < 0) return esth_fail(ESTH_ERR_INVALID, "batch fusion: m %u", m);
+    *mode = e->batch_fusion[lm].load(std::memory_order_relaxed);
+    return ESTH_OK;
 }
 
 struct esth_batch {
@@ -309,7 +362,7 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
     /* size classes whose sub-filters sample and sort in one launch; the
      * other filters' particles go through kb_sample_weight's chunks */
     bool fuse[11] = {};
-    for (int lm = 7; lm <= 10; ++lm) fuse[lm] = esth_kb_fuse_pays(b->d.model, S, lm);
+    for (int lm = 6; lm <= 10; ++lm) fuse[lm] = esth_kb_fuse_pays(e, b->d.model, S, lm);
     auto class_of = [](uint32_t m) { int k = 0; while ((1u << k) < m) ++k; return k; };
     for (int i = 0; i < n; ++i)
         if (!fuse[class_of(b->hf[i].m)]) b->nchunks += (b->hf[i].N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
@@ -372,6 +425,13 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
         rc = balloc(b, sizeof(uint2) * c.n, (void**)&c.d_sub);
         if (rc == ESTH_OK && hipMemcpy(c.d_sub, by_class[lm].data(), sizeof(uint2) * c.n, hipMemcpyHostToDevice) != hipSuccess)
             rc = esth_fail(ESTH_ERR_DEVICE, "batch class upload");
+        if (rc == ESTH_OK && c.fused && lm == 6) {
+            std::vector<uint2> units(c.n);
+            c.nunits = (uint32_t)esth_pair_units(by_class[lm].data(), c.n, units.data());
+            rc = balloc(b, sizeof(uint2) * c.nunits, (void**)&c.d_units);
+            if (rc == ESTH_OK && hipMemcpy(c.d_units, units.data(), sizeof(uint2) * c.nunits, hipMemcpyHostToDevice) != hipSuccess)
+                rc = esth_fail(ESTH_ERR_DEVICE, "batch unit upload");
+        }
         if (rc != ESTH_OK) { batch_free(b); return rc; }
         b->cls.push_back(c);
     }
@@ -430,7 +490,7 @@ static void launch_kb_resample(esth_batch* b, const esth_bclass& c, const float*
                        log_w, b->d.topology, argmax_blocks);
 }
 
-/* sample+weight and sort of a fused size class (m >= 128) of n sub-filters
+/* sample+weight and sort of a fused size class (m >= 64) of n sub-filters
  * (filter, sub-filter) in `sub`, filters from F */
 template <int MODEL, int S>
 static int launch_kbss_m(int lm, const esth_bfilter* F, const uint2* sub, uint32_t n, const float* R,
@@ -442,6 +502,10 @@ static int launch_kbss_m(int lm, const esth_bfilter* F, const uint2* sub, uint32
     hipLaunchKernelGGL((kb_sample_sort<MODEL, LM, S>), dim3(n), dim3(ESTH_WAVE * esth_ss_waves<LM>()), 0, st, F, \
                        sub, R, frames, log_w, thr, keep_all)
         switch (lm) {
+        case 6:   /* sub: n units (esth_pair_units), one per wave, four waves per work-group */
+            hipLaunchKernelGGL((kb_sample_sort64<MODEL, S>), dim3((n + 3) / 4), dim3(256), 0, st, F, sub, n, R, frames,
+                               log_w, thr, keep_all);
+            break;
         case 7: ESTH_KBSS(7); break;
         case 8: ESTH_KBSS(8); break;
         case 9: ESTH_KBSS(9); break;
@@ -490,7 +554,8 @@ static int batch_kernels_s(esth_batch* b, int p) {
         }));
     for (const esth_bclass& c : b->cls) {
         if (c.fused)
-            ESTH_TRY(esth_launch_kb_sample_sort(b->d.model, S, c.logm, b->d_F, c.d_sub, c.n, R, frames, log_w,
+            ESTH_TRY(esth_launch_kb_sample_sort(b->d.model, S, c.logm, b->d_F, c.logm == 6 ? c.d_units : c.d_sub,
+                                                c.logm == 6 ? c.nunits : c.n, R, frames, log_w,
                                                 b->d.resampling_threshold,
                                                 b->d.resampler == ESTH_RESAMPLE_ALIAS && b->d.adaptive, s));
         else
@@ -697,40 +762,87 @@ extern "C" int esth_batch_info(esth_batch* b, uint64_t* particles, int* streams,
 }
 
 /* ============================================================ autotuner
- * Device cost of the step's kernels (sample+weight, sort, argmax, exchange,
- * resampling) per million particles for each sub-filter size 64..1024, measured
- * on a 2^20-particle filter of the given model and settings. */
+ * The batch path itself, timed per sub-filter size: for m = 64 ... 1024 a
+ * one-class batch of 64 filters x 65,536 particles (2^22) with base's model
+ * and settings takes 3 untimed steps and 10 timed ones (device time of the
+ * graph-replayed step, median), once with sample+weight and the sort as two
+ * launches and once fused where the fused kernel fits.  us_per_mparticle[k]
+ * is the faster form's cost per million particles, and the engine's batch
+ * fusion for that m is set to the faster form.  A size the descriptor
+ * rejects (e.g. t too large for m) costs +inf. */
+static int time_batch_class(esth_engine* e, const esth_filter_desc* base, const void* params, size_t psize,
+                            const void* control, const void* measurement, float dt, uint32_t m, float* us_per_m) {
+    const int nf = 64;
+    const uint32_t P = 1u << 16;
+    std::vector<esth_filter_desc> descs(nf, *base);
+    for (int i = 0; i < nf; ++i) {
+        descs[i].num_particles = m;
+        descs[i].num_blocks = P / m;
+        descs[i].torus_dimx = 0;
+        descs[i].torus_dimy = 0;
+        descs[i].rng = ESTH_RNG_MTGP_PER_FILTER;
+        descs[i].state_layout = ESTH_LAYOUT_SOA;
+        descs[i].seed = base->seed + 7919u * (uint32_t)i;
+    }
+    int S, V;
+    if (esth_check_desc(&descs[0], &S, &V) != ESTH_OK) {
+        *us_per_m = INFINITY;
+        return ESTH_OK;
+    }
+    std::vector<unsigned char> pbuf(psize * nf);
+    for (int i = 0; i < nf && psize; ++i) memcpy(pbuf.data() + psize * i, params, psize);
+    esth_batch* b = nullptr;
+    ESTH_TRY(esth_batch_create(e, nf, descs.data(), psize ? pbuf.data() : nullptr, psize, &b));
+    std::vector<float> zero((size_t)nf * S, 0.0f), dts(nf, dt);
+    int rc = esth_batch_init_uniform(b, zero.data());
+    if (rc == ESTH_OK) rc = esth_batch_use_graph(b, 1);
+    for (int k = 0; rc == ESTH_OK && k < 3; ++k) rc = esth_batch_step(b, control, 0, measurement, 0, dts.data());
+    float us[16];
+    int cnt = 0;
+    if (rc == ESTH_OK) rc = esth_batch_step_times(b, us, 16, &cnt);
+    for (int k = 0; rc == ESTH_OK && k < 10; ++k) rc = esth_batch_step(b, control, 0, measurement, 0, dts.data());
+    if (rc == ESTH_OK) rc = esth_batch_step_times(b, us, 16, &cnt);
+    esth_batch_destroy(b);
+    ESTH_TRY(rc);
+    if (cnt <= 0) return esth_fail(ESTH_ERR_STATE, "autotune: no timed steps");
+    std::sort(us, us + cnt);
+    *us_per_m = us[cnt / 2] / ((float)nf * (float)P * 1e-6f);
+    return ESTH_OK;
+}
+
 extern "C" int esth_autotune(esth_engine* e, const esth_filter_desc* base, const void* params, size_t params_size,
                              const void* control, const void* measurement, float dt, float us_per_mparticle[5]) {
-    if (!e || !base || !us_per_mparticle) return esth_fail(ESTH_ERR_INVALID, "null");
+    if (!e || !base || !us_per_mparticle || !measurement) return esth_fail(ESTH_ERR_INVALID, "null");
+    size_t psize = 0;
+    ESTH_TRY(esth_params_size(base->model, &psize));
+    if (psize && (!params || params_size < psize))
+        return esth_fail(ESTH_ERR_INVALID, "autotune: params of %zu bytes, model needs %zu", params_size, psize);
+    int S, V;
+    ESTH_TRY(esth_model_dims(base->model, &S, &V) ? esth_fail(ESTH_ERR_INVALID, "model") : ESTH_OK);
     for (int k = 0; k < 5; ++k) {
-        esth_filter_desc d = *base;
-        d.num_particles = 64u << k;
-        d.num_blocks = (1u << 20) / d.num_particles;
-        d.torus_dimx = 0;
-        d.torus_dimy = 0;
-        d.rng = ESTH_RNG_MTGP_PER_FILTER;
-        esth_filter* f = nullptr;
-        ESTH_TRY(esth_filter_create(e, &d, params, params_size, &f));
-        int S, V;
-        esth_model_dims(d.model, &S, &V);
-        std::vector<float> zero(S, 0.0f);
-        int rc = esth_filter_init_uniform(f, zero.data());
-        for (int i = 0; rc == ESTH_OK && i < 3; ++i) rc = esth_filter_step(f, control, measurement, dt);
-        if (rc == ESTH_OK) rc = esth_filter_sync(f);
-        if (rc == ESTH_OK) rc = esth_filter_set_timing(f, 2);
-        for (int i = 0; rc == ESTH_OK && i < 10; ++i) rc = esth_filter_step(f, control, measurement, dt);
-        float us[6];
-        if (rc == ESTH_OK) rc = esth_filter_stage_times(f, us);
-        esth_filter_destroy(f);
-        if (rc != ESTH_OK) return rc;
-        us_per_mparticle[k] = us[1] + us[2] + us[3] + us[4] + us[5];
+        const uint32_t m = 64u << k;
+        const int lm = 6 + k;
+        const int saved = esth_kb_fuse_mode(e, lm);
+        float c_off = INFINITY, c_on = INFINITY;
+        ESTH_TRY(esth_engine_set_batch_fusion(e, m, ESTH_FUSION_OFF));
+        int rc = time_batch_class(e, base, params, psize, control, measurement, dt, m, &c_off);
+        if (rc == ESTH_OK && kb_fuse_grade(base->model, S, lm) > 0) {
+            rc = esth_engine_set_batch_fusion(e, m, ESTH_FUSION_ON);
+            if (rc == ESTH_OK) rc = time_batch_class(e, base, params, psize, control, measurement, dt, m, &c_on);
+        }
+        if (rc != ESTH_OK) {
+            esth_engine_set_batch_fusion(e, m, saved);
+            return rc;
+        }
+        us_per_mparticle[k] = std::min(c_off, c_on);
+        ESTH_TRY(esth_engine_set_batch_fusion(e, m, c_on < c_off ? ESTH_FUSION_ON : ESTH_FUSION_OFF));
     }
     return ESTH_OK;
 }
 
 /* Device time per step of `base`'s filter in each state layout: 3 warm-up
- * steps, then 10 steps timed stage by stage (rng wait excluded). */
+ * steps, then 10 whole steps bracketed by events on the filter's stream
+ * (sample+weight and the sort fused as the filter runs them). */
 extern "C" int esth_autotune_layout(esth_engine* e, const esth_filter_desc* base, const void* params,
                                     size_t params_size, const void* control, const void* measurement, float dt,
                                     float us_per_step[3], int* best) {
@@ -745,17 +857,9 @@ extern "C" int esth_autotune_layout(esth_engine* e, const esth_filter_desc* base
         std::vector<float> zero(S, 0.0f);
         int rc = esth_filter_init_uniform(f, zero.data());
         for (int i = 0; rc == ESTH_OK && i < 3; ++i) rc = esth_filter_step(f, control, measurement, dt);
-        if (rc == ESTH_OK) rc = esth_filter_sync(f);
-        if (rc == ESTH_OK) rc = esth_filter_set_timing(f, 2);
-        for (int i = 0; rc == ESTH_OK && i < 10; ++i) rc = esth_filter_step(f, control, measurement, dt);
-        float us[6];
-        if (rc == ESTH_OK) rc = esth_filter_stage_times(f, us);
+        if (rc == ESTH_OK) rc = esth_filter_time_steps(f, 10, control, measurement, dt, &us_per_step[L]);
         esth_filter_destroy(f);
-        if (rc != ESTH_OK) return rc;
-        float tot = 0.0f;
-        for (int k = 1; k < 6; ++k)
-            if (us[k] == us[k]) tot += us[k];
-        us_per_step[L] = tot;
+        ESTH_TRY(rc);
     }
     if (best) {
         int b = ESTH_LAYOUT_SOA;

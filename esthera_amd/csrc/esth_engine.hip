@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <cstring>
 
 #include "esth_internal.h"
@@ -280,8 +281,18 @@ int esth_launch_mtgp(const uint32_t* d_params, const uint32_t* d_status_in, uint
     if (nsteps > 1 && d_status_in == d_status_out)
         return esth_fail(ESTH_ERR_INVALID, "multi-step MTGP32 launch needs separate status buffers");
     const uint32_t mask = g_mtgp_table[3];
-    hipLaunchKernelGGL(k_mtgp, dim3(nstreams), dim3(ESTH_MTGP_THREADS), 0, s, d_params, mask, d_status_in,
-                       d_status_out, d_streams, (uint32_t*)d_out, float_mode, nsteps, out_step, status_step);
+    /* at most 8 resident work-groups per CU; more streams loop in the grid */
+    static std::atomic<int> cus[64];
+    int dev = 0;
+    ESTH_HIP(hipGetDevice(&dev));
+    int ncu = (dev >= 0 && dev < 64) ? cus[dev].load(std::memory_order_relaxed) : 0;
+    if (ncu <= 0) {
+        ESTH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (dev >= 0 && dev < 64) cus[dev].store(ncu, std::memory_order_relaxed);
+    }
+    const int grid = std::min(nstreams, 8 * std::max(ncu, 1));
+    hipLaunchKernelGGL(k_mtgp, dim3(grid), dim3(ESTH_MTGP_THREADS), 0, s, d_params, mask, d_status_in, d_status_out,
+                       d_streams, (uint32_t)nstreams, (uint32_t*)d_out, float_mode, nsteps, out_step, status_step);
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }

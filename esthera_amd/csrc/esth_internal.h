@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstddef>
 #include <cstdint>
@@ -39,6 +40,10 @@ struct esth_engine {
     size_t used = 0;
     std::mutex mu;
     uint32_t* d_params = nullptr;   /* 200 x 52 MTGP parameter words */
+    std::atomic<int> batch_fusion[11];   /* per log2(m): ESTH_FUSION_* */
+    esth_engine() {
+        for (auto& f : batch_fusion) f.store(ESTH_FUSION_AUTO, std::memory_order_relaxed);
+    }
 };
 
 int esth_engine_alloc(esth_engine* e, size_t bytes, void** p);
@@ -68,7 +73,22 @@ int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_
 int esth_waves_per_simd(const void* kernel, int threads);   /* resident waves per SIMD (registers, LDS) */
 /* table-driven (batch, server) sample+weight + sort for size class lm: taken
  * where kb_sample_sort keeps kb_sample_weight's occupancy (cached answer) */
-bool esth_kb_fuse_pays(int model, int S, int lm);
+/* whether batches and servers of engine e sample and sort size class lm
+ * (m = 2^lm) in one launch: the engine's setting (esth_engine_set_batch_fusion),
+ * by default the fused kernel where it keeps as many waves resident as the
+ * separate sampling */
+bool esth_kb_fuse_pays(const esth_engine* e, int model, int S, int lm);
+/* the same decision from a mode read once (esth_kb_fuse_mode), so that the
+ * tables and launches of one server wave agree */
+int esth_kb_fuse_mode(const esth_engine* e, int lm);
+/* mean device time of n whole steps of f (events on its stream around them) */
+int esth_filter_time_steps(esth_filter* f, int n, const void* control, const void* measurement, float dt,
+                           float* us_per_step);
+bool esth_kb_fuse_with(int mode, int model, int S, int lm);
+/* the m = 64 fused class works on units of two consecutive sub-filters of one
+ * filter (kb_sample_sort64): from a class list ordered by filter and
+ * sub-filter, writes the units to `units` (room for n) and returns their count */
+size_t esth_pair_units(const uint2* cls, size_t n, uint2* units);
 int esth_launch_kb_sample_sort(int model, int S, int lm, const esth_bfilter* F, const uint2* sub, uint32_t n,
                                const float* R, const float* frames, int log_w, float thr, int keep_all,
                                hipStream_t st);

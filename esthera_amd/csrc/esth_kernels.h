@@ -176,14 +176,20 @@ __device__ __forceinline__ void esth_mtgp_wg(const uint32_t* __restrict__ params
     }
 }
 
+/* streams j = blockIdx.x, blockIdx.x + gridDim.x, ...: a launch over many
+ * short streams (a batch's per-filter streams) keeps a resident grid instead
+ * of one work-group launch per stream */
 static __global__ __launch_bounds__(256) void k_mtgp(const uint32_t* __restrict__ params, uint32_t mask,
                                               const uint32_t* status_in, uint32_t* status_out,
-                                              const esth_mtgp_stream* __restrict__ streams,
+                                              const esth_mtgp_stream* __restrict__ streams, uint32_t nstreams,
                                               uint32_t* __restrict__ out, int float_mode, int nsteps,
                                               uint64_t out_step, uint64_t status_step) {
     __shared__ uint32_t st[ESTH_MTGP_LARGE + 32];
-    esth_mtgp_wg(params, mask, status_in, status_out, streams, blockIdx.x, out, float_mode, nsteps, out_step,
-                 status_step, st);
+    for (uint32_t j = blockIdx.x; j < nstreams; j += gridDim.x) {
+        esth_mtgp_wg(params, mask, status_in, status_out, streams, j, out, float_mode, nsteps, out_step, status_step,
+                     st);
+        __syncthreads();   /* the ring and tables are reloaded for the next stream */
+    }
 }
 
 /* one step of stream j for a sampling launch, kept out of line so that its
@@ -2072,6 +2078,72 @@ __attribute__((amdgpu_waves_per_eu(esth_kbss_waves_per_eu<MODEL>()))) void kb_sa
     if (threadIdx.x >= ESTH_WAVE) return;
     esth_sort_sub<LOGM, S, false, true>(f.w, nullptr, f.tmp, f.N, f.G, e.y, true, (float2*)f.winner, nullptr, f.t,
                                         f.bt_w, f.bt_s, f.part, log_w, nullptr, ESTH_LAYOUT_SOA, lds);
+}
+
+/* kb_sample_sort for the m = 64 class, over units of one filter's
+ * sub-filters g0 and g0 + 1 (unit = (filter, g0 | single << 31); a single
+ * unit has sub-filter g0 only): a wave samples and weights the unit's
+ * particles (lanes 0-31 the first sub-filter's 32 pairs, lanes 32-63 the
+ * second's) into its own LDS, then sorts the sub-filters one after the other
+ * (esth_sort_sub, FROM_LDS).  A unit never spans two filters, so the model
+ * hook's frame reads stay wave-uniform.  Per-particle arithmetic and the sort
+ * are those of kb_sample_weight + kb_sort. */
+#define ESTH_UNIT_SINGLE 0x80000000u
+template <int MODEL, int S>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(esth_kbss_waves_per_eu<MODEL>()))) void kb_sample_sort64(
+    const esth_bfilter* __restrict__ F, const uint2* __restrict__ units, uint32_t nunits, const float* __restrict__ R,
+    const float* __restrict__ frames, int log_w, float thr, int keep_all) {
+    using T = esth_model_traits<MODEL>;
+    static_assert(T::S == S, "state dimension of the model");
+    constexpr int M = 64;
+    constexpr int SUB = (1 + S) * M;   /* LDS floats of one sub-filter: weights, then the planes */
+    __shared__ __align__(16) float lds[4][2 * SUB];
+    const int lane = threadIdx.x & (ESTH_WAVE - 1);
+    const int wv = threadIdx.x / ESTH_WAVE;
+    float* L = lds[wv];
+    const uint32_t u = blockIdx.x * 4u + (uint32_t)wv;
+    const bool valid = u < nunits;
+    const uint2 e = units[valid ? u : 0];
+    const esth_bfilter& f = F[e.x];
+    const uint32_t g0 = e.y & ~ESTH_UNIT_SINGLE;
+    const int nsubs = (e.y & ESTH_UNIT_SINGLE) ? 1 : 2;
+    const int half = lane / 32;
+    if (valid && half < nsubs) {
+        const int q = 2 * (lane & 31);
+        const uint32_t g = g0 + (uint32_t)half;
+        const int carry = (int)(f.flags & 1u);
+        const float* Rf = R + f.r_off;
+        const bool keep = !carry && (keep_all || (Rf[f.uoff + f.N + g] - 1.0f) > thr);
+        const uint32_t p = g * (uint32_t)M + (uint32_t)q;
+        float s0[S], s1[S], u0[T::V], u1[T::V];
+        esth_load_pair<S, ESTH_LAYOUT_SOA>(f.state, f.N, p, s0, s1);
+#pragma unroll
+        for (int k = 0; k < T::V; ++k) {
+            const float2 uu = *reinterpret_cast<const float2*>(Rf + (size_t)k * f.N + p);
+            u0[k] = uu.x;
+            u1[k] = uu.y;
+        }
+        float lw0, lw1;
+        esth_sample_pair_regs<MODEL>(s0, s1, u0, u1, frames + f.f_off, lw0, lw1);
+        if (keep) esth_store_pair<S, ESTH_LAYOUT_SOA>(f.state, f.N, p, s0, s1);
+        float2 wo = log_w ? make_float2(lw0, lw1) : make_float2(esth_expf(lw0), esth_expf(lw1));
+        if (carry) {
+            const float2 wp = *reinterpret_cast<const float2*>(f.w + p);
+            wo = log_w ? make_float2(wp.x + wo.x, wp.y + wo.y) : make_float2(wp.x * wo.x, wp.y * wo.y);
+        }
+        float* Lh = L + half * SUB;
+        *reinterpret_cast<float2*>(Lh + q) = wo;
+#pragma unroll
+        for (int c = 0; c < S; ++c) *reinterpret_cast<float2*>(Lh + (1 + c) * M + q) = make_float2(s0[c], s1[c]);
+    }
+    __syncthreads();
+    if (!valid) return;
+#pragma unroll 1
+    for (int j = 0; j < nsubs; ++j)
+        esth_sort_sub<6, S, false, true>(f.w, nullptr, f.tmp, f.N, f.G, g0 + (uint32_t)j, true, (float2*)f.winner,
+                                         nullptr, f.t, f.bt_w, f.bt_s, f.part, log_w, nullptr, ESTH_LAYOUT_SOA,
+                                         L + j * SUB);
 }
 
 template <int S>

@@ -167,6 +167,8 @@ struct esth_wave_group {
     uint64_t sub0 = 0, nsub = 0;    /* their sub-filters in sub_all */
     uint64_t chk0 = 0, nchk = 0;    /* their 64-pair chunks (sample+weight) */
     uint64_t cls0[11] = {0}, ncls[11] = {0};   /* per log2(m) class, in sub_cls */
+    uint64_t unit0 = 0, nunit = 0;  /* the fused m = 64 class's units (esth_pair_units), in units */
+    uint32_t fuse_bits = 0;         /* bit lm: class lm samples inside kb_sample_sort */
 };
 
 struct esth_wslot {
@@ -686,8 +688,8 @@ static void srv_alias(const esth_bfilter* F, const uint2* sub, uint32_t n, const
 
 template <int S>
 static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wave_group& g, const esth_bfilter* F,
-                         const uint2* sub_all, const uint2* sub_cls, const uint2* chunks, const float* R,
-                         const float* frames) {
+                         const uint2* sub_all, const uint2* sub_cls, const uint2* units, const uint2* chunks,
+                         const float* R, const float* frames) {
     hipStream_t st = s->stream;
     const int log_w = k.weights == ESTH_WEIGHTS_LOG;
     const esth_bfilter* Fg = F + g.f0;
@@ -704,9 +706,10 @@ static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wa
         }));
     for (int lm = 6; lm <= 10; ++lm) {
         if (!g.ncls[lm]) continue;
-        if (esth_kb_fuse_pays(k.model, S, lm))
-            ESTH_TRY(esth_launch_kb_sample_sort(k.model, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], R, frames,
-                                                log_w, k.thr, k.resampler == ESTH_RESAMPLE_ALIAS && k.adaptive, st));
+        if ((g.fuse_bits >> lm) & 1u)
+            ESTH_TRY(esth_launch_kb_sample_sort(k.model, S, lm, Fg, lm == 6 ? units + g.unit0 : sub_cls + g.cls0[lm],
+                                                (uint32_t)(lm == 6 ? g.nunit : g.ncls[lm]), R, frames, log_w, k.thr,
+                                                k.resampler == ESTH_RESAMPLE_ALIAS && k.adaptive, st));
         else
             ESTH_SRV_LOGM(srv_sort, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], log_w, st);
     }
@@ -777,19 +780,24 @@ static int launch_wave(esth_server* s, bool* launched) {
         return a.t->group < b.t->group;
     });
     const size_t nw = upd.size();
+    /* the engine's fusion setting, read once for the whole wave */
+    int fmode[11];
+    for (int lm = 0; lm <= 10; ++lm) fmode[lm] = esth_kb_fuse_mode(s->e, lm);
+    auto fused = [&](const esth_strack* t) { return esth_kb_fuse_with(fmode[t->logm], t->d.model, t->S, t->logm); };
     uint64_t words = 0, nsub = 0, nstreams = 0, frames = 0, nchk = 0;
     for (auto& u : upd) {
         words += u.t->words;
         nsub += u.t->d.num_blocks;
         /* fused size classes sample inside kb_sample_sort, without chunks */
-        if (!esth_kb_fuse_pays(u.t->d.model, u.t->S, u.t->logm)) nchk += (u.t->bf.N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
+        if (!fused(u.t)) nchk += (u.t->bf.N / 2 + ESTH_WAVE - 1) / ESTH_WAVE;
         nstreams += (uint64_t)u.t->K;
         frames += u.t->frame_floats;
     }
     const size_t o_F = 0;
     const size_t o_all = align_up(o_F + sizeof(esth_bfilter) * nw, ESTH_SRV_ALIGN);
     const size_t o_cls = align_up(o_all + sizeof(uint2) * nsub, ESTH_SRV_ALIGN);
-    const size_t o_chk = align_up(o_cls + sizeof(uint2) * nsub, ESTH_SRV_ALIGN);
+    const size_t o_unt = align_up(o_cls + sizeof(uint2) * nsub, ESTH_SRV_ALIGN);
+    const size_t o_chk = align_up(o_unt + sizeof(uint2) * nsub, ESTH_SRV_ALIGN);
     const size_t o_str = align_up(o_chk + sizeof(uint2) * nchk, ESTH_SRV_ALIGN);
     const size_t o_frm = align_up(o_str + sizeof(esth_mtgp_stream) * nstreams, ESTH_SRV_ALIGN);
     const size_t o_ini = align_up(o_frm + sizeof(float) * frames, ESTH_SRV_ALIGN);
@@ -820,6 +828,7 @@ static int launch_wave(esth_server* s, bool* launched) {
     esth_bfilter* hF = (esth_bfilter*)(h + o_F);
     uint2* hall = (uint2*)(h + o_all);
     uint2* hcls = (uint2*)(h + o_cls);
+    uint2* hunt = (uint2*)(h + o_unt);
     uint2* hchk = (uint2*)(h + o_chk);
     esth_mtgp_stream* hst = (esth_mtgp_stream*)(h + o_str);
     float* hfr = (float*)(h + o_frm);
@@ -847,7 +856,9 @@ static int launch_wave(esth_server* s, bool* launched) {
         const uint32_t li = (uint32_t)(i - g.f0);
         for (uint32_t b = 0; b < t->d.num_blocks; ++b) hall[sub_pos + b] = make_uint2(li, b);
         sub_pos += t->d.num_blocks;
-        if (!esth_kb_fuse_pays(t->d.model, t->S, t->logm))
+        if (fused(t))
+            g.fuse_bits |= 1u << t->logm;
+        else
             for (uint32_t q = 0; q < t->bf.N / 2; q += ESTH_WAVE) hchk[chk_pos++] = make_uint2(li, q);
         g.nchk = chk_pos - g.chk0;
         g.nf++;
@@ -872,6 +883,13 @@ static int launch_wave(esth_server* s, bool* launched) {
             const esth_strack* t = upd[g.f0 + hall[j].x].t;
             hcls[c0[t->logm]++] = hall[j];
         }
+    }
+    uint64_t upos = 0;
+    for (esth_wave_group& g : groups) {
+        g.unit0 = upos;
+        if (g.ncls[6] && ((g.fuse_bits >> 6) & 1u))
+            g.nunit = esth_pair_units(hcls + g.cls0[6], g.ncls[6], hunt + upos);
+        upos += g.nunit;
     }
     for (size_t i = 0; i < inits.size(); ++i) {
         esth_strack* t = inits[i];
@@ -912,10 +930,11 @@ static int launch_wave(esth_server* s, bool* launched) {
         const esth_sgroup_key& k = s->keys[g.key];
         const uint2* sa = (const uint2*)(dt + o_all);
         const uint2* sc = (const uint2*)(dt + o_cls);
+        const uint2* su = (const uint2*)(dt + o_unt);
         const uint2* ck = (const uint2*)(dt + o_chk);
         const float* fr = (const float*)(dt + o_frm);
         ESTH_TRY(esth_with_model_state_dim(upd[g.f0].t->S, [&](auto sv) {
-            return group_kernels<decltype(sv)::value>(s, k, g, dF, sa, sc, ck, s->R[p], fr);
+            return group_kernels<decltype(sv)::value>(s, k, g, dF, sa, sc, su, ck, s->R[p], fr);
         }));
     }
     ESTH_HIP(hipGetLastError());

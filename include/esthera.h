@@ -244,6 +244,19 @@ typedef struct esth_filter esth_filter;
 int esth_engine_create(int device, size_t hbm_budget, esth_engine** out);
 int esth_engine_destroy(esth_engine* e);
 int esth_engine_memory(const esth_engine* e, size_t* used, size_t* budget);
+/* Batches and servers: sample+weight and the block sort of sub-filter size
+ * class m (64, 128, 256, 512 or 1024) in one launch (ESTH_FUSION_ON), in two
+ * (ESTH_FUSION_OFF), or as the engine judges from the kernels' register use
+ * (ESTH_FUSION_AUTO, the default).  ON falls back to two launches where the
+ * fused kernel does not fit the model (it would spill).  Takes effect for
+ * batches created and server waves formed after the call; esth_autotune
+ * sets it from measurements.  No counterpart in the reference (one filter,
+ * one launch per stage, pf_generic.c:521-585). */
+#define ESTH_FUSION_AUTO (-1)
+#define ESTH_FUSION_OFF 0
+#define ESTH_FUSION_ON 1
+int esth_engine_set_batch_fusion(esth_engine* e, uint32_t m, int mode);
+int esth_engine_batch_fusion(const esth_engine* e, uint32_t m, int* mode);
 
 /* Replaces the set-up half of particle_filter() (pf_generic.c:245-436). */
 int esth_filter_create(esth_engine* e, const esth_filter_desc* desc, const void* params,
@@ -393,18 +406,23 @@ int esth_batch_step_times(esth_batch* b, float* us, int max, int* count);
 int esth_batch_info(esth_batch* b, uint64_t* particles, int* streams, uint32_t* sub_filters);
 
 /* ------------------------------------------------------------ autotuner
- * Measures, on this device, the cost of a step's kernels per million
- * particles for sub-filter sizes 64, 128, 256, 512, 1024 (a 2^20-particle
- * filter with `base`'s model and settings and the given control/measurement).
- * esth_autotune_pick then chooses a filter's sub-filter size: among the powers
- * of two in [max(64, min_m), 1024] dividing `particles`, the cheapest (ties:
- * larger); if none is >= min_m, the largest divisor. */
+ * Measures, on this device, the batch path's step cost per million particles
+ * for sub-filter sizes 64, 128, 256, 512, 1024: a one-class batch of 64
+ * filters x 65,536 particles with `base`'s model and settings and the given
+ * control/measurement, graph-replayed, median of 10 steps, with sample+weight
+ * and the sort as two launches and as one (where the fused kernel fits); the
+ * cost is the faster form's, and the engine's batch fusion for that size
+ * (esth_engine_set_batch_fusion) is set to it.  Sizes the descriptor rejects
+ * cost +inf.  esth_autotune_pick then chooses a filter's sub-filter size:
+ * among the powers of two in [max(64, min_m), 1024] dividing `particles`,
+ * the cheapest (ties: larger); if none is >= min_m, the largest divisor. */
 int esth_autotune(esth_engine* e, const esth_filter_desc* base, const void* params, size_t params_size,
                   const void* control, const void* measurement, float dt, float us_per_mparticle[5]);
 int esth_autotune_pick(const float us_per_mparticle[5], uint32_t particles, uint32_t min_m, uint32_t* m_out);
 /* Measures, on this device, the device time per step in microseconds of a
  * filter shaped as `base` (its m, G, model and settings) with each state
- * layout SOA, AOS, PACKED; *best receives the fastest layout. */
+ * layout SOA, AOS, PACKED (10 whole steps after 3 warm-up steps);
+ * *best receives the fastest layout. */
 int esth_autotune_layout(esth_engine* e, const esth_filter_desc* base, const void* params, size_t params_size,
                          const void* control, const void* measurement, float dt, float us_per_step[3],
                          int* best);

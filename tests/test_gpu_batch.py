@@ -5,6 +5,7 @@ to the oracle's run of that filter, in eager and graph-replay mode, for each
 resampler and topology; posterior means within 1e-4 * |m64| + 1e-6 of the
 oracle's FP64 twin."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -33,6 +34,7 @@ def check_vs_oracle(i, st, w, est, ref, S):
     assert np.all(np.abs(mean - m64) <= 1e-4 * np.abs(m64) + 1e-6), f"filter {i}: {mean} vs FP64 {m64}"
 
 pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def make_filters(nf, seed, sizes, topology=ea.TOPO_RING, t=1, resampler=ea.RESAMPLE_SYSTEMATIC, threshold=1.0,
@@ -203,13 +205,80 @@ def test_batch_info_and_memory(engine):
 
 
 def test_autotune_measures_all_sizes(engine):
+    """The autotuner times the batch path per size class and sets the
+    engine's fusion choice for each; a size the descriptor rejects (t = 40
+    on a ring needs m > 64) costs +inf instead of failing the sweep."""
     p, ctrl, meas, truth = ea.unicycle_scenario(99, 8, 20.0, 2)
     d = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=4, t=1, topology=ea.TOPO_RING, resampler=ea.RESAMPLE_SYSTEMATIC,
                        weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=1)
-    us = ea.autotune(engine, d, p, ctrl[0], meas[0], 0.1)
-    assert us.shape == (5,) and np.all(us > 0) and np.all(np.isfinite(us))
-    m = ea.autotune_pick(us, 65536, 256)
-    assert m in (256, 512, 1024)
+    try:
+        us = ea.autotune(engine, d, p, ctrl[0], meas[0], 0.1)
+        assert us.shape == (5,) and np.all(us > 0) and np.all(np.isfinite(us))
+        assert all(engine.batch_fusion(64 << k) in (ea.FUSION_ON, ea.FUSION_OFF) for k in range(5))
+        m = ea.autotune_pick(us, 65536, 256)
+        assert m in (256, 512, 1024)
+        d.num_transfer = 40
+        us = ea.autotune(engine, d, p, ctrl[0], meas[0], 0.1)
+        assert np.isinf(us[0]) and np.all(np.isfinite(us[1:]))
+    finally:
+        for k in range(5):
+            engine.set_batch_fusion(64 << k, ea.FUSION_AUTO)
+
+
+@pytest.mark.parametrize("model", [ea.MODEL_UNICYCLE, ea.MODEL_ROBOTARM5])
+def test_batch_fused_and_unfused_classes_identical(engine, model):
+    """Every size class sampled and sorted in one launch and in two: the
+    same particles, bit for bit (the autotuner picks either form)."""
+    sizes = [(64, 5), (128, 3), (64, 2), (256, 2), (512, 2), (1024, 1), (64, 1)]
+    nf, steps = 7, 3
+    descs, params, scen, inits = [], [], [], []
+    meas_ra, ctrl_ra, dts_ra, _ = ea.load_robotarm_trace(os.path.join(HERE, "golden", "robotarm_input5.txt"))
+    for i in range(nf):
+        m, G = sizes[i]
+        descs.append(ea.filter_desc(model, m=m, G=G, t=1, topology=ea.TOPO_RING, resampler=ea.RESAMPLE_SYSTEMATIC,
+                                    weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=300 + i))
+        if model == ea.MODEL_UNICYCLE:
+            p, c, ms, _ = ea.unicycle_scenario(40 + i, 8, 20.0, steps)
+            params.append(p)
+            scen.append((c, ms, [0.1] * steps))
+            inits.append(np.zeros(3, np.float32))
+        else:
+            params.append(None)
+            scen.append((ctrl_ra, meas_ra, dts_ra))
+            inits.append(np.array([0.75 * np.pi] * 5 + [7.0, 0.0, 0.0, 0.0], np.float32))
+    S = ea.STATE_DIM[model]
+    out = {}
+    try:
+        for mode in (ea.FUSION_OFF, ea.FUSION_ON):
+            for k in range(5):
+                engine.set_batch_fusion(64 << k, mode)
+            b = ea.Batch(engine, descs, params if model == ea.MODEL_UNICYCLE else None)
+            b.init_uniform(np.array(inits, np.float32))
+            for k in range(steps):
+                ct = ea.CONTROL_TYPE[model]
+                mt = ea.MEAS_TYPE[model]
+                ctrls = (ct * nf)(*[sc[0][k] for sc in scen])
+                meas = (mt * nf)(*[sc[1][k] for sc in scen])
+                b.step(ctrls, meas, np.array([sc[2][k] for sc in scen], np.float32))
+            out[mode] = [b.particles(i) for i in range(nf)]
+            b.close()
+    finally:
+        for k in range(5):
+            engine.set_batch_fusion(64 << k, ea.FUSION_AUTO)
+    for i in range(nf):
+        (sa, wa), (sb, wb) = out[ea.FUSION_OFF][i], out[ea.FUSION_ON][i]
+        assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32)), f"filter {i} state"
+        assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32)), f"filter {i} weights"
+    # and the oracle agrees with both
+    for i in (0, 1, 5):
+        o = O.Filter(descs[i], params[i], C.sizeof(params[i]) if params[i] is not None else 0)
+        o.init_uniform(inits[i])
+        for k in range(steps):
+            o.step(scen[i][0][k], scen[i][1][k], float(scen[i][2][k]))
+        ost, ow = o.read(S)
+        sa, wa = out[ea.FUSION_ON][i]
+        assert np.array_equal(sa.view(np.uint32), ost.view(np.uint32)), f"filter {i} vs oracle"
+        assert np.array_equal(wa.view(np.uint32), ow.view(np.uint32)), f"filter {i} weights vs oracle"
 
 
 def test_many_filters_parallel_frames(engine):

@@ -13,7 +13,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 GiB = 1 << 30
-SIZES = [(64, 6), (128, 3), (256, 4), (512, 2), (1024, 2)]
+SIZES = [(64, 5), (128, 3), (64, 3), (256, 4), (512, 2), (1024, 2), (64, 6)]
 
 
 def rw3_specs(nf, steps, seed):
