@@ -1305,6 +1305,11 @@ __device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, 
     }
     float x[E];
     esth_load_run<E>(w + base + le, x);
+    /* the sorted particles' planes, loaded here so that their latency runs
+     * beside the block max and the scan instead of after them */
+    float in[S][E];
+#pragma unroll
+    for (int c = 0; c < S; ++c) esth_load_run<E>(st_tmp + (size_t)c * N + base + le, in[c]);
     /* outputs: particle h M/2 + lane + 64 r */
     const int ob = h * (M / 2) + lane;
     float uu[E];
@@ -1364,17 +1369,15 @@ __device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, 
     /* the sorted particles (with the received ones), staged for the gather */
 #pragma unroll
     for (int c = 0; c < S; ++c) {
-        float in[E];
-        esth_load_run<E>(st_tmp + (size_t)c * N + base + le, in);
         if (recv) {
             const size_t Gt = (size_t)xch.G * xch.t;
 #pragma unroll
             for (int r = 0; r < E; ++r)
-                if (rsrc[r] >= 0) in[r] = xch.bt_s[(size_t)c * Gt + (size_t)rsrc[r]];
+                if (rsrc[r] >= 0) in[c][r] = xch.bt_s[(size_t)c * Gt + (size_t)rsrc[r]];
         }
-        esth_store_run<E>(stage + c * M + le, in);
+        esth_store_run<E>(stage + c * M + le, in[c]);
         /* carried weights: a skipped sub-filter keeps its sorted, exchanged particles */
-        if (!go && carry && valid) esth_store_plane_run<E, S>(st_main, layout, N, c, base + le, in);
+        if (!go && carry && valid) esth_store_plane_run<E, S>(st_main, layout, N, c, base + le, in[c]);
     }
     __syncthreads();
     const float u0 = rnd[base] - 1.0f;
