@@ -355,7 +355,7 @@ class Engine:
 
     def set_batch_fusion(self, m, mode):
         """Batches/servers: sub-filter size class m sampled and sorted in one
-        launch (FUSION_ON), two (FUSION_OFF) or as the engine judges (FUSION_AUTO)."""
+        launch (FUSION_ON), two (FUSION_OFF) or as the engine decides (FUSION_AUTO)."""
         check(lib().esth_engine_set_batch_fusion(self.h, m, mode))
 
     def batch_fusion(self, m):

@@ -246,7 +246,7 @@ int esth_engine_destroy(esth_engine* e);
 int esth_engine_memory(const esth_engine* e, size_t* used, size_t* budget);
 /* Batches and servers: sample+weight and the block sort of sub-filter size
  * class m (64, 128, 256, 512 or 1024) in one launch (ESTH_FUSION_ON), in two
- * (ESTH_FUSION_OFF), or as the engine judges from the kernels' register use
+ * (ESTH_FUSION_OFF), or as the engine decides from the kernels' register use
  * (ESTH_FUSION_AUTO, the default).  ON falls back to two launches where the
  * fused kernel does not fit the model (it would spill).  Takes effect for
  * batches created and server waves formed after the call; esth_autotune
