@@ -25,7 +25,7 @@ MODEL_ROBOTARM5, MODEL_UNICYCLE, MODEL_DH6, MODEL_UNICYCLE10, MODEL_ROBOTARM5_CA
 TOPO_NONE, TOPO_RING, TOPO_TORUS, TOPO_ALL2ALL = 0, 1, 2, 3
 RESAMPLE_MULTINOMIAL, RESAMPLE_SYSTEMATIC, RESAMPLE_ALIAS = 0, 1, 2
 WEIGHTS_LINEAR, WEIGHTS_LOG = 0, 1
-RNG_MTGP_COMPAT, RNG_MTGP_PER_FILTER = 0, 1
+RNG_MTGP_COMPAT, RNG_MTGP_PER_FILTER, RNG_MTGP_PER_FILTER_LONG = 0, 1, 2
 LAYOUT_SOA, LAYOUT_AOS, LAYOUT_PACKED = 0, 1, 2
 FUSION_AUTO, FUSION_OFF, FUSION_ON = -1, 0, 1
 FRAME_FLOATS = 336

@@ -273,8 +273,8 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
             d.resampler != d0.resampler || d.resampling_threshold != d0.resampling_threshold ||
             d.adaptive != d0.adaptive || d.weights != d0.weights)
             return esth_fail(ESTH_ERR_INVALID, "batch: filter %d differs from filter 0 in model/topology/t/resampler/weights", i);
-        if (d.rng != ESTH_RNG_MTGP_PER_FILTER)
-            return esth_fail(ESTH_ERR_INVALID, "batch: filter %d must use ESTH_RNG_MTGP_PER_FILTER", i);
+        if (!esth_rng_is_per_filter(d.rng))
+            return esth_fail(ESTH_ERR_INVALID, "batch: filter %d must use a per-filter RNG mode", i);
         if (d.state_layout != ESTH_LAYOUT_SOA)
             return esth_fail(ESTH_ERR_UNSUPPORTED, "batch: filter %d: batches keep their state in ESTH_LAYOUT_SOA", i);
         if (d.num_particles < 64) return esth_fail(ESTH_ERR_UNSUPPORTED, "batch: sub-filters need m >= 64 (filter %d)", i);
@@ -310,7 +310,7 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
         b->frame_stride = std::max(b->frame_stride, frame_floats(d.model, pp));
         const uint64_t N = (uint64_t)d.num_particles * d.num_blocks;
         const uint64_t need = esth_rng_needed(V, d.resampler, d.adaptive, d.num_particles, d.num_blocks);
-        b->K[i] = esth_rng_streams(ESTH_RNG_MTGP_PER_FILTER, need);
+        b->K[i] = esth_rng_streams(descs[i].rng, need);
         b->rsize[i] = esth_rng_stream_size(b->K[i], need);
         b->soff[i] = b->Ntot * (uint64_t)S;
         b->woff[i] = b->Ntot;
@@ -780,7 +780,7 @@ static int time_batch_class(esth_engine* e, const esth_filter_desc* base, const 
         descs[i].num_blocks = P / m;
         descs[i].torus_dimx = 0;
         descs[i].torus_dimy = 0;
-        descs[i].rng = ESTH_RNG_MTGP_PER_FILTER;
+        descs[i].rng = esth_rng_is_per_filter(base->rng) ? base->rng : ESTH_RNG_MTGP_PER_FILTER;
         descs[i].state_layout = ESTH_LAYOUT_SOA;
         descs[i].seed = base->seed + 7919u * (uint32_t)i;
     }

@@ -165,7 +165,7 @@ int esth_check_desc(const esth_filter_desc* d, int* S, int* V) {
         return esth_fail(ESTH_ERR_INVALID, "resampling threshold %f outside [0, 1]", (double)d->resampling_threshold);
     if (d->weights != ESTH_WEIGHTS_LINEAR && d->weights != ESTH_WEIGHTS_LOG)
         return esth_fail(ESTH_ERR_INVALID, "bad weights mode %d", d->weights);
-    if (d->rng != ESTH_RNG_MTGP_COMPAT && d->rng != ESTH_RNG_MTGP_PER_FILTER)
+    if (d->rng != ESTH_RNG_MTGP_COMPAT && !esth_rng_is_per_filter(d->rng))
         return esth_fail(ESTH_ERR_INVALID, "bad rng mode %d", d->rng);
     if (d->state_layout < ESTH_LAYOUT_SOA || d->state_layout > ESTH_LAYOUT_PACKED)
         return esth_fail(ESTH_ERR_INVALID, "bad state layout %d", d->state_layout);

@@ -439,8 +439,8 @@ extern "C" int esth_server_open(esth_server* s, const esth_filter_desc* desc, co
     if (!s || !desc || !init_state) return esth_fail(ESTH_ERR_INVALID, "server open: null argument");
     int S, V;
     ESTH_TRY(esth_check_desc(desc, &S, &V));
-    if (desc->rng != ESTH_RNG_MTGP_PER_FILTER)
-        return esth_fail(ESTH_ERR_INVALID, "server open: tracks use ESTH_RNG_MTGP_PER_FILTER");
+    if (!esth_rng_is_per_filter(desc->rng))
+        return esth_fail(ESTH_ERR_INVALID, "server open: tracks use a per-filter RNG mode");
     if (desc->num_particles < 64) return esth_fail(ESTH_ERR_UNSUPPORTED, "server open: sub-filters need m >= 64");
     if (desc->state_layout != ESTH_LAYOUT_SOA)
         return esth_fail(ESTH_ERR_UNSUPPORTED, "server open: tracks keep their state in ESTH_LAYOUT_SOA");
@@ -461,7 +461,7 @@ extern "C" int esth_server_open(esth_server* s, const esth_filter_desc* desc, co
     memcpy(t->init, init_state, sizeof(float) * (size_t)S);
     while ((1u << t->logm) < desc->num_particles) ++t->logm;
     const uint64_t need = esth_rng_needed(V, desc->resampler, desc->adaptive, desc->num_particles, desc->num_blocks);
-    t->K = esth_rng_streams(ESTH_RNG_MTGP_PER_FILTER, need);
+    t->K = esth_rng_streams(desc->rng, need);
     t->rsize = (uint32_t)esth_rng_stream_size(t->K, need);
     t->words = (uint64_t)t->rsize * (uint64_t)t->K;
     t->frame_floats = align_up(srv_frame_floats(desc->model, psize ? t->params.data() : nullptr), 4);

@@ -3,12 +3,13 @@
 batched_10k: 10,000 independent unicycle filters, each with its own set of 8
 landmarks (64 seeded sets), particle counts log-uniform in [256, 65536]
 rounded to multiples of 64 (seed 99), ring exchange t = 1, systematic
-resampling, log weights, per-filter MTGP32 streams; the sub-filter size of
+resampling, log weights, per-filter MTGP32 streams of about 16 rounds each
+(ESTH_RNG_MTGP_PER_FILTER_LONG: the throughput form); the sub-filter size of
 each filter comes from the autotuner's per-size costs.
 """
 import numpy as np
 
-from . import (MODEL_UNICYCLE, RESAMPLE_SYSTEMATIC, RNG_MTGP_PER_FILTER, TOPO_RING, WEIGHTS_LOG, UnicycleControl,
+from . import (MODEL_UNICYCLE, RESAMPLE_SYSTEMATIC, RNG_MTGP_PER_FILTER_LONG, TOPO_RING, WEIGHTS_LOG, UnicycleControl,
                UnicycleMeasurement, autotune_pick, filter_desc, unicycle_scenario)
 
 BATCH_SCENARIOS = 64
@@ -41,7 +42,7 @@ def batched_filters(us_per_mparticle, nf=10000, seed=99, scen=None, sub_filter_s
         if int(n) // m < 2:
             m //= 2
         descs.append(filter_desc(MODEL_UNICYCLE, m=m, G=int(n) // m, t=1, topology=TOPO_RING,
-                                 resampler=RESAMPLE_SYSTEMATIC, weights=WEIGHTS_LOG, rng=RNG_MTGP_PER_FILTER,
+                                 resampler=RESAMPLE_SYSTEMATIC, weights=WEIGHTS_LOG, rng=RNG_MTGP_PER_FILTER_LONG,
                                  seed=seed * 1000003 + i))
         params.append(scen[i % len(scen)][0])
     return counts, descs, params, scen

@@ -172,9 +172,15 @@ int esth_model_build_frame(int model, const void* params, const void* control,
  * pf_generic.c:208-210,282; mtgp32-opencl.c:46-48):
  *   COMPAT     : 96 streams, exactly the reference's buffer.
  *   PER_FILTER : stream count chosen from the filter's own size, so a filter's
- *                random numbers never depend on other filters in a batch. */
+ *                random numbers never depend on other filters in a batch;
+ *                about 4 rounds of 768 words per stream and step (short
+ *                chains: the lowest latency of one filter's words).
+ *   PER_FILTER_LONG : the same with about 16 rounds per stream (a quarter of
+ *                the streams): less status traffic and fewer work-groups per
+ *                word, for throughput over many filters (batches). */
 #define ESTH_RNG_MTGP_COMPAT     0
 #define ESTH_RNG_MTGP_PER_FILTER 1
+#define ESTH_RNG_MTGP_PER_FILTER_LONG 2
 
 /* Device layout of a filter's particle state (S floats per particle, N
  * particles), the three the reference ships as model variants:

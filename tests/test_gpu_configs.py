@@ -87,7 +87,7 @@ def test_batched_10k_filters_vs_oracle(engine):
     scenario's truth; the smallest and largest filters and three others are
     bit-identical to the oracle, posterior means within the FP64 bar."""
     d0 = ea.filter_desc(ea.MODEL_UNICYCLE, m=256, G=4, t=1, topology=ea.TOPO_RING,
-                        resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER, seed=1)
+                        resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER_LONG, seed=1)
     scen = workloads.batched_scenarios()
     p0, c0, m0, _ = scen[0]
     us = ea.autotune(engine, d0, p0, c0[0], m0[0], 0.1)

@@ -116,6 +116,9 @@ CASES = [
     ("randomwalk3-reference-linear", dict(model=ea.MODEL_RANDOMWALK3, m=64, G=16, t=1)),
     ("randomwalk3-alias-carry-gate", dict(model=ea.MODEL_RANDOMWALK3, m=256, G=8, t=2, resampler=ea.RESAMPLE_ALIAS,
                                           weights=ea.WEIGHTS_LOG, carry=1, threshold=0.5)),
+    ("unicycle-per-filter-long-rng", dict(model=ea.MODEL_UNICYCLE, m=512, G=24, t=2, topology=ea.TOPO_TORUS,
+                                          resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG,
+                                          rng=ea.RNG_MTGP_PER_FILTER_LONG)),
     ("unicycle-small-m", dict(model=ea.MODEL_UNICYCLE, m=16, G=12, t=2, weights=ea.WEIGHTS_LOG,
                               rng=ea.RNG_MTGP_PER_FILTER)),
 ]

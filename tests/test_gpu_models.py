@@ -17,12 +17,14 @@ SIZES = [(64, 5), (128, 3), (64, 3), (256, 4), (512, 2), (1024, 2), (64, 6)]
 
 
 def rw3_specs(nf, steps, seed):
+    """nf randomwalk3 filters of the SIZES mix; every third with the
+    long-stream per-filter RNG mode."""
     specs = []
     for i in range(nf):
         m, G = SIZES[i % len(SIZES)]
+        rng = ea.RNG_MTGP_PER_FILTER_LONG if i % 3 == 2 else ea.RNG_MTGP_PER_FILTER
         d = ea.filter_desc(ea.MODEL_RANDOMWALK3, m=m, G=G, t=1, topology=ea.TOPO_RING,
-                           resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, rng=ea.RNG_MTGP_PER_FILTER,
-                           seed=seed + i)
+                           resampler=ea.RESAMPLE_SYSTEMATIC, weights=ea.WEIGHTS_LOG, rng=rng, seed=seed + i)
         p, ctrl, meas, _ = ea.randomwalk3_scenario(seed + 100 + i, steps, sigma_q=0.05 + 0.01 * i)
         specs.append((d, p, [(ctrl[k], meas[k], 0.1) for k in range(steps)]))
     return specs
