@@ -172,6 +172,8 @@ _SIGS = {
     "esth_engine_memory": ([_P, _P, _P], C.c_int),
     "esth_engine_set_batch_fusion": ([_P, C.c_uint32, C.c_int], C.c_int),
     "esth_engine_batch_fusion": ([_P, C.c_uint32, _P], C.c_int),
+    "esth_engine_set_batch_resample_lanes": ([_P, C.c_uint32, C.c_int], C.c_int),
+    "esth_engine_batch_resample_lanes": ([_P, C.c_uint32, _P], C.c_int),
     "esth_filter_create": ([_P, _P, _P, C.c_size_t, _P], C.c_int),
     "esth_filter_destroy": ([_P], C.c_int),
     "esth_filter_init_uniform": ([_P, _P], C.c_int),
@@ -362,6 +364,16 @@ class Engine:
         mode = C.c_int()
         check(lib().esth_engine_batch_fusion(self.h, m, C.byref(mode)))
         return mode.value
+
+    def set_batch_resample_lanes(self, m, lanes):
+        """Batches/servers: lanes per sub-filter of size class m's resampling:
+        32 (m = 64), 64 (one wave), 128 (m >= 256); 0 the default."""
+        check(lib().esth_engine_set_batch_resample_lanes(self.h, m, lanes))
+
+    def batch_resample_lanes(self, m):
+        lanes = C.c_int()
+        check(lib().esth_engine_batch_resample_lanes(self.h, m, C.byref(lanes)))
+        return lanes.value
 
     def close(self):
         if self.h:

@@ -88,6 +88,7 @@ struct esth_pool {
 
 struct esth_bclass {
     int logm = 0;
+    int lanes = 64;       /* the resampling's lanes per sub-filter (esth_kb_resample_lanes) */
     uint32_t n = 0;
     uint2* d_sub = nullptr;
     bool fused = false;   /* sample+weight and sort in one launch (kb_sample_sort) */
@@ -173,6 +174,30 @@ extern "C" int esth_engine_set_batch_fusion(esth_engine* e, uint32_t m, int mode
     if (!e || lm < 0 || (mode != ESTH_FUSION_AUTO && mode != ESTH_FUSION_OFF && mode != ESTH_FUSION_ON))
         return esth_fail(ESTH_ERR_INVALID, "batch fusion: m %u, mode %d", m, mode);
     e->batch_fusion[lm].store(mode, std::memory_order_relaxed);
+    return ESTH_OK;
+}
+
+static bool lanes_valid(int lm, int lanes) {
+    return lanes == 0 || lanes == 64 || (lanes == 32 && lm == 6) || (lanes == 128 && lm >= 8);
+}
+
+int esth_kb_resample_lanes(const esth_engine* e, int lm) {
+    const int l = (e && lm >= 0 && lm <= 10) ? e->batch_lanes[lm].load(std::memory_order_relaxed) : 0;
+    return (l && lanes_valid(lm, l)) ? l : 64;
+}
+
+extern "C" int esth_engine_set_batch_resample_lanes(esth_engine* e, uint32_t m, int lanes) {
+    const int lm = fusion_class(m);
+    if (!e || lm < 0 || !lanes_valid(lm, lanes))
+        return esth_fail(ESTH_ERR_INVALID, "batch resampling: m %u with %d lanes per sub-filter", m, lanes);
+    e->batch_lanes[lm].store(lanes, std::memory_order_relaxed);
+    return ESTH_OK;
+}
+
+extern "C" int esth_engine_batch_resample_lanes(const esth_engine* e, uint32_t m, int* lanes) {
+    const int lm = fusion_class(m);
+    if (!e || !lanes || lm < 0) return esth_fail(ESTH_ERR_INVALID, "batch resampling: m %u", m);
+    *lanes = esth_kb_resample_lanes(e, lm);
     return ESTH_OK;
 }
 
@@ -422,6 +447,7 @@ extern "C" int esth_batch_create(esth_engine* e, int n, const esth_filter_desc* 
         c.logm = lm;
         c.n = (uint32_t)by_class[lm].size();
         c.fused = fuse[lm];
+        c.lanes = esth_kb_resample_lanes(e, lm);
         rc = balloc(b, sizeof(uint2) * c.n, (void**)&c.d_sub);
         if (rc == ESTH_OK && hipMemcpy(c.d_sub, by_class[lm].data(), sizeof(uint2) * c.n, hipMemcpyHostToDevice) != hipSuccess)
             rc = esth_fail(ESTH_ERR_DEVICE, "batch class upload");
@@ -481,13 +507,45 @@ static void launch_kb_alias(esth_batch* b, const esth_bclass& c, const float* R,
     hipLaunchKernelGGL((kb_alias_w<LOGM, S>), dim3((c.n + WPB - 1) / WPB), dim3(WPB * ESTH_WAVE), 0, b->stream, b->d_F,
                        c.d_sub, c.n, R, b->d.resampling_threshold, b->d.adaptive, log_w);
 }
-/* argmax_blocks: the first launch after the sorts also runs every filter's
- * argmax (one work-group each, ahead of the resampling work-groups) */
+/* am: work-groups running every filter's argmax ahead of the resampling
+ * work-groups (the server's first resampling launch) */
 template <int LOGM, int S>
-static void launch_kb_resample(esth_batch* b, const esth_bclass& c, const float* R, int log_w, uint32_t argmax_blocks) {
-    hipLaunchKernelGGL((kb_resample<LOGM, S>), dim3(argmax_blocks + (c.n + 3) / 4), dim3(256), 0, b->stream, b->d_F,
-                       c.d_sub, c.n, R, b->d.resampling_threshold, b->d.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0,
-                       log_w, b->d.topology, argmax_blocks);
+static void launch_kb_resample_t(int lanes, const esth_bfilter* F, const uint2* sub, uint32_t n, const float* R,
+                                 float thr, int sys, int log_w, int topo, uint32_t am, hipStream_t st) {
+    if constexpr (LOGM == 6) {
+        if (lanes == 32) {   /* two entries per wave, four waves per work-group */
+            hipLaunchKernelGGL((kb_resample64x2<S>), dim3(am + (n + 7) / 8), dim3(256), 0, st, F, sub, n, R, thr, sys,
+                               log_w, topo, am);
+            return;
+        }
+    }
+    if constexpr (LOGM >= 8) {
+        if (lanes == 128) {   /* two waves per entry, two entries per work-group */
+            hipLaunchKernelGGL((kb_resample2<LOGM, S>), dim3(am + (n + 1) / 2), dim3(256), 0, st, F, sub, n, R, thr,
+                               sys, log_w, topo, am);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((kb_resample<LOGM, S>), dim3(am + (n + 3) / 4), dim3(256), 0, st, F, sub, n, R, thr, sys, log_w,
+                       topo, am);
+}
+
+int esth_launch_kb_resample(int S, int lm, int lanes, const esth_bfilter* F, const uint2* sub, uint32_t n,
+                            const float* R, float thr, int systematic, int log_w, int topo, uint32_t argmax_blocks,
+                            hipStream_t st) {
+    if (n == 0 && argmax_blocks == 0) return ESTH_OK;
+    return esth_with_model_state_dim(S, [&](auto sv) {
+        constexpr int SV = decltype(sv)::value;
+        switch (lm) {
+        case 6: launch_kb_resample_t<6, SV>(lanes, F, sub, n, R, thr, systematic, log_w, topo, argmax_blocks, st); break;
+        case 7: launch_kb_resample_t<7, SV>(lanes, F, sub, n, R, thr, systematic, log_w, topo, argmax_blocks, st); break;
+        case 8: launch_kb_resample_t<8, SV>(lanes, F, sub, n, R, thr, systematic, log_w, topo, argmax_blocks, st); break;
+        case 9: launch_kb_resample_t<9, SV>(lanes, F, sub, n, R, thr, systematic, log_w, topo, argmax_blocks, st); break;
+        case 10: launch_kb_resample_t<10, SV>(lanes, F, sub, n, R, thr, systematic, log_w, topo, argmax_blocks, st); break;
+        default: return esth_fail(ESTH_ERR_INVALID, "m = 2^%d", lm);
+        }
+        return ESTH_OK;
+    });
 }
 
 /* sample+weight and sort of a fused size class (m >= 64) of n sub-filters
@@ -575,7 +633,10 @@ static int batch_kernels_s(esth_batch* b, int p) {
     } else {
         /* the exchange's receiving side runs inside the resampling */
         hipLaunchKernelGGL(kb_argmax<S>, dim3(b->n), dim3(256), 0, s, b->d_F, log_w);
-        for (const esth_bclass& c : b->cls) ESTH_BCLASS(launch_kb_resample, S, b, c, R, log_w, 0u);
+        for (const esth_bclass& c : b->cls)
+            ESTH_TRY(esth_launch_kb_resample(S, c.logm, c.lanes, b->d_F, c.d_sub, c.n, R, b->d.resampling_threshold,
+                                             b->d.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0, log_w, b->d.topology,
+                                             0u, b->stream));
     }
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
@@ -766,10 +827,12 @@ extern "C" int esth_batch_info(esth_batch* b, uint64_t* particles, int* streams,
  * one-class batch of 64 filters x 65,536 particles (2^22) with base's model
  * and settings takes 3 untimed steps and 10 timed ones (device time of the
  * graph-replayed step, median), once with sample+weight and the sort as two
- * launches and once fused where the fused kernel fits.  us_per_mparticle[k]
- * is the faster form's cost per million particles, and the engine's batch
- * fusion for that m is set to the faster form.  A size the descriptor
- * rejects (e.g. t too large for m) costs +inf. */
+ * launches and once fused where the fused kernel fits; then, at the faster
+ * form, once more with the resampling's other tiling (32 lanes per
+ * sub-filter at m = 64, 128 at m >= 256).  us_per_mparticle[k] is the best
+ * cost per million particles, and the engine's batch fusion and resampling
+ * tiling for that m are set to it.  A size the descriptor rejects (e.g. t
+ * too large for m) costs +inf. */
 static int time_batch_class(esth_engine* e, const esth_filter_desc* base, const void* params, size_t psize,
                             const void* control, const void* measurement, float dt, uint32_t m, float* us_per_m) {
     const int nf = 64;
@@ -836,6 +899,23 @@ extern "C" int esth_autotune(esth_engine* e, const esth_filter_desc* base, const
         }
         us_per_mparticle[k] = std::min(c_off, c_on);
         ESTH_TRY(esth_engine_set_batch_fusion(e, m, c_on < c_off ? ESTH_FUSION_ON : ESTH_FUSION_OFF));
+        /* the resampling's other tiling at the chosen form: two sub-filters per
+         * wave (m = 64) or two waves per sub-filter (m >= 256) */
+        const int alt = lm == 6 ? 32 : (lm >= 8 ? 128 : 0);
+        ESTH_TRY(esth_engine_set_batch_resample_lanes(e, m, 64));
+        if (alt && us_per_mparticle[k] < INFINITY) {
+            float c_alt = INFINITY;
+            ESTH_TRY(esth_engine_set_batch_resample_lanes(e, m, alt));
+            rc = time_batch_class(e, base, params, psize, control, measurement, dt, m, &c_alt);
+            if (rc != ESTH_OK) {
+                esth_engine_set_batch_resample_lanes(e, m, 64);
+                return rc;
+            }
+            if (c_alt < us_per_mparticle[k])
+                us_per_mparticle[k] = c_alt;
+            else
+                ESTH_TRY(esth_engine_set_batch_resample_lanes(e, m, 64));
+        }
     }
     return ESTH_OK;
 }

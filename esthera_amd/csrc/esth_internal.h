@@ -41,8 +41,10 @@ struct esth_engine {
     std::mutex mu;
     uint32_t* d_params = nullptr;   /* 200 x 52 MTGP parameter words */
     std::atomic<int> batch_fusion[11];   /* per log2(m): ESTH_FUSION_* */
+    std::atomic<int> batch_lanes[11];    /* per log2(m): lanes per sub-filter of the resampling, 0 auto */
     esth_engine() {
         for (auto& f : batch_fusion) f.store(ESTH_FUSION_AUTO, std::memory_order_relaxed);
+        for (auto& l : batch_lanes) l.store(0, std::memory_order_relaxed);
     }
 };
 
@@ -81,6 +83,16 @@ bool esth_kb_fuse_pays(const esth_engine* e, int model, int S, int lm);
 /* the same decision from a mode read once (esth_kb_fuse_mode), so that the
  * tables and launches of one server wave agree */
 int esth_kb_fuse_mode(const esth_engine* e, int lm);
+/* lanes per sub-filter of the batched resampling of class lm (m = 2^lm) on
+ * engine e: 32 (m = 64: two sub-filters per wave), 64 (one wave) or 128 (two
+ * waves, m >= 256), as set by esth_engine_set_batch_resample_lanes, 64 by
+ * default */
+int esth_kb_resample_lanes(const esth_engine* e, int lm);
+/* the batched resampling of n entries of class lm at that tiling; work-groups
+ * below argmax_blocks run the argmax of filters 0 .. argmax_blocks - 1 */
+int esth_launch_kb_resample(int S, int lm, int lanes, const esth_bfilter* F, const uint2* sub, uint32_t n,
+                            const float* R, float thr, int systematic, int log_w, int topo, uint32_t argmax_blocks,
+                            hipStream_t st);
 /* mean device time of n whole steps of f (events on its stream around them) */
 int esth_filter_time_steps(esth_filter* f, int n, const void* control, const void* measurement, float dt,
                            float* us_per_step);

@@ -398,13 +398,18 @@ __global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state
  * pos & stride == 0 swaps iff (w[pos] > w[pos+stride]) == ddd, where
  * ddd = (pos & size) != 0 for size < m and 0 in the final merge.  Keys equal
  * to each other therefore swap in descending segments, exactly as there. */
-template <int LOGM>
+__host__ __device__ constexpr int esth_ilog2(int x) { return x <= 1 ? 0 : 1 + esth_ilog2(x / 2); }
+
+/* a sub-filter of M = 2^LOGM particles held by LANES lanes (by default one
+ * wave, or all M lanes when M < 64), E = M / LANES consecutive positions each */
+template <int LOGM, int LANES = ((1 << LOGM) >= ESTH_WAVE ? ESTH_WAVE : (1 << LOGM))>
 struct esth_tile {
+    static_assert(LANES >= 1 && LANES <= ESTH_WAVE && (1 << LOGM) % LANES == 0, "lanes per sub-filter");
     static constexpr int M = 1 << LOGM;
-    static constexpr int E = M >= ESTH_WAVE ? M / ESTH_WAVE : 1;   /* elements per lane */
-    static constexpr int LPS = M / E;                              /* lanes per sub-filter */
-    static constexpr int LOGE = LOGM >= 6 ? LOGM - 6 : 0;
-    static constexpr int LOGL = LOGM - LOGE;
+    static constexpr int LPS = LANES;                  /* lanes per sub-filter */
+    static constexpr int E = M / LANES;                /* elements per lane */
+    static constexpr int LOGL = esth_ilog2(LANES);
+    static constexpr int LOGE = LOGM - LOGL;
 };
 
 /* a lane's E consecutive floats at p (p aligned to 4 * min(E, 4) bytes) */
@@ -1044,7 +1049,7 @@ struct esth_xch {
 /* one sub-filter per wave (or 64/m per wave); every wave of the work-group
  * must call it (one barrier inside); cdf_w: this wave's 64*E floats of LDS;
  * recv: apply the exchange xch on the way in */
-template <int LOGM, int S, bool HOIST = false>
+template <int LOGM, int S, bool HOIST = false, int LANES = esth_tile<LOGM>::LPS>
 __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                                                   const float* __restrict__ st_tmp,
                                                   const float* w,   /* xch.w_out may write it */
@@ -1054,7 +1059,7 @@ __device__ __forceinline__ void esth_resample_sub(float* __restrict__ st_main,
                                                   bool recv = false, esth_xch xch = esth_xch{},
                                                   int carry = 0, float* w_mut = nullptr,
                                                   int layout = ESTH_LAYOUT_SOA) {
-    using TL = esth_tile<LOGM>;
+    using TL = esth_tile<LOGM, LANES>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS, LOGE = TL::LOGE, LOGL = TL::LOGL;
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
     const int laneg = lane % LPS;
@@ -2240,6 +2245,69 @@ __global__ __launch_bounds__(256) void kb_resample(const esth_bfilter* __restric
     esth_resample_sub<LOGM, S, true>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, valid, thr, systematic, log_w,
                                nullptr, cdf_s[threadIdx.x / ESTH_WAVE], topo != ESTH_TOPO_NONE, x,
                                (int)(f.flags & 1u), f.w);
+}
+
+/* kb_resample's other tilings of a size class; work-groups below
+ * argmax_blocks run filter blockIdx.x's argmax + posterior instead.
+ * kb_resample64x2 (m = 64): 32 lanes per sub-filter, two positions each, so
+ * a wave resamples entries 2w and 2w + 1 of the class list (which may belong
+ * to different filters: nothing here reads a frame).  kb_resample2 (m >= 256):
+ * two waves per sub-filter (esth_resample_sub2), a work-group of four waves
+ * two entries.  The scan's binary tree over positions, the searches and the
+ * selections are the one-wave kernel's, so the particles are the same bit
+ * for bit. */
+template <int S>
+__global__ __launch_bounds__(256) void kb_resample64x2(const esth_bfilter* __restrict__ F,
+                                                       const uint2* __restrict__ sub, uint32_t nsub,
+                                                       const float* __restrict__ R, float thr, int systematic,
+                                                       int log_w, int topo, uint32_t argmax_blocks) {
+    __shared__ float cdf_s[4][2 * ESTH_WAVE];
+    __shared__ float2 sc[128];
+    __shared__ float smax[4];
+    __shared__ double sd[4 * (2 + S)];
+    if (blockIdx.x < argmax_blocks) {
+        const esth_bfilter& f = F[blockIdx.x];
+        esth_argmax_block<S, 256>((const float2*)f.winner, f.G, log_w, f.part, f.tmp, f.N, f.m, (float2*)f.result,
+                                  f.est, sc, smax, sd);
+        return;
+    }
+    const uint32_t wv = ((blockIdx.x - argmax_blocks) * blockDim.x + threadIdx.x) / ESTH_WAVE;
+    const uint32_t i = 2u * wv + (uint32_t)((threadIdx.x & (ESTH_WAVE - 1)) / 32);
+    const bool valid = i < nsub;
+    const uint2 e = sub[valid ? i : nsub - 1];
+    const esth_bfilter& f = F[e.x];
+    const esth_xch x{f.bt_w, f.bt_s, f.top, f.w, f.G, f.t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, f.dimx, f.dimy};
+    esth_resample_sub<6, S, true, 32>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, valid, thr, systematic,
+                                      log_w, nullptr, cdf_s[threadIdx.x / ESTH_WAVE], topo != ESTH_TOPO_NONE, x,
+                                      (int)(f.flags & 1u), f.w);
+}
+
+template <int LOGM, int S>
+__global__ __launch_bounds__(256) void kb_resample2(const esth_bfilter* __restrict__ F, const uint2* __restrict__ sub,
+                                                    uint32_t nsub, const float* __restrict__ R, float thr,
+                                                    int systematic, int log_w, int topo, uint32_t argmax_blocks) {
+    constexpr int M = 1 << LOGM;
+    __shared__ __align__(16) float cdf_s[2][M];
+    __shared__ __align__(16) float stage_s[2][S * M];
+    __shared__ float xl_s[2][4];
+    __shared__ float2 sc[128];
+    __shared__ float smax[4];
+    __shared__ double sd[4 * (2 + S)];
+    if (blockIdx.x < argmax_blocks) {
+        const esth_bfilter& f = F[blockIdx.x];
+        esth_argmax_block<S, 256>((const float2*)f.winner, f.G, log_w, f.part, f.tmp, f.N, f.m, (float2*)f.result,
+                                  f.est, sc, smax, sd);
+        return;
+    }
+    const int wv = threadIdx.x / ESTH_WAVE;   /* 0..3: entry wv / 2, half wv % 2 */
+    const int j = wv >> 1;
+    const uint32_t i = (blockIdx.x - argmax_blocks) * 2u + (uint32_t)j;
+    const bool valid = i < nsub;
+    const uint2 e = sub[valid ? i : nsub - 1];
+    const esth_bfilter& f = F[e.x];
+    const esth_xch x{f.bt_w, f.bt_s, f.top, f.w, f.G, f.t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, f.dimx, f.dimy};
+    esth_resample_sub2<LOGM, S>(f.state, f.tmp, f.w, R + f.r_off + f.uoff, f.N, e.y, valid, thr, systematic, log_w,
+                                cdf_s[j], stage_s[j], xl_s[j], wv & 1, x, (int)(f.flags & 1u), f.w, ESTH_LAYOUT_SOA);
 }
 
 /* one wave per (filter, sub-filter) entry of a size class (m >= 64) */

@@ -169,6 +169,7 @@ struct esth_wave_group {
     uint64_t cls0[11] = {0}, ncls[11] = {0};   /* per log2(m) class, in sub_cls */
     uint64_t unit0 = 0, nunit = 0;  /* the fused m = 64 class's units (esth_pair_units), in units */
     uint32_t fuse_bits = 0;         /* bit lm: class lm samples inside kb_sample_sort */
+    int lanes[11] = {0};            /* the resampling's lanes per sub-filter, per class */
 };
 
 struct esth_wslot {
@@ -661,12 +662,6 @@ template <int LOGM, int S>
 static void srv_sort(const esth_bfilter* F, const uint2* sub, uint32_t n, int log_w, hipStream_t st) {
     hipLaunchKernelGGL((kb_sort<LOGM, S>), dim3((n + 3) / 4), dim3(256), 0, st, F, sub, n, log_w);
 }
-template <int LOGM, int S>
-static void srv_resample(const esth_bfilter* F, const uint2* sub, uint32_t n, const float* R, float thr, int sys,
-                         int log_w, int topo, uint32_t argmax_blocks, hipStream_t st) {
-    hipLaunchKernelGGL((kb_resample<LOGM, S>), dim3(argmax_blocks + (n + 3) / 4), dim3(256), 0, st, F, sub, n, R, thr,
-                       sys, log_w, topo, argmax_blocks);
-}
 
 #define ESTH_SRV_LOGM(FN, S, LM, ...)                  \
     switch (LM) {                                      \
@@ -732,8 +727,8 @@ static int group_kernels(esth_server* s, const esth_sgroup_key& k, const esth_wa
     uint32_t am = g.nf;
     for (int lm = 6; lm <= 10; ++lm) {
         if (!g.ncls[lm]) continue;
-        ESTH_SRV_LOGM(srv_resample, S, lm, Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], R, k.thr,
-                      k.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0, log_w, k.topology, am, st);
+        ESTH_TRY(esth_launch_kb_resample(S, lm, g.lanes[lm], Fg, sub_cls + g.cls0[lm], (uint32_t)g.ncls[lm], R, k.thr,
+                                         k.resampler == ESTH_RESAMPLE_SYSTEMATIC ? 1 : 0, log_w, k.topology, am, st));
         am = 0;
     }
     return ESTH_OK;
@@ -784,6 +779,8 @@ static int launch_wave(esth_server* s, bool* launched) {
     int fmode[11];
     for (int lm = 0; lm <= 10; ++lm) fmode[lm] = esth_kb_fuse_mode(s->e, lm);
     auto fused = [&](const esth_strack* t) { return esth_kb_fuse_with(fmode[t->logm], t->d.model, t->S, t->logm); };
+    int flanes[11];
+    for (int lm = 0; lm <= 10; ++lm) flanes[lm] = esth_kb_resample_lanes(s->e, lm);
     uint64_t words = 0, nsub = 0, nstreams = 0, frames = 0, nchk = 0;
     for (auto& u : upd) {
         words += u.t->words;
@@ -844,6 +841,7 @@ static int launch_wave(esth_server* s, bool* launched) {
             g.f0 = (uint32_t)i;
             g.sub0 = sub_pos;
             g.chk0 = chk_pos;
+            for (int lm = 0; lm <= 10; ++lm) g.lanes[lm] = flanes[lm];
             groups.push_back(g);
         }
         esth_wave_group& g = groups.back();

@@ -263,6 +263,15 @@ int esth_engine_memory(const esth_engine* e, size_t* used, size_t* budget);
 #define ESTH_FUSION_ON 1
 int esth_engine_set_batch_fusion(esth_engine* e, uint32_t m, int mode);
 int esth_engine_batch_fusion(const esth_engine* e, uint32_t m, int* mode);
+/* Batches and servers: the tiling of the prefix-sum resampling of size class
+ * m, as lanes per sub-filter: 32 (m = 64 only: two sub-filters per wave, two
+ * positions per lane), 64 (one wave per sub-filter, the default), 128 (m >=
+ * 256: two waves per sub-filter); 0 restores the default.  Results are the
+ * same bit for bit at every tiling.  Takes effect for batches created and
+ * server waves formed after the call; esth_autotune sets it from
+ * measurements. */
+int esth_engine_set_batch_resample_lanes(esth_engine* e, uint32_t m, int lanes);
+int esth_engine_batch_resample_lanes(const esth_engine* e, uint32_t m, int* lanes);
 
 /* Replaces the set-up half of particle_filter() (pf_generic.c:245-436). */
 int esth_filter_create(esth_engine* e, const esth_filter_desc* desc, const void* params,
@@ -416,9 +425,12 @@ int esth_batch_info(esth_batch* b, uint64_t* particles, int* streams, uint32_t* 
  * for sub-filter sizes 64, 128, 256, 512, 1024: a one-class batch of 64
  * filters x 65,536 particles with `base`'s model and settings and the given
  * control/measurement, graph-replayed, median of 10 steps, with sample+weight
- * and the sort as two launches and as one (where the fused kernel fits); the
- * cost is the faster form's, and the engine's batch fusion for that size
- * (esth_engine_set_batch_fusion) is set to it.  Sizes the descriptor rejects
+ * and the sort as two launches and as one (where the fused kernel fits),
+ * then at the faster form with the resampling's other tiling (32 lanes per
+ * sub-filter at m = 64, 128 at m >= 256); the cost is the best of these, and
+ * the engine's batch fusion and resampling tiling for that size
+ * (esth_engine_set_batch_fusion, esth_engine_set_batch_resample_lanes) are
+ * set to it.  Sizes the descriptor rejects
  * cost +inf.  esth_autotune_pick then chooses a filter's sub-filter size:
  * among the powers of two in [max(64, min_m), 1024] dividing `particles`,
  * the cheapest (ties: larger); if none is >= min_m, the largest divisor. */

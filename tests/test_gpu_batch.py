@@ -215,6 +215,8 @@ def test_autotune_measures_all_sizes(engine):
         us = ea.autotune(engine, d, p, ctrl[0], meas[0], 0.1)
         assert us.shape == (5,) and np.all(us > 0) and np.all(np.isfinite(us))
         assert all(engine.batch_fusion(64 << k) in (ea.FUSION_ON, ea.FUSION_OFF) for k in range(5))
+        assert engine.batch_resample_lanes(64) in (32, 64) and engine.batch_resample_lanes(128) == 64
+        assert all(engine.batch_resample_lanes(64 << k) in (64, 128) for k in range(2, 5))
         m = ea.autotune_pick(us, 65536, 256)
         assert m in (256, 512, 1024)
         d.num_transfer = 40
@@ -223,12 +225,15 @@ def test_autotune_measures_all_sizes(engine):
     finally:
         for k in range(5):
             engine.set_batch_fusion(64 << k, ea.FUSION_AUTO)
+            engine.set_batch_resample_lanes(64 << k, 0)
 
 
 @pytest.mark.parametrize("model", [ea.MODEL_UNICYCLE, ea.MODEL_ROBOTARM5])
 def test_batch_fused_and_unfused_classes_identical(engine, model):
-    """Every size class sampled and sorted in one launch and in two: the
-    same particles, bit for bit (the autotuner picks either form)."""
+    """Every size class sampled and sorted in one launch and in two, and
+    resampled at 64 lanes per sub-filter and at the other tiling (32 at
+    m = 64, 128 at m >= 256): the same particles, bit for bit (the autotuner
+    picks any of these)."""
     sizes = [(64, 5), (128, 3), (64, 2), (256, 2), (512, 2), (1024, 1), (64, 1)]
     nf, steps = 7, 3
     descs, params, scen, inits = [], [], [], []
@@ -252,6 +257,8 @@ def test_batch_fused_and_unfused_classes_identical(engine, model):
         for mode in (ea.FUSION_OFF, ea.FUSION_ON):
             for k in range(5):
                 engine.set_batch_fusion(64 << k, mode)
+                alt = 32 if k == 0 else (128 if k >= 2 else 64)
+                engine.set_batch_resample_lanes(64 << k, 64 if mode == ea.FUSION_OFF else alt)
             b = ea.Batch(engine, descs, params if model == ea.MODEL_UNICYCLE else None)
             b.init_uniform(np.array(inits, np.float32))
             for k in range(steps):
@@ -265,6 +272,7 @@ def test_batch_fused_and_unfused_classes_identical(engine, model):
     finally:
         for k in range(5):
             engine.set_batch_fusion(64 << k, ea.FUSION_AUTO)
+            engine.set_batch_resample_lanes(64 << k, 0)
     for i in range(nf):
         (sa, wa), (sb, wb) = out[ea.FUSION_OFF][i], out[ea.FUSION_ON][i]
         assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32)), f"filter {i} state"

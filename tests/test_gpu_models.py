@@ -61,9 +61,26 @@ def test_randomwalk3_batch_vs_oracle(engine):
     b.close()
 
 
-def test_randomwalk3_served_vs_oracle(engine):
+@pytest.mark.parametrize("tiled", [False, True])
+def test_randomwalk3_served_vs_oracle(engine, tiled):
+    """Served randomwalk3 tracks against the oracle; `tiled`: every class
+    fused and resampled at the other tiling (32 lanes at m = 64, two waves at
+    m >= 256)."""
     steps = 4
     specs = rw3_specs(5, steps, 900)
+    if tiled:
+        for k in range(5):
+            engine.set_batch_fusion(64 << k, ea.FUSION_ON)
+            engine.set_batch_resample_lanes(64 << k, 32 if k == 0 else (128 if k >= 2 else 64))
+    try:
+        served_vs_oracle(engine, specs, steps)
+    finally:
+        for k in range(5):
+            engine.set_batch_fusion(64 << k, ea.FUSION_AUTO)
+            engine.set_batch_resample_lanes(64 << k, 0)
+
+
+def served_vs_oracle(engine, specs, steps):
     srv = ea.Server(engine, 16, GiB)
     tids = [srv.open(d, p, np.zeros(3, np.float32))[0] for d, p, _ in specs]
     expect = {}
