@@ -475,10 +475,24 @@ __device__ __forceinline__ void esth_gather_lds(const float* __restrict__ plane,
     esth_permute_lds<M, E>(in, le, buf, pos, v);
 }
 
-template <int LOGM, int SIZE, int STRIDE>
-__device__ __forceinline__ void esth_bitonic_stage(float (&key)[esth_tile<LOGM>::E],
-                                                   int (&idx)[esth_tile<LOGM>::E], int le, int laneg) {
-    using TL = esth_tile<LOGM>;
+/* a[0] + a[1], then (a[0] + a[1]) + (a[2] + a[3]), ...: the order of a
+ * butterfly sum over K adjacent lanes */
+template <int K>
+__device__ __forceinline__ float esth_tree_sum(float (&a)[K]) {
+    if constexpr (K == 1) {
+        return a[0];
+    } else {
+        float h[K / 2];
+#pragma unroll
+        for (int j = 0; j < K / 2; ++j) h[j] = a[2 * j] + a[2 * j + 1];
+        return esth_tree_sum<K / 2>(h);
+    }
+}
+
+template <int LOGM, int SIZE, int STRIDE, int LANES = esth_tile<LOGM>::LPS>
+__device__ __forceinline__ void esth_bitonic_stage(float (&key)[(esth_tile<LOGM, LANES>::E)],
+                                                   int (&idx)[(esth_tile<LOGM, LANES>::E)], int le, int laneg) {
+    using TL = esth_tile<LOGM, LANES>;
     constexpr int M = TL::M, E = TL::E;
     /* compare-exchanges as selects: a swap under a divergent branch costs an
      * exec-mask save/restore and register moves per pair */
@@ -531,28 +545,31 @@ __device__ __forceinline__ void esth_bitonic_stage(float (&key)[esth_tile<LOGM>:
     }
 }
 
-template <int LOGM, int SIZE, int STRIDE>
-__device__ __forceinline__ void esth_bitonic_strides(float (&key)[esth_tile<LOGM>::E],
-                                                     int (&idx)[esth_tile<LOGM>::E], int le, int laneg) {
+template <int LOGM, int SIZE, int STRIDE, int LANES = esth_tile<LOGM>::LPS>
+__device__ __forceinline__ void esth_bitonic_strides(float (&key)[(esth_tile<LOGM, LANES>::E)],
+                                                     int (&idx)[(esth_tile<LOGM, LANES>::E)], int le, int laneg) {
     if constexpr (STRIDE > 0) {
-        esth_bitonic_stage<LOGM, SIZE, STRIDE>(key, idx, le, laneg);
-        esth_bitonic_strides<LOGM, SIZE, STRIDE / 2>(key, idx, le, laneg);
+        esth_bitonic_stage<LOGM, SIZE, STRIDE, LANES>(key, idx, le, laneg);
+        esth_bitonic_strides<LOGM, SIZE, STRIDE / 2, LANES>(key, idx, le, laneg);
     }
 }
 
-template <int LOGM, int SIZE>
-__device__ __forceinline__ void esth_bitonic_sizes(float (&key)[esth_tile<LOGM>::E],
-                                                   int (&idx)[esth_tile<LOGM>::E], int le, int laneg) {
+template <int LOGM, int SIZE, int LANES = esth_tile<LOGM>::LPS>
+__device__ __forceinline__ void esth_bitonic_sizes(float (&key)[(esth_tile<LOGM, LANES>::E)],
+                                                   int (&idx)[(esth_tile<LOGM, LANES>::E)], int le, int laneg) {
     if constexpr (SIZE <= (1 << LOGM)) {
-        esth_bitonic_strides<LOGM, SIZE, SIZE / 2>(key, idx, le, laneg);
-        esth_bitonic_sizes<LOGM, 2 * SIZE>(key, idx, le, laneg);
+        esth_bitonic_strides<LOGM, SIZE, SIZE / 2, LANES>(key, idx, le, laneg);
+        esth_bitonic_sizes<LOGM, 2 * SIZE, LANES>(key, idx, le, laneg);
     }
 }
 
-template <int LOGM>
-__device__ __forceinline__ void esth_bitonic(float (&key)[esth_tile<LOGM>::E],
-                                             int (&idx)[esth_tile<LOGM>::E], int le) {
-    esth_bitonic_sizes<LOGM, 2>(key, idx, le, le / esth_tile<LOGM>::E);
+/* the network over one sub-filter held by LANES lanes, E positions each
+ * (lane-major: lane l holds positions l E .. l E + E - 1); the same
+ * compare-exchanges on the same positions whatever LANES is */
+template <int LOGM, int LANES = esth_tile<LOGM>::LPS>
+__device__ __forceinline__ void esth_bitonic(float (&key)[(esth_tile<LOGM, LANES>::E)],
+                                             int (&idx)[(esth_tile<LOGM, LANES>::E)], int le) {
+    esth_bitonic_sizes<LOGM, 2, LANES>(key, idx, le, le / esth_tile<LOGM, LANES>::E);
 }
 
 /* ============================================================== argmax
@@ -736,16 +753,16 @@ __device__ __forceinline__ void esth_store_plane_strided(float* __restrict__ st,
     for (int r = 0; r < E; ++r) st[esth_state_index(layout, S, N, c, i0 + (size_t)r * stride)] = v[r];
 }
 
-template <int LOGM, int S, bool HOIST = false, bool FROM_LDS = false>
+template <int LOGM, int S, bool HOIST = false, bool FROM_LDS = false, int LANES = esth_tile<LOGM>::LPS>
 __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float* __restrict__ st_in,
                                               float* __restrict__ st_out, uint32_t N, uint32_t G, uint32_t g,
                                               bool valid, float2* __restrict__ winner, int* __restrict__ perm,
                                               uint32_t t, float* __restrict__ bt_w, float* __restrict__ bt_s,
                                               float* __restrict__ part, int log_w, float* stage_w,
                                               int layout = ESTH_LAYOUT_SOA, const float* src = nullptr) {
-    using TL = esth_tile<LOGM>;
+    using TL = esth_tile<LOGM, LANES>;
     constexpr int M = TL::M, E = TL::E, LPS = TL::LPS;
-    static_assert(!FROM_LDS || LPS == ESTH_WAVE, "an LDS-resident sub-filter fills one wave");
+    static_assert(!FROM_LDS || M >= ESTH_WAVE, "an LDS-resident sub-filter of at least one wave's positions");
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
     const int le = (lane % LPS) * E;
     float* stage = FROM_LDS ? nullptr : stage_w + (lane - lane % LPS) * E;   /* this sub-filter's M floats */
@@ -770,17 +787,25 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
     }
 #pragma unroll
     for (int r = 0; r < E; ++r) idx[r] = le + r;
-    esth_bitonic<LOGM>(key, idx, le);
+    esth_bitonic<LOGM, LANES>(key, idx, le);
     /* block max M_g = key of position 0 */
     const int lead = lane - (lane % LPS);
     const float mx = esth_shfl_idx(key[0], lead);
+    /* the partial sums follow the one-wave tiling's order whatever LANES is:
+     * each run of ED positions (that tiling's positions per lane) is chained
+     * in a lane, the runs then join pairwise as the lanes' butterfly does */
+    constexpr int ED = esth_tile<LOGM>::E, K = E / ED;
+    static_assert(E % ED == 0, "a narrower tiling holds whole runs of the one-wave tiling");
     float wt[E];
-    float sw = 0.0f;
+    float swg[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) swg[q] = 0.0f;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         wt[r] = log_w ? esth_expf(key[r] - esth_max_shift(mx)) : key[r];
-        sw = sw + wt[r];
+        swg[r / ED] = swg[r / ED] + wt[r];
     }
+    float sw = esth_tree_sum<K>(swg);
     float sx[S];
     if (valid) {
 #pragma unroll
@@ -796,7 +821,9 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
 #pragma unroll
     for (int c = 0; c < S; ++c) {
         float* dst = st_out + (size_t)c * N + base;
-        float acc = 0.0f;
+        float accg[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) accg[q] = 0.0f;
         float v[E];
         if constexpr (FROM_LDS) {
 #pragma unroll
@@ -809,7 +836,8 @@ __device__ __forceinline__ void esth_sort_sub(float* __restrict__ w, const float
             esth_permute_lds<M, E>(in[c % PB], le, stage, idx, v);
         }
 #pragma unroll
-        for (int r = 0; r < E; ++r) acc = esth_fma(wt[r], v[r], acc);
+        for (int r = 0; r < E; ++r) accg[r / ED] = esth_fma(wt[r], v[r], accg[r / ED]);
+        const float acc = esth_tree_sum<K>(accg);
         if (valid) esth_store_run<E>(dst + le, v);
         if (valid && t > 0) {
 #pragma unroll
@@ -2092,8 +2120,8 @@ __attribute__((amdgpu_waves_per_eu(esth_kbss_waves_per_eu<MODEL>()))) void kb_sa
  * sub-filters g0 and g0 + 1 (unit = (filter, g0 | single << 31); a single
  * unit has sub-filter g0 only): a wave samples and weights the unit's
  * particles (lanes 0-31 the first sub-filter's 32 pairs, lanes 32-63 the
- * second's) into its own LDS, then sorts the sub-filters one after the other
- * (esth_sort_sub, FROM_LDS).  A unit never spans two filters, so the model
+ * second's) into its own LDS, then sorts both at once from there, 32 lanes
+ * and two positions per lane each (esth_sort_sub, FROM_LDS).  A unit never spans two filters, so the model
  * hook's frame reads stay wave-uniform.  Per-particle arithmetic and the sort
  * are those of kb_sample_weight + kb_sort. */
 #define ESTH_UNIT_SINGLE 0x80000000u
@@ -2147,11 +2175,10 @@ __attribute__((amdgpu_waves_per_eu(esth_kbss_waves_per_eu<MODEL>()))) void kb_sa
     }
     __syncthreads();
     if (!valid) return;
-#pragma unroll 1
-    for (int j = 0; j < nsubs; ++j)
-        esth_sort_sub<6, S, false, true>(f.w, nullptr, f.tmp, f.N, f.G, g0 + (uint32_t)j, true, (float2*)f.winner,
-                                         nullptr, f.t, f.bt_w, f.bt_s, f.part, log_w, nullptr, ESTH_LAYOUT_SOA,
-                                         L + j * SUB);
+    /* both sub-filters at once, 32 lanes and two positions per lane each */
+    esth_sort_sub<6, S, false, true, 32>(f.w, nullptr, f.tmp, f.N, f.G, g0 + (uint32_t)half, half < nsubs,
+                                         (float2*)f.winner, nullptr, f.t, f.bt_w, f.bt_s, f.part, log_w, nullptr,
+                                         ESTH_LAYOUT_SOA, L + half * SUB);
 }
 
 template <int S>
