@@ -248,6 +248,7 @@ _SIGS = {
     "esth_stage_boxmuller": ([_P, C.c_int], C.c_int),
     "esth_stage_sample_weight": ([C.c_int, _P, C.c_uint32, _P, _P, C.c_int, _P], C.c_int),
     "esth_stage_sort": ([_P, _P, _P, C.c_int, C.c_uint32, C.c_uint32, _P, _P], C.c_int),
+    "esth_filter_set_cross_step": ([_P, C.c_int, _P], C.c_int),
     "esth_filter_rng_mode": ([_P, _P, _P], C.c_int),
     "esth_stage_sort_noalloc": ([_P, _P, _P, C.c_int, C.c_uint32, C.c_uint32, _P, _P, _P], C.c_int),
     "esth_stage_argmax": ([_P, C.c_uint32, C.c_int, _P], C.c_int),
@@ -455,6 +456,13 @@ class Filter:
 
     def set_prefetch(self, on=True):
         check(lib().esth_filter_set_prefetch(self.h, 1 if on else 0))
+
+    def set_cross_step(self, on=True):
+        """One launch per step across the step boundary where it applies;
+        returns whether it is taken."""
+        a = C.c_int()
+        check(lib().esth_filter_set_cross_step(self.h, 1 if on else 0, C.byref(a)))
+        return bool(a.value)
 
     def rng_mode(self):
         """(inline, block_steps): whether the next step's MTGP32 words come

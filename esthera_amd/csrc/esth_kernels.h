@@ -1309,12 +1309,12 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
  * M floats of LDS; stage: S M floats (the gather's planes); xl: 4 floats
  * (the waves' maxima and totals).  Both waves (and every wave of the
  * work-group) must call it: three work-group barriers inside. */
-template <int LOGM, int S>
+template <int LOGM, int S, bool TO_LDS = false>
 __device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, const float* __restrict__ st_tmp,
                                                    const float* w, const float* __restrict__ rnd, uint32_t N,
                                                    uint32_t g, bool valid, float thr, int systematic, int log_w,
                                                    float* cdf, float* stage, float* xl, int h, esth_xch xch,
-                                                   int carry, float* w_mut, int layout) {
+                                                   int carry, float* w_mut, int layout, float* res = nullptr) {
     constexpr int M = 1 << LOGM, E = M / 128, LOGE = LOGM - 7;
     static_assert(LOGM >= 8, "two waves of at least two particles per lane");
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
@@ -1431,7 +1431,17 @@ __device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, 
         }
         pick[r] = mn & (M - 1);
     }
-    if (go) {
+    if constexpr (TO_LDS) {
+        /* the sub-filter's next particles into res (S planes of M): the
+         * selections, or, for a sub-filter the gate skips, the propagated
+         * state the previous launch kept for it in st_main */
+#pragma unroll
+        for (int c = 0; c < S; ++c)
+#pragma unroll
+            for (int r = 0; r < E; ++r)
+                res[c * M + ob + ESTH_WAVE * r] =
+                    go ? stage[c * M + pick[r]] : st_main[(size_t)c * N + base + ob + ESTH_WAVE * r];
+    } else if (go) {
 #pragma unroll
         for (int c = 0; c < S; ++c) {
             float v[E];
@@ -1479,6 +1489,91 @@ __global__ __launch_bounds__(256) void k_resample2_argmax(float* __restrict__ st
     if (!valid) g = G - 1;
     esth_resample_sub2<LOGM, S>(st_main, st_tmp, w, rnd, N, g, valid, thr, systematic, log_w, cdf_s[j], stage_s[j],
                                 xl_s[j], wv & 1, xch, carry, const_cast<float*>(w), layout);
+}
+
+/* One launch per step across the step boundary: work-group b resamples
+ * sub-filter b of step k (waves 0 and 1, esth_resample_sub2 with the
+ * particles kept in LDS), then samples, weights and sorts it for step k + 1
+ * (four waves, as k_sample_sort) from there; work-group mi.n runs step k's
+ * argmax and posterior, the first mi.n work-groups step k + 2's MTGP32 words.
+ * Between the steps a resampled particle never leaves LDS.  Step k's
+ * exchange reads the senders' best t of step k while this launch writes
+ * step k + 1's into the other buffer of a pair (bt_*_k / bt_*_k1, and the
+ * winners and partials the same way); w and tmp of sub-filter b are read and
+ * then rewritten by its own work-group only.  For the scan resamplers on a
+ * ring or torus with t > 0, SOA state and no carried weights. */
+template <int MODEL, int LOGM, bool FA>
+__global__ __launch_bounds__(256) void k_step_x(
+    float* tmp, float* w, const float* __restrict__ rnd_k, float thr, int systematic, const esth_xch xk,
+    const float2* __restrict__ winner_k, const float* __restrict__ part_k, const float* __restrict__ best_k,
+    float2* __restrict__ result, double* __restrict__ est, float* __restrict__ state, uint32_t N, uint32_t G,
+    const float* __restrict__ R_k1, const float* __restrict__ frame_ptr, const esth_frame_arg fa, int log_w,
+    float2* __restrict__ winner_k1, float* __restrict__ bt_w_k1, float* __restrict__ bt_s_k1,
+    float* __restrict__ part_k1, const float* __restrict__ gate_k1, float keep_thr, const esth_mtgp_inline mi) {
+    using T = esth_model_traits<MODEL>;
+    constexpr int M = 1 << LOGM, S = T::S;
+    static_assert(LOGM >= 9 && LOGM <= 10, "two resampling waves and four sampling waves per sub-filter");
+    /* res: S planes of M; then the resampling's cdf (M), staged planes (S M)
+     * and xl (4), whose space the sampling's weights and planes ((1 + S) M)
+     * take over */
+    __shared__ __align__(16) float lds[S * M + (1 + S) * M + 4];
+    __shared__ float2 sc[128];
+    __shared__ float smax[4];
+    __shared__ double sd[4 * (2 + S)];
+    float* res = lds;
+    float* cdf = lds + S * M;
+    float* stage = cdf + M;
+    float* xl = stage + S * M;
+    float* sub = lds + S * M;
+    if (mi.n && esth_mtgp_inline_wg(mi, reinterpret_cast<uint32_t*>(lds))) return;
+    const uint32_t b0 = (uint32_t)mi.n;
+    if (blockIdx.x == b0) {
+        esth_argmax_block<S, 256>(winner_k, G, log_w, part_k, best_k, (size_t)G * xk.t, xk.t, result, est, sc, smax,
+                                  sd);
+        return;
+    }
+    const uint32_t g = blockIdx.x - b0 - 1u;
+    if (g >= G) return;
+    const int wv = threadIdx.x / ESTH_WAVE;
+    if (wv < 2) {
+        esth_resample_sub2<LOGM, S, true>(state, tmp, w, rnd_k, N, g, true, thr, systematic, log_w, cdf, stage, xl, wv,
+                                          xk, 0, w, ESTH_LAYOUT_SOA, res);
+    } else {   /* the resampling's three work-group barriers */
+        __syncthreads();
+        __syncthreads();
+        __syncthreads();
+    }
+    __syncthreads();   /* res complete; cdf and stage free */
+    const float* __restrict__ frame = FA ? fa.v : frame_ptr;
+    const bool keep = (gate_k1[g] - 1.0f) > keep_thr;
+#pragma unroll 1
+    for (int q = 2 * (int)threadIdx.x; q < M; q += 2 * 256) {
+        const uint32_t p = g * (uint32_t)M + (uint32_t)q;
+        float s0[S], s1[S], u0[T::V], u1[T::V];
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+            const float2 v = *reinterpret_cast<const float2*>(res + c * M + q);
+            s0[c] = v.x;
+            s1[c] = v.y;
+        }
+#pragma unroll
+        for (int k = 0; k < T::V; ++k) {
+            const float2 u = *reinterpret_cast<const float2*>(R_k1 + (size_t)k * N + p);
+            u0[k] = u.x;
+            u1[k] = u.y;
+        }
+        float lw0, lw1;
+        esth_sample_pair_regs<MODEL>(s0, s1, u0, u1, frame, lw0, lw1);
+        if (keep) esth_store_pair<S, ESTH_LAYOUT_SOA>(state, N, p, s0, s1);
+        const float2 wo = log_w ? make_float2(lw0, lw1) : make_float2(esth_expf(lw0), esth_expf(lw1));
+        *reinterpret_cast<float2*>(sub + q) = wo;
+#pragma unroll
+        for (int c = 0; c < S; ++c) *reinterpret_cast<float2*>(sub + (1 + c) * M + q) = make_float2(s0[c], s1[c]);
+    }
+    __syncthreads();
+    if (threadIdx.x >= ESTH_WAVE) return;
+    esth_sort_sub<LOGM, S, false, true>(w, nullptr, tmp, N, G, g, true, winner_k1, nullptr, xk.t, bt_w_k1, bt_s_k1,
+                                        part_k1, log_w, nullptr, ESTH_LAYOUT_SOA, sub);
 }
 
 /* =========================================================== Vose alias

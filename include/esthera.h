@@ -330,6 +330,19 @@ int esth_filter_set_fusion(esth_filter* f, int enable, int* active);
  * no counterpart in the reference, which generates them before each step,
  * pf_generic.c:191-232). */
 int esth_filter_rng_mode(esth_filter* f, int* inline_words, int* block_steps);
+/* One launch per step across the step boundary: step k's resampling, argmax
+ * and posterior run at the start of step k + 1's launch (each work-group
+ * resamples its sub-filter into LDS and samples and sorts it from there), so
+ * resampled particles never go through HBM between the steps.  Taken by
+ * default where it applies (the unicycle at m = 512 with its next step's
+ * random words generated inline, a scan resampler on a ring or torus with
+ * t > 0, SOA state, no carried weights, eager steps without per-stage
+ * timing); reading the particles or the estimate, a snapshot, graph replay
+ * or a change of settings first runs the pending resampling.  Results are
+ * the same bit for bit either way.  enable 0 switches it off; *active
+ * (optional) reports whether it is taken.  No counterpart in the reference,
+ * whose step ends with the resampling (pf_generic.c:732-757). */
+int esth_filter_set_cross_step(esth_filter* f, int enable, int* active);
 /* Capture one step into a HIP graph and replay it for later steps with the
  * same control/measurement layout (the frame buffer is updated per call). */
 int esth_filter_use_graph(esth_filter* f, int enable);
