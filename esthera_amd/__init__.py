@@ -220,6 +220,7 @@ _SIGS = {
     "esth_autotune": ([_P, _P, _P, C.c_size_t, _P, _P, C.c_float, _P], C.c_int),
     "esth_autotune_pick": ([_P, C.c_uint32, C.c_uint32, _P], C.c_int),
     "esth_autotune_layout": ([_P, _P, _P, C.c_size_t, _P, _P, C.c_float, _P, _P], C.c_int),
+    "esth_engine_time_hook": ([_P, C.c_int, _P, C.c_uint64, C.c_int, _P], C.c_int),
     "esth_state_convert": ([C.c_int, C.c_size_t, C.c_int, _P, C.c_int, _P], C.c_int),
     "esth_server_create": ([_P, C.c_int, C.c_size_t, _P], C.c_int),
     "esth_server_configure": ([_P, C.c_int, C.c_int], C.c_int),
@@ -687,6 +688,18 @@ def autotune_layout(engine, desc, params, control, measurement, dt):
     check(lib().esth_autotune_layout(engine.h, ptr(desc), ptr(params), psize, ptr(control), ptr(measurement), dt,
                                      ptr(us), C.byref(best)))
     return us, best.value
+
+
+def time_hook(engine, model, frame, n, reps=20):
+    """Device time (us) of the model's sample+weight arithmetic alone over n
+    particles (no memory access but one store per pair): the floor of the
+    sample+weight launches' arithmetic on this device."""
+    fr = np.ascontiguousarray(frame, dtype=np.float32)
+    if fr.size != FRAME_FLOATS:
+        raise ValueError(f"frame must hold {FRAME_FLOATS} floats")
+    us = C.c_float()
+    check(lib().esth_engine_time_hook(engine.h, model, ptr(fr), n, reps, C.byref(us)))
+    return us.value
 
 
 def state_convert(state, state_dim, n, src_layout, dst_layout):

@@ -950,6 +950,56 @@ extern "C" int esth_autotune_layout(esth_engine* e, const esth_filter_desc* base
     return ESTH_OK;
 }
 
+/* The sample+weight arithmetic alone (k_hook_floor) over n particles, the host
+ * frame a kernel argument: one untimed launch, then the median device time of
+ * `reps` launches, each bracketed by events on a stream of its own. */
+extern "C" int esth_engine_time_hook(esth_engine* e, int model, const float* frame, uint64_t n, int reps,
+                                     float* us) {
+    if (!e || !frame || !us) return esth_fail(ESTH_ERR_INVALID, "null");
+    if (n < 2 || n % 2 || n > (1ull << 32) || reps < 1 || reps > 1000)
+        return esth_fail(ESTH_ERR_INVALID, "n must be even in [2, 2^32], reps in [1, 1000]");
+    ESTH_TRY(esth_with_model(model, [](auto) { return ESTH_OK; }));
+    ESTH_HIP(hipSetDevice(e->device));
+    const uint32_t npairs = (uint32_t)(n / 2);
+    const size_t bytes = (size_t)npairs * sizeof(float);
+    float* out = nullptr;
+    ESTH_TRY(esth_engine_alloc(e, bytes, (void**)&out));
+    esth_frame_arg fa;
+    memcpy(fa.v, frame, sizeof(fa.v));
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<float> t;
+    auto run = [&]() -> int {
+        ESTH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        ESTH_HIP(hipEventCreate(&e0));
+        ESTH_HIP(hipEventCreate(&e1));
+        const int grid = (int)((npairs + 255u) / 256u);
+        for (int i = 0; i <= reps; ++i) {
+            ESTH_HIP(hipEventRecord(e0, s));
+            ESTH_TRY(esth_with_model(model, [&](auto id) {
+                hipLaunchKernelGGL((k_hook_floor<decltype(id)::value>), dim3(grid), dim3(256), 0, s, out, npairs, fa);
+                return ESTH_OK;
+            }));
+            ESTH_HIP(hipGetLastError());
+            ESTH_HIP(hipEventRecord(e1, s));
+            ESTH_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            ESTH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (i) t.push_back(ms * 1e3f);
+        }
+        return ESTH_OK;
+    };
+    const int rc = run();
+    if (e1) hipEventDestroy(e1);
+    if (e0) hipEventDestroy(e0);
+    if (s) hipStreamDestroy(s);
+    esth_engine_release(e, out, bytes);
+    ESTH_TRY(rc);
+    std::sort(t.begin(), t.end());
+    *us = t[t.size() / 2];
+    return ESTH_OK;
+}
+
 /* the sub-filter size for a filter of `particles` particles: among the powers
  * of two in [max(64, min_m), 1024] that divide it, the one with the lowest
  * measured cost (ties: the larger); if none is >= min_m, the largest divisor. */

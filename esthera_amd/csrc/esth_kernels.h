@@ -392,6 +392,37 @@ __global__ __launch_bounds__(256) void k_sample_weight(float* __restrict__ state
     if (p < N) esth_sample_weight_pair<MODEL, L>(state, N, R, frame, log_w, w, p, carry);
 }
 
+/* Calibration: the sample+weight arithmetic of k_sample_weight (Box-Muller,
+ * propagation, log-weight of a particle pair) with no memory access but one
+ * 4-byte store per pair.  States in [-0.5, 0.5) and uniforms in [1, 2) are
+ * made from the pair index with shifts, xors and adds (about 3 operations
+ * per value, against ~10^3 per pair in the hooks). */
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_hook_floor(float* __restrict__ out, uint32_t npairs,
+                                                    const esth_frame_arg fa) {
+    using T = esth_model_traits<MODEL>;
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npairs) return;
+    uint32_t h = q ^ 0x9E3779B9u;
+    h ^= h << 13;
+    h ^= h >> 17;
+    h ^= h << 5;
+    float s0[T::S], s1[T::S], u0[T::V], u1[T::V];
+#pragma unroll
+    for (int c = 0; c < T::S; ++c) {
+        s0[c] = esth_u2f(0x3f800000u | ((h + 0x1F123u * (uint32_t)c) & 0x7fffffu)) - 1.5f;
+        s1[c] = esth_u2f(0x3f800000u | ((h ^ (0x2E571u * (uint32_t)c + 0x5A5u)) & 0x7fffffu)) - 1.5f;
+    }
+#pragma unroll
+    for (int k = 0; k < T::V; ++k) {
+        u0[k] = esth_u2f(0x3f800000u | ((h + 0x3B9ACu * (uint32_t)k + 0x11u) & 0x7fffffu));
+        u1[k] = esth_u2f(0x3f800000u | ((h ^ (0x4C4B4u * (uint32_t)k + 0x7Fu)) & 0x7fffffu));
+    }
+    float lw0, lw1;
+    esth_sample_pair_regs<MODEL>(s0, s1, u0, u1, fa.v, lw0, lw1);
+    out[q] = lw0 + lw1;
+}
+
 /* ============================================================ block sort
  * The reference's descending bitonic network (bitonic_sort.cl:28-61 with
  * dir = 0): for size = 2..m, stride = size/2..1, pair (pos, pos+stride) with

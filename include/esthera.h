@@ -457,6 +457,16 @@ int esth_autotune_pick(const float us_per_mparticle[5], uint32_t particles, uint
 int esth_autotune_layout(esth_engine* e, const esth_filter_desc* base, const void* params, size_t params_size,
                          const void* control, const void* measurement, float dt, float us_per_step[3],
                          int* best);
+/* Calibration, no counterpart in the reference: the device time in
+ * microseconds of the model's sample+weight arithmetic alone (Box-Muller,
+ * propagation and log-weight of each particle pair, on states and uniforms
+ * made up in registers from the pair index; one 4-byte store per pair and no
+ * other memory access) over n particles (even, at most 2^32), with `frame`
+ * (ESTH_FRAME_FLOATS floats, esth_model_build_frame) as a kernel argument;
+ * the median of `reps` launches after one untimed launch.  This is the time
+ * the sample+weight launches' arithmetic takes with every memory access
+ * hidden: their floor on this device. */
+int esth_engine_time_hook(esth_engine* e, int model, const float* frame, uint64_t n, int reps, float* us);
 
 /* ------------------------------------------------------------ serving
  * A request loop with continuous batching.  Tracks (filters) are opened,

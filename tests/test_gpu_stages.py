@@ -3,6 +3,8 @@ through the C ABI on device buffers.  Integer/byte/index outputs (MTGP words,
 permutations, selections, alias tables, exchanged slots) and, because both
 sides evaluate the same IEEE operation sequence, float outputs are compared
 bit for bit."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -388,3 +390,21 @@ def test_alias_table_pairing_regimes(shape, m, engine):
     op, oa = O.alias_table(lw, m, G, ea.WEIGHTS_LOG)
     assert_bits(prob, op)
     assert_bits(alias, oa)
+
+
+# ------------------------------------------------- arithmetic floor timing
+@pytest.mark.parametrize("model", [ea.MODEL_ROBOTARM5, ea.MODEL_UNICYCLE, ea.MODEL_DH6, ea.MODEL_UNICYCLE10,
+                                   ea.MODEL_ROBOTARM5_CAMERA, ea.MODEL_RANDOMWALK3])
+def test_time_hook(model, engine):
+    """esth_engine_time_hook: a positive device time for every model, longer
+    for 16x the particles; sizes outside its range are rejected."""
+    params, ctrl, meas, _, _, dt, _, _ = _model_case(model, 64, 3)
+    fr = ea.build_frame(model, params, ctrl, meas, dt)
+    small = ea.time_hook(engine, model, fr, 1 << 16, 5)
+    big = ea.time_hook(engine, model, fr, 1 << 20, 5)
+    assert np.isfinite(small) and small > 0.0
+    assert big > small
+    us = C.c_float()
+    for n, reps in [(3, 5), (0, 5), (1 << 16, 0), ((1 << 32) + 2, 5)]:
+        assert ea.lib().esth_engine_time_hook(engine.h, model, ea.ptr(fr), n, reps, C.byref(us)) == ea.ESTH_ERR_INVALID
+    assert ea.lib().esth_engine_time_hook(engine.h, 99, ea.ptr(fr), 1 << 16, 5, C.byref(us)) == ea.ESTH_ERR_INVALID
