@@ -1330,26 +1330,37 @@ __global__ __launch_bounds__(256) void k_resample_argmax(float* __restrict__ st_
                                cdf_s[threadIdx.x / ESTH_WAVE], true, xch, carry, const_cast<float*>(w), layout);
 }
 
-/* The same resampling of one sub-filter by two waves (m >= 256): wave h
- * holds particles h M/2 .. (h + 1) M/2 - 1, E = M/128 per lane, lane-major.
- * Every value is the one-wave version's: the scan tree is the same binary
- * tree over positions (register levels, cross-lane levels, then the wave
- * level, whose left node wave 1 adds first, as bit 8 of its positions asks),
- * the block max is a max, the search and the selections are per output.
- * Twice the waves per launch cover twice the latency.  cdf: the sub-filter's
- * M floats of LDS; stage: S M floats (the gather's planes); xl: 4 floats
- * (the waves' maxima and totals).  Both waves (and every wave of the
- * work-group) must call it: three work-group barriers inside. */
-template <int LOGM, int S, bool TO_LDS = false>
+/* the sum of the wave-level nodes n[a .. a + K), as the position tree forms
+ * it: (n[a] + n[a+1]) + (n[a+2] + n[a+3]) for K = 4 */
+template <int K>
+__device__ __forceinline__ float esth_wave_node(const float* n, int a) {
+    if constexpr (K == 1) return n[a];
+    else return esth_wave_node<K / 2>(n, a) + esth_wave_node<K / 2>(n, a + K / 2);
+}
+
+/* The same resampling of one sub-filter by W waves (W = 2: m >= 256; W = 4:
+ * m >= 512): wave h holds particles h M/W .. (h + 1) M/W - 1, E = M/(64 W)
+ * per lane, lane-major.  Every value is the one-wave version's: the scan
+ * tree is the same binary tree over positions (register levels, cross-lane
+ * levels, then the wave levels, whose left nodes a wave adds top-down as the
+ * high bits of its positions ask), the block max is a max, the search and the
+ * selections are per output.  W times the waves per sub-filter cover W
+ * times the latency.  cdf: the sub-filter's M floats of LDS; stage: S M
+ * floats (the gather's planes); xl: 2 W floats (the waves' maxima and
+ * totals).  All W waves (and every wave of the work-group) must call it:
+ * three work-group barriers inside. */
+template <int LOGM, int S, bool TO_LDS = false, int W = 2>
 __device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, const float* __restrict__ st_tmp,
                                                    const float* w, const float* __restrict__ rnd, uint32_t N,
                                                    uint32_t g, bool valid, float thr, int systematic, int log_w,
                                                    float* cdf, float* stage, float* xl, int h, esth_xch xch,
                                                    int carry, float* w_mut, int layout, float* res = nullptr) {
-    constexpr int M = 1 << LOGM, E = M / 128, LOGE = LOGM - 7;
-    static_assert(LOGM >= 8, "two waves of at least two particles per lane");
+    static_assert(W == 2 || W == 4, "two or four waves per sub-filter");
+    constexpr int LOGW = W == 4 ? 2 : 1;
+    constexpr int M = 1 << LOGM, E = M / (64 * W), LOGE = LOGM - 6 - LOGW;
+    static_assert(LOGE >= 1, "at least two particles per lane");
     const int lane = threadIdx.x & (ESTH_WAVE - 1);
-    const int le = h * (M / 2) + lane * E;   /* first position of this lane */
+    const int le = h * (M / W) + lane * E;   /* first position of this lane */
     const size_t base = (size_t)g * M;
     const bool go = valid && !((rnd[N + g] - 1.0f) > thr);   /* gate, pf_generic.cl:403 */
     int rsrc[E];
@@ -1374,8 +1385,8 @@ __device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, 
     float in[S][E];
 #pragma unroll
     for (int c = 0; c < S; ++c) esth_load_run<E>(st_tmp + (size_t)c * N + base + le, in[c]);
-    /* outputs: particle h M/2 + lane + 64 r */
-    const int ob = h * (M / 2) + lane;
+    /* outputs: particle h M/W + lane + 64 r */
+    const int ob = h * (M / W) + lane;
     float uu[E];
     if (!systematic) {
 #pragma unroll
@@ -1396,8 +1407,9 @@ __device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, 
         mx = esth_bfly_max<1, ESTH_WAVE>(mx);
         if (lane == 0) xl[h] = mx;
         __syncthreads();
-        const float m0 = xl[0], m1 = xl[1];
-        mx = m1 > m0 ? m1 : m0;
+        mx = xl[0];
+#pragma unroll
+        for (int i = 1; i < W; ++i) mx = xl[i] > mx ? xl[i] : mx;
 #pragma unroll
         for (int r = 0; r < E; ++r) x[r] = esth_expf(x[r] - esth_max_shift(mx));
     } else {
@@ -1413,12 +1425,15 @@ __device__ __forceinline__ void esth_resample_sub2(float* __restrict__ st_main, 
     float node = lvl[LOGE][0];
     float lx[6];
     esth_scan_levels<0, 6>(node, lx);
-    if (lane == 0) xl[2 + h] = node;   /* the wave's node: [h M/2, (h + 1) M/2) */
+    if (lane == 0) xl[W + h] = node;   /* the wave's node: [h M/W, (h + 1) M/W) */
     __syncthreads();
-    const float left = xl[2];
-    const float total = left + xl[3];
+    const float* wn = xl + W;
+    const float total = esth_wave_node<W>(wn, 0);
     float acc = 0.0f;
-    if (h) acc = acc + left;
+    if constexpr (W == 4) {
+        if (h >= 2) acc = acc + esth_wave_node<2>(wn, 0);
+    }
+    if (h & 1) acc = acc + wn[h - 1];
 #pragma unroll
     for (int k = 5; k >= 0; --k)
         if ((lane >> k) & 1) acc = acc + lx[k];
@@ -1523,8 +1538,8 @@ __global__ __launch_bounds__(256) void k_resample2_argmax(float* __restrict__ st
 }
 
 /* One launch per step across the step boundary: work-group b resamples
- * sub-filter b of step k (waves 0 and 1, esth_resample_sub2 with the
- * particles kept in LDS), then samples, weights and sorts it for step k + 1
+ * sub-filter b of step k (all four waves, esth_resample_sub2 with W = 4 and
+ * the particles kept in LDS), then samples, weights and sorts it for step k + 1
  * (four waves, as k_sample_sort) from there; work-group mi.n runs step k's
  * argmax and posterior, the first mi.n work-groups step k + 2's MTGP32 words.
  * Between the steps a resampled particle never leaves LDS.  Step k's
@@ -1534,7 +1549,7 @@ __global__ __launch_bounds__(256) void k_resample2_argmax(float* __restrict__ st
  * then rewritten by its own work-group only.  For the scan resamplers on a
  * ring or torus with t > 0, SOA state and no carried weights. */
 template <int MODEL, int LOGM, bool FA>
-__global__ __launch_bounds__(256) void k_step_x(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_step_x(
     float* tmp, float* w, const float* __restrict__ rnd_k, float thr, int systematic, const esth_xch xk,
     const float2* __restrict__ winner_k, const float* __restrict__ part_k, const float* __restrict__ best_k,
     float2* __restrict__ result, double* __restrict__ est, float* __restrict__ state, uint32_t N, uint32_t G,
@@ -1543,11 +1558,11 @@ __global__ __launch_bounds__(256) void k_step_x(
     float* __restrict__ part_k1, const float* __restrict__ gate_k1, float keep_thr, const esth_mtgp_inline mi) {
     using T = esth_model_traits<MODEL>;
     constexpr int M = 1 << LOGM, S = T::S;
-    static_assert(LOGM >= 9 && LOGM <= 10, "two resampling waves and four sampling waves per sub-filter");
+    static_assert(LOGM >= 9 && LOGM <= 10, "four resampling and sampling waves per sub-filter");
     /* res: S planes of M; then the resampling's cdf (M), staged planes (S M)
-     * and xl (4), whose space the sampling's weights and planes ((1 + S) M)
+     * and xl (8), whose space the sampling's weights and planes ((1 + S) M)
      * take over */
-    __shared__ __align__(16) float lds[S * M + (1 + S) * M + 4];
+    __shared__ __align__(16) float lds[S * M + (1 + S) * M + 8];
     __shared__ float2 sc[128];
     __shared__ float smax[4];
     __shared__ double sd[4 * (2 + S)];
@@ -1566,14 +1581,8 @@ __global__ __launch_bounds__(256) void k_step_x(
     const uint32_t g = blockIdx.x - b0 - 1u;
     if (g >= G) return;
     const int wv = threadIdx.x / ESTH_WAVE;
-    if (wv < 2) {
-        esth_resample_sub2<LOGM, S, true>(state, tmp, w, rnd_k, N, g, true, thr, systematic, log_w, cdf, stage, xl, wv,
-                                          xk, 0, w, ESTH_LAYOUT_SOA, res);
-    } else {   /* the resampling's three work-group barriers */
-        __syncthreads();
-        __syncthreads();
-        __syncthreads();
-    }
+    esth_resample_sub2<LOGM, S, true, 4>(state, tmp, w, rnd_k, N, g, true, thr, systematic, log_w, cdf, stage, xl, wv,
+                                         xk, 0, w, ESTH_LAYOUT_SOA, res);
     __syncthreads();   /* res complete; cdf and stage free */
     const float* __restrict__ frame = FA ? fa.v : frame_ptr;
     const bool keep = (gate_k1[g] - 1.0f) > keep_thr;
