@@ -474,7 +474,8 @@ static bool sample_sort_pays_l(int lm) {
     const int ss_waves[11] = {0, 0, 0, 0, 0, 0, 0, esth_ss_waves<7>(), esth_ss_waves<8>(), esth_ss_waves<9>(),
                               esth_ss_waves<10>()};
     const int fused = esth_waves_per_simd(fn, ESTH_WAVE * ss_waves[lm]);
-    return fused > 0 && fused >= esth_waves_per_simd((const void*)&k_sample_weight<MODEL, L, true>, 256);
+    return fused > 0 &&
+           (fused >= 3 || fused >= esth_waves_per_simd((const void*)&k_sample_weight<MODEL, L, true>, 256));
 }
 
 template <int MODEL>
@@ -484,9 +485,15 @@ static bool sample_sort_pays_t(int layout, int lm) {
     return sample_sort_pays_l<MODEL, ESTH_LAYOUT_SOA>(lm);
 }
 
-/* the fused launch is taken where it keeps as many waves resident as the
- * sampling alone: the sort's registers (E = m/64 keys, indices and weights
- * per lane) would otherwise cost the VALU-bound sampling its occupancy */
+/* the fused launch is taken where it keeps 3 waves per SIMD resident, or as
+ * many as the sampling alone.  Measured (whole steps, fused against two
+ * launches): with 3-4 waves it wins even where the sampling alone keeps 5-8
+ * (dh6 at m = 512 -11 %, robotarm5 and camera at 512 -8 %, unicycle at 1024
+ * -9 %, randomwalk3 at 1024 -18 %: the state's round trip through HBM
+ * between the two launches outweighs the occupancy), with 2 it loses or
+ * ties (dh6 at 1024 +13 %, robotarm5 at 1024 even): the sort's registers
+ * (E = m/64 keys, indices and gathered planes per lane) then leave too few
+ * waves to hide the sampling's memory and arithmetic latency */
 bool esth_sample_sort_pays(int model, int layout, uint32_t m) {
     if (!esth_sample_sort_fits(m)) return false;
     const int lm = log2u(m);

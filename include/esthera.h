@@ -318,8 +318,8 @@ int esth_filter_set_prefetch(esth_filter* f, int enable);
 /* Sample+weight and the block sort as one launch per sub-filter (the
  * propagated state stays in LDS between them; no counterpart in the
  * reference, whose two kernels are pf_generic.c:521-533 and :572-585).  On by
- * default for 128 <= m <= 1024 where the fused kernel keeps as many waves
- * resident as the sampling alone; enable = 0 keeps two launches.  *active
+ * default for 128 <= m <= 1024 where the fused kernel keeps 3 waves per SIMD
+ * resident, or as many as the sampling alone; enable = 0 keeps two launches.  *active
  * (may be NULL) receives whether steps use the fused launch.  Results are
  * bit-identical either way.  Timing level 2 always runs the two launches. */
 int esth_filter_set_fusion(esth_filter* f, int enable, int* active);

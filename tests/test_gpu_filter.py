@@ -332,6 +332,9 @@ FUSION_CASES = [
     ("robotarm-512-soa-log", ea.MODEL_ROBOTARM5, 512, ea.LAYOUT_SOA, dict(G=8, t=2, weights=ea.WEIGHTS_LOG)),
     ("dh6-256-soa-torus", ea.MODEL_DH6, 256, ea.LAYOUT_SOA, dict(G=8, t=1, topology=ea.TOPO_TORUS,
                                                                  weights=ea.WEIGHTS_LOG)),
+    ("dh6-512-soa-torus", ea.MODEL_DH6, 512, ea.LAYOUT_SOA, dict(G=8, t=1, topology=ea.TOPO_TORUS,
+                                                                 weights=ea.WEIGHTS_LOG)),
+    ("dh6-1024-soa-ring", ea.MODEL_DH6, 1024, ea.LAYOUT_SOA, dict(G=4, t=1, weights=ea.WEIGHTS_LOG)),
     ("unicycle10-256-soa-all2all", ea.MODEL_UNICYCLE10, 256, ea.LAYOUT_SOA,
      dict(G=16, t=2, topology=ea.TOPO_ALL2ALL, weights=ea.WEIGHTS_LOG)),
     ("camera-256-aos-log", ea.MODEL_ROBOTARM5_CAMERA, 256, ea.LAYOUT_AOS, dict(G=8, t=1, weights=ea.WEIGHTS_LOG)),
@@ -349,8 +352,8 @@ def test_fused_sample_sort_identical(name, model, m, layout, kw, engine):
     a, b = ea.Filter(engine, desc, params), ea.Filter(engine, desc, params)
     fused = a.set_fusion(True)
     assert b.set_fusion(False) is False
-    if model == ea.MODEL_UNICYCLE and m <= 512:
-        assert fused, "the fused launch is expected for the unicycle at m <= 512"
+    if model in (ea.MODEL_UNICYCLE, ea.MODEL_DH6, ea.MODEL_ROBOTARM5) and m <= 512:
+        assert fused, "the fused launch is expected at m <= 512 for these models"
     print(name, "fused" if fused else "not fused (occupancy)", flush=True)
     for f in (a, b):
         f.init_uniform(init)
