@@ -530,19 +530,25 @@ int esth_launch_sample_sort(int model, uint32_t m, float* state, uint32_t N, uin
 int esth_launch_step_x(int model, uint32_t m, float* tmp, float* w, const float* rnd_k, float thr, int systematic,
                        const float* bt_w_k, const float* bt_s_k, const float2* winner_k, const float* part_k,
                        float2* result, double* est, float* state, uint32_t N, uint32_t G, uint32_t t, int topo,
-                       int dimx, int dimy, const float* R_k1, const float* host_frame, int log_w, float2* winner_k1,
-                       float* bt_w_k1, float* bt_s_k1, float* part_k1, const float* gate_k1, const esth_mtgp_inline* mtgp,
-                       hipStream_t s) {
-    if (model != ESTH_MODEL_UNICYCLE || m != 512 || (uint64_t)m * G != N || !host_frame || t == 0 ||
+                       int dimx, int dimy, const float* R_k1, const float* frame, const float* host_frame, int log_w,
+                       float2* winner_k1, float* bt_w_k1, float* bt_s_k1, float* part_k1, const float* gate_k1,
+                       const esth_mtgp_inline* mtgp, hipStream_t s) {
+    if (model != ESTH_MODEL_UNICYCLE || m != 512 || (uint64_t)m * G != N || (!host_frame && !frame) || t == 0 ||
         (topo != ESTH_TOPO_RING && topo != ESTH_TOPO_TORUS))
         return esth_fail(ESTH_ERR_UNSUPPORTED, "cross-step launch: unicycle, m = 512, ring or torus with t > 0");
     const esth_mtgp_inline mi = mtgp ? *mtgp : esth_mtgp_inline{};
     esth_frame_arg fa;
-    memcpy(fa.v, host_frame, sizeof(fa.v));
+    if (host_frame) memcpy(fa.v, host_frame, sizeof(fa.v));
+    else memset(fa.v, 0, sizeof(fa.v));
     const esth_xch xk{bt_w_k, bt_s_k, nullptr, w, G, t, topo == ESTH_TOPO_TORUS ? 4u : 1u, topo, dimx, dimy};
-    hipLaunchKernelGGL((k_step_x<ESTH_MODEL_UNICYCLE, 9, true>), dim3(mi.n + 1 + G), dim3(256), 0, s, tmp, w, rnd_k, thr,
-                       systematic, xk, winner_k, part_k, bt_s_k, result, est, state, N, G, R_k1, nullptr, fa, log_w,
-                       winner_k1, bt_w_k1, bt_s_k1, part_k1, gate_k1, thr, mi);
+    if (host_frame)
+        hipLaunchKernelGGL((k_step_x<ESTH_MODEL_UNICYCLE, 9, true>), dim3(mi.n + 1 + G), dim3(256), 0, s, tmp, w, rnd_k,
+                           thr, systematic, xk, winner_k, part_k, bt_s_k, result, est, state, N, G, R_k1, nullptr, fa,
+                           log_w, winner_k1, bt_w_k1, bt_s_k1, part_k1, gate_k1, thr, mi);
+    else
+        hipLaunchKernelGGL((k_step_x<ESTH_MODEL_UNICYCLE, 9, false>), dim3(mi.n + 1 + G), dim3(256), 0, s, tmp, w, rnd_k,
+                           thr, systematic, xk, winner_k, part_k, bt_s_k, result, est, state, N, G, R_k1, frame, fa,
+                           log_w, winner_k1, bt_w_k1, bt_s_k1, part_k1, gate_k1, thr, mi);
     ESTH_HIP(hipGetLastError());
     return ESTH_OK;
 }

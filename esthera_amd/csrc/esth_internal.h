@@ -68,11 +68,14 @@ int esth_launch_sample_weight(int model, float* state, uint32_t N, const float* 
 /* one launch across the step boundary (k_step_x): step k's resampling, argmax
  * and posterior from its sorted buffers (*_k), then step k + 1's
  * sample+weight and sort into the other buffers (*_k1); the unicycle at
- * m = 512 on a ring or torus with t > 0, SOA state, no carried weights */
+ * m = 512 on a ring or torus with t > 0, SOA state, no carried weights.
+ * host_frame (non-null): the frame as a kernel argument; otherwise the
+ * kernel reads it from `frame` in device memory (graph replay) */
 int esth_launch_step_x(int model, uint32_t m, float* tmp, float* w, const float* rnd_k, float thr, int systematic,
                        const float* bt_w_k, const float* bt_s_k, const float2* winner_k, const float* part_k,
                        float2* result, double* est, float* state, uint32_t N, uint32_t G, uint32_t t, int topo,
-                       int dimx, int dimy, const float* R_k1, const float* host_frame, int log_w, float2* winner_k1,
+                       int dimx, int dimy, const float* R_k1, const float* frame, const float* host_frame, int log_w,
+                       float2* winner_k1,
                        float* bt_w_k1, float* bt_s_k1, float* part_k1, const float* gate_k1, const esth_mtgp_inline* mtgp,
                        hipStream_t s);
 int esth_launch_sort(int S, uint32_t m, float* w, const float* st_in, float* st_out, uint32_t N,

@@ -336,9 +336,9 @@ int esth_filter_rng_mode(esth_filter* f, int* inline_words, int* block_steps);
  * resampled particles never go through HBM between the steps.  Taken by
  * default where it applies (the unicycle at m = 512 with its next step's
  * random words generated inline, a scan resampler on a ring or torus with
- * t > 0, SOA state, no carried weights, eager steps without per-stage
- * timing); reading the particles or the estimate, a snapshot, graph replay
- * or a change of settings first runs the pending resampling.  Results are
+ * t > 0, SOA state, no carried weights, steps without per-stage timing,
+ * eager or graph-replayed); reading the particles or the estimate, a
+ * snapshot or a change of settings first runs the pending resampling.  Results are
  * the same bit for bit either way.  enable 0 switches it off; *active
  * (optional) reports whether it is taken.  No counterpart in the reference,
  * whose step ends with the resampling (pf_generic.c:732-757). */
